@@ -1,0 +1,50 @@
+# Builds the reference Fortran hot-path modules, unmodified and straight from
+# /root/reference/playground/src, together with this repository's own driver
+# (oracle/ref_harness.f90), into oracle/_ref/<kernel>/ref_harness.
+#
+# Test infrastructure only: the binaries generate golden vectors and time the
+# reference OpenMP path.  Nothing in the product links or loads them.
+#
+# Only the modules on the two-loop path are compiled (the reference Makefile's
+# SRC list minus dumper/printer/IC/args/err*/main, which need HDF5 or are not on
+# the path).  The reference Makefile itself is not run: it hard-codes gfortran
+# and a Homebrew HDF5 path.
+#
+#   make -f oracle/ref.mk KERNEL=m6      (also m4, gauss)
+#
+# Flags: FP64 default reals exactly as the reference Makefile (-fdefault-real-8
+# -fdefault-double-8), OpenMP on, -O2 without fast-math, and FP contraction off
+# so that `sd + d**2` in the neighbour test is rounded as written (the
+# reference's generic x86-64 gfortran build emits no FMA either).
+
+REFSRC  ?= /root/reference/playground/src
+KERNEL  ?= m6
+REF_FC  ?= /opt/rocm/llvm/bin/amdflang
+OUT     := oracle/_ref/$(KERNEL)
+FFLAGS  := -fdefault-real-8 -fdefault-double-8 -O2 -cpp -fopenmp -ffp-contract=off
+
+MODS := utils/errprinter utils/arrayresize utils/timing utils/list utils/kdtree2 \
+        const kernel/base_$(KERNEL) state kernel/kernel BC utils/neighboursearch \
+        eos circuit1 circuit2 iterator timestep sts_integrator
+
+OBJS := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(MODS))))
+
+all: $(OUT)/ref_harness
+
+$(OUT):
+	mkdir -p $(OUT)
+
+# Modules must compile in dependency order (each writes its .mod into $(OUT)).
+$(OUT)/stamp: | $(OUT)
+	set -e; for m in $(MODS); do \
+	  $(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) -c $(REFSRC)/$$m.f90 -o $(OUT)/$$(basename $$m).o; \
+	done
+	touch $@
+
+$(OUT)/ref_harness: $(OUT)/stamp oracle/ref_harness.f90
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) oracle/ref_harness.f90 $(OBJS) -o $@
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean

@@ -1,0 +1,264 @@
+"""Seeded synthetic particle sets in the reference's ``store``/``statevars`` form.
+
+The reference's own placements are unseeded (placeRandom.f90:65) and its
+``closepacked`` lattice is not periodic in a cube (placeClose.f90:54,103), so
+the workloads are generated here with a stated PRNG: splitmix64 for uniform
+doubles and Box-Muller for Gaussians.  Every generator returns
+``(state, store)``: ``state`` is the 49-slot FP64 ``statevars`` vector
+(const.f90:14-66) and ``store`` a C-contiguous ``(n, 32)`` FP64 array whose
+row ``i`` is the reference's column ``store(:, i)`` (const.f90:69-103).
+
+Field defaults follow the reference CLI (args.f90:36-63: hfac 1.2, au 0,
+tfinish 1) and set-up (place.f90:56-67: bordersize = (int(krad*hfac)+1)*dx).
+"""
+import math
+
+import numpy as np
+
+from . import consts as C
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, n):
+    """First ``n`` outputs of splitmix64 started at ``seed`` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * _GOLD
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n):
+    """Uniform doubles in (0, 1): the top 53 bits of splitmix64, centred."""
+    z = splitmix64(seed, n)
+    return ((z >> np.uint64(11)).astype(np.float64) + 0.5) * (2.0 ** -53)
+
+
+def gaussian(seed, n):
+    """Standard normal doubles by Box-Muller on pairs of ``uniform`` draws."""
+    u = uniform(seed, 2 * n)
+    u1, u2 = u[0::2], u[1::2]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def make_state(*, dim, equations="hydro", ddw="fab", adden="yes", artts="yes",
+               au=0.0, disotropic="yes", dcond=1.0, muzero=1.0, hfac=1.2,
+               box=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0), spacing=None,
+               bordersize=None, n_real=0, n_fixed=0, gamma=5.0 / 3.0,
+               process="fullyperiodic", kernel=C.KERNEL_M6, tfinish=1.0,
+               dtprint=1.0, eqon=None, molecularmass=1.0, ststype=C.ests_fixeds,
+               stsfixeds=3, ics=C.e_none):
+    """Build a ``statevars`` vector the way args.f90:29-176 and place.f90 fill it."""
+    s = np.full(C.ec_total, -1.0)
+
+    def put(slot, v):
+        s[slot - 1] = float(v)
+
+    put(C.ec_dim, dim)
+    put(C.ec_eqs, C.EQUATIONS[equations])
+    put(C.ec_ddw, C.DDW[ddw])
+    put(C.ec_ics, ics)
+    put(C.ec_adden, C.YESNO[adden])
+    put(C.ec_artts, C.YESNO[artts])
+    put(C.ec_coordsys, 1)
+    put(C.ec_disotropic, C.YESNO[disotropic])
+    put(C.ec_dcondconst, dcond)
+    put(C.ec_muzero, muzero)
+    put(C.ec_tfinish, tfinish)
+    put(C.ec_npics, 1)
+    put(C.ec_hfac, hfac)
+    put(C.ec_silent, 1)
+    xmin, xmax, ymin, ymax, zmin, zmax = box
+    if dim < 2:
+        ymin = ymax = 0.0
+    if dim < 3:
+        zmin = zmax = 0.0
+    put(C.ec_xmin, xmin)
+    put(C.ec_xmax, xmax)
+    put(C.ec_ymin, ymin)
+    put(C.ec_ymax, ymax)
+    put(C.ec_zmin, zmin)
+    put(C.ec_zmax, zmax)
+    if spacing is not None:
+        put(C.ec_spacing, spacing)
+        put(C.ec_resolution, round((xmax - xmin) / spacing))
+    if bordersize is None and spacing is not None:
+        bordersize = (int(C.KRAD[kernel] * hfac) + 1) * spacing
+    put(C.ec_bordsize, bordersize if bordersize is not None else 0.0)
+    put(C.ec_padding, 0.0)
+    put(C.ec_realpn, n_real)
+    put(C.ec_fixedpn, n_fixed)
+    put(C.ec_gamma, gamma)
+    put(C.ec_lastprint, 0)
+    put(C.ec_usedumps, 0)
+    put(C.ec_dtprint, dtprint)
+    put(C.ec_process, C.PROCESS[process])
+    put(C.ec_au, au)
+    put(C.ec_placement, C.epl_uniform)
+    eqon = eqon or {}
+    put(C.ec_eqonhydro, 1 if eqon.get("hydro") else 0)
+    put(C.ec_eqondiff, 1 if eqon.get("diff") else 0)
+    put(C.ec_eqonfluxlim, 1 if eqon.get("fld") else 0)
+    put(C.ec_eqonradexch, 1 if eqon.get("radexch") else 0)
+    put(C.ec_eqonsts, 1 if eqon.get("sts") else 0)
+    for slot in (C.ec_udist, C.ec_umass, C.ec_utime, C.ec_uvelocity,
+                 C.ec_udensity, C.ec_upressure, C.ec_uacceller):
+        put(slot, 1.0)
+    put(C.ec_time, 0.0)
+    put(C.ec_molecularmass, molecularmass)
+    put(C.ec_ststype, ststype)
+    put(C.ec_stsfixeds, stsfixeds)
+    return s
+
+
+def empty_store(n):
+    st = np.zeros((n, C.es_total), dtype=np.float64)
+    st[:, C.es_type - 1] = C.ept_real
+    st[:, C.es_om - 1] = 1.0
+    return st
+
+
+def _eos_fill(st, gamma):
+    rho = st[:, C.es_den - 1]
+    p = (gamma - 1.0) * rho * st[:, C.es_u - 1]
+    st[:, C.es_p - 1] = p
+    st[:, C.es_c - 1] = np.sqrt(gamma * p / rho)
+
+
+def lattice_positions(dim, n_side, box, kind="square"):
+    """Cell-centred lattice in ``box``; ``kind='staggered'`` (3D) shifts x by
+    dx/2 on every other (y+z) row, a close-packed-like arrangement that tiles a
+    periodic cube for even ``n_side``: x = (i + 1/4 + ((j+k) mod 2)/2) dx,
+    y = (j + 1/2) dx, z = (k + 1/2) dx."""
+    lo = np.array(box[0::2][:dim])
+    hi = np.array(box[1::2][:dim])
+    dx = (hi[0] - lo[0]) / n_side
+    idx = np.indices((n_side,) * dim).reshape(dim, -1)[::-1].T.astype(np.float64)
+    pos = np.zeros((idx.shape[0], 3))
+    if kind == "staggered":
+        assert dim == 3 and n_side % 2 == 0
+        par = (idx[:, 1] + idx[:, 2]) % 2
+        pos[:, 0] = lo[0] + (idx[:, 0] + 0.25 + 0.5 * par) * dx
+        pos[:, 1] = lo[1] + (idx[:, 1] + 0.5) * dx
+        pos[:, 2] = lo[2] + (idx[:, 2] + 0.5) * dx
+    else:
+        for a in range(dim):
+            pos[:, a] = lo[a] + (idx[:, a] + 0.5) * dx
+    return pos, dx
+
+
+def hydro2d(n_side=64, seed=1, jitter=0.01, hfac=1.0, artts="yes", adden="yes",
+            kernel=C.KERNEL_M6, au=0.0):
+    """2D hydro on a jittered square lattice in periodic [-1,1]^2: rho=1, u=1,
+    gamma=5/3, v=0, h=dx, m=4/N (the 64x64 one-leapfrog-step workload)."""
+    box = (-1.0, 1.0, -1.0, 1.0, 0.0, 0.0)
+    pos, dx = lattice_positions(2, n_side, box)
+    n = pos.shape[0]
+    g = gaussian(seed, 2 * n).reshape(n, 2)
+    pos[:, :2] += jitter * dx * g
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = 4.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = dx * hfac
+    st[:, C.es_u - 1] = 1.0
+    gamma = 5.0 / 3.0
+    _eos_fill(st, gamma)
+    state = make_state(dim=2, equations="hydro", adden=adden, artts=artts,
+                       hfac=hfac, box=box, spacing=dx, n_real=n, gamma=gamma,
+                       kernel=kernel, au=au)
+    return state, st
+
+
+def diffusion3d(n_side=128, hfac=1.0, ddw="fab", b=(1.0, 1.0, 0.0),
+                disotropic="no", dcond=1.0, kernel=C.KERNEL_M6, kind="staggered",
+                adden="yes", sts=True, h_scale=1.0):
+    """3D anisotropic diffusion in the periodic unit cube: T = sin(2 pi x)
+    sin(2 pi y) sin(2 pi z), B = b (field direction for the tensor), rho=1,
+    h = hfac*dx, m = 1/N (the 2M-particle density+force workload at 128^3)."""
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    pos, dx = lattice_positions(3, n_side, box, kind=kind)
+    n = pos.shape[0]
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = 1.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = dx * hfac * h_scale
+    st[:, C.es_t - 1] = (np.sin(2 * math.pi * pos[:, 0]) * np.sin(2 * math.pi * pos[:, 1])
+                         * np.sin(2 * math.pi * pos[:, 2]))
+    st[:, C.es_kappa - 1] = 1.0
+    st[:, C.es_bx - 1:C.es_bz] = np.asarray(b, dtype=np.float64)
+    state = make_state(dim=3, equations="manual", ddw=ddw, adden=adden, artts="no",
+                       disotropic=disotropic, dcond=dcond, hfac=hfac, box=box,
+                       spacing=dx, n_real=n, kernel=kernel,
+                       eqon=dict(diff=True, sts=sts))
+    return state, st
+
+
+def diffusion2d(n_side=32, hfac=1.0, ddw="fab", b=(1.0, 0.0, 0.0), disotropic="yes",
+                dcond=1.0, kernel=C.KERNEL_M6, seed=None, jitter=0.0, adden="yes",
+                h_scale=1.0):
+    """2D diffusion in periodic [0,1]^2 with T = sin(2 pi x) sin(2 pi y)."""
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 0.0)
+    pos, dx = lattice_positions(2, n_side, box)
+    n = pos.shape[0]
+    if seed is not None and jitter > 0:
+        pos[:, :2] += jitter * dx * gaussian(seed, 2 * n).reshape(n, 2)
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = 1.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = dx * hfac * h_scale
+    st[:, C.es_t - 1] = np.sin(2 * math.pi * pos[:, 0]) * np.sin(2 * math.pi * pos[:, 1])
+    st[:, C.es_kappa - 1] = 1.0
+    st[:, C.es_bx - 1:C.es_bz] = np.asarray(b, dtype=np.float64)
+    state = make_state(dim=2, equations="diffusion", ddw=ddw, adden=adden, artts="no",
+                       disotropic=disotropic, dcond=dcond, hfac=hfac, box=box,
+                       spacing=dx, n_real=n, kernel=kernel)
+    return state, st
+
+
+def random_slab3d(n, seed=3, box=(-1.0, 1.0, 0.0, 0.125, 0.0, 0.125), hfac=1.2,
+                  equations="magnetohydro", b=(0.75, 1.0, 0.0), kernel=C.KERNEL_M6,
+                  fld=False, adden="yes", artts="yes", au=1.0, process="fullyperiodic",
+                  muzero=1.0, velocity_seed=None):
+    """Uniform-random particles in a slab with a left/right shock-tube state
+    (rho 1 | 0.125, P 1 | 0.1, E_rad 1 | 0.1, equal number density so m ~ rho),
+    uniform B.  ``fld=True`` turns on diffusion+FLD (manual equation set, the
+    only way the reference enables FLD in loop 2, state.f90:140-145)."""
+    lo = np.array(box[0::2])
+    hi = np.array(box[1::2])
+    u = uniform(seed, 3 * n).reshape(n, 3)
+    pos = lo + (hi - lo) * u
+    vol = float(np.prod(hi - lo))
+    left = pos[:, 0] < 0.5 * (lo[0] + hi[0])
+    rho = np.where(left, 1.0, 0.125)
+    pres = np.where(left, 1.0, 0.1)
+    erad = np.where(left, 1.0, 0.1)
+    gamma = 5.0 / 3.0
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = rho * vol / n
+    st[:, C.es_den - 1] = rho
+    dxm = (vol / n) ** (1.0 / 3.0)
+    st[:, C.es_h - 1] = hfac * dxm * (1.0 / rho) ** (1.0 / 3.0) * np.where(left, 1.0, 0.5)
+    st[:, C.es_u - 1] = pres / ((gamma - 1.0) * rho)
+    st[:, C.es_t - 1] = erad / rho
+    st[:, C.es_kappa - 1] = 1.0
+    st[:, C.es_bx - 1:C.es_bz] = np.asarray(b, dtype=np.float64)
+    if velocity_seed is not None:
+        st[:, C.es_vx - 1:C.es_vz] = 0.1 * gaussian(velocity_seed, 3 * n).reshape(n, 3)
+    _eos_fill(st, gamma)
+    eqon = None
+    if fld:
+        equations = "manual"
+        eqon = dict(diff=True, fld=True, hydro=True)
+    state = make_state(dim=3, equations=equations, adden=adden, artts=artts, au=au,
+                       hfac=hfac, box=tuple(box), spacing=dxm, n_real=n, gamma=gamma,
+                       kernel=kernel, process=process, eqon=eqon, muzero=muzero,
+                       bordersize=(int(C.KRAD[kernel] * hfac) + 1) * dxm * 2.0)
+    return state, st
