@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Throughput of the 2M-particle density+force step on one MI355X.
+
+Workload: 3D anisotropic diffusion, 128^3 = 2,097,152 particles on the
+staggered close-packed-like lattice in the periodic unit cube, M6 kernel,
+`fab` second derivative, kappa = kappa_par b b + kappa_perp (I - b b) with
+b = (1,1,0)/sqrt(2), kappa_par = 1, kappa_perp = 1e-3, T = sin sin sin,
+Newton h-iteration in FP64.  One "step" is the reference's iterate()
+(iterator.f90:44): neighbour search + loop 1 (Newton sweeps) + EOS + loop 2,
+on particles resident in HBM.
+
+Metric: pair-interactions/s = P / step time, P = sum_i |N_i| (ordered pairs
+within krad*h_i, periodic images included).  Both paths are timed: the FP64
+path (reference arithmetic) and the mixed path (FP32 pair math, FP64
+separations/Newton/sums); `value` is the mixed path, whose density error
+against the FP64 path is reported and must stay <= 1e-5.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic work per pair / per particle (fixed figures used for the roofline)
+FLOP_PER_PAIR_LOOP2_FAB = 84      # loop 2, fab, per-particle kappa
+FLOP_PER_PAIR_LOOP1_GRAD = 50     # loop 1 sweep with grad T
+PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector and dense matrix)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def run_path(state, st, precision, steps, warmup, torch):
+    from sph_playground_amd.engine import Engine
+    eng = Engine(state, kernel="m6", precision=precision, tensor="full", kappa_perp=1e-3)
+    eng.load(st)
+    for _ in range(warmup):
+        eng.step()
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.step()
+        per.append(eng.stats())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return eng, per, t1 - t0
+
+
+def cpu_baseline(size):
+    """Reference Fortran hot path (oracle/_ref, built from the reference
+    sources) on a bounded sample of the same workload, all host cores."""
+    from oracle import refrun
+    from sph_playground_amd import problems as P
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    state, st = P.diffusion3d(size, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    if refrun.available(1):
+        r = refrun.run(state, st, kernel=1, mode=103, threads=cores)
+        pairs = int(r["n_pairs"])
+        return dict(value=pairs / r["seconds"], unit="pair-interactions/s", cores=cores, kind="reference",
+                    sample="%d^3 = %d particles of the same lattice/kernel/fab workload (reference diag(b^2) "
+                           "tensor), second iterate() of the reference OpenMP build, %.3f s, P=%d"
+                           % (size, size ** 3, r["seconds"], pairs))
+    from oracle import oracle
+    t0 = time.perf_counter()
+    o = oracle.iterate(state, st, kernel=1, tensor_mode=1, kappa_perp=1e-3)
+    dt = time.perf_counter() - t0
+    pairs = int(o["off"][-1])
+    return dict(value=pairs / dt, unit="pair-interactions/s", cores=cores, kind="port",
+                sample="%d^3 particles, C restatement, %.3f s" % (size, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--cpu-size", type=int, default=96)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from sph_playground_amd import consts as C
+    from sph_playground_amd import problems as P
+
+    state, st = P.diffusion3d(args.size, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    n = st.shape[0]
+    log("workload %d particles; FP64 path" % n)
+    e64, per64, t64 = run_path(state, st, "fp64", args.steps, args.warmup, torch)
+    log("mixed path")
+    emx, permx, tmx = run_path(state, st, "mixed", args.steps, args.warmup, torch)
+    pairs = per64[-1]["n_pairs"]
+    assert all(p["n_pairs"] == pairs for p in per64 + permx), "pair counts differ between paths"
+    v64 = pairs * args.steps / t64
+    vmx = pairs * args.steps / tmx
+    out64 = e64.save()
+    outmx = emx.save()
+    den64 = out64[:, C.es_den - 1]
+    den_err = float(np.max(np.abs(outmx[:, C.es_den - 1] - den64) / np.abs(den64)))
+    ddt64 = out64[:, C.es_ddt - 1]
+    ddt_err = float(np.max(np.abs(outmx[:, C.es_ddt - 1] - ddt64)) / np.max(np.abs(ddt64)))
+
+    def avg(per, k):
+        return float(np.mean([p[k] for p in per]))
+
+    k64 = {k: avg(per64, k) for k in ("ms_k_neigh", "ms_k_loop1", "ms_k_loop2", "ms_total")}
+    kmx = {k: avg(permx, k) for k in ("ms_k_neigh", "ms_k_loop1", "ms_k_loop2", "ms_total")}
+    sweeps = avg(permx, "sweeps")
+    # dominant kernel of the headline (mixed) path
+    dom = max(("ms_k_neigh", "ms_k_loop1", "ms_k_loop2"), key=lambda k: kmx[k])
+    if dom == "ms_k_loop2":
+        flops = pairs * FLOP_PER_PAIR_LOOP2_FAB
+        ach = flops / (kmx[dom] * 1e-3) / 1e12
+        roof = dict(kernel="k_loop2<3,M6,float> (loop 2, fab, full tensor)", bound="valu", achieved=ach,
+                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=None,
+                    flop_per_pair=FLOP_PER_PAIR_LOOP2_FAB, pairs_per_launch=pairs, ms_per_launch=kmx[dom])
+    elif dom == "ms_k_loop1":
+        p1 = avg(permx, "loop1_pairs") / max(sweeps, 1)
+        flops = p1 * FLOP_PER_PAIR_LOOP1_GRAD
+        ms = kmx[dom] / max(sweeps, 1)
+        ach = flops / (ms * 1e-3) / 1e12
+        roof = dict(kernel="k_loop1<3,M6,grad,float> (one Newton sweep)", bound="valu", achieved=ach,
+                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=None,
+                    flop_per_pair=FLOP_PER_PAIR_LOOP1_GRAD, pairs_per_launch=p1, ms_per_launch=ms)
+    else:
+        # neighbour build: algorithmic bytes = read x,y,z,m,h (40 B) + write the list (4 B/pair) + count
+        byts = n * 44 + pairs * 4
+        ach = byts / (kmx[dom] * 1e-3) / 1e9
+        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm", achieved=ach,
+                    peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=None,
+                    bytes_per_launch=byts, ms_per_launch=kmx[dom])
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        try:
+            cpu = cpu_baseline(args.cpu_size)
+        except Exception as ex:  # reported, not fatal
+            cpu = dict(value=None, unit="pair-interactions/s", cores=None, kind=None, sample="failed: %s" % ex)
+
+    line = {
+        "metric": "pair-interactions/s, 2M-particle density+force step; mixed vs FP64",
+        "value": vmx,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic (f64 positions, separations, Newton h, sums)",
+        "data": "synthetic (seeded lattice, T = sin sin sin)",
+        "config": {"workload": "3D anisotropic diffusion density+force step, %d^3 = %d particles, M6, fab, "
+                               "kappa_par=1 kappa_perp=1e-3 b=(1,1,0)/sqrt2, periodic unit cube" % (args.size, n),
+                   "pairs_per_step": pairs, "newton_sweeps_per_step": sweeps},
+        "fp64": {"value": v64, "ms_per_step": t64 * 1e3 / args.steps, "dtype": "f64", "kernels_ms": k64},
+        "mixed": {"value": vmx, "kernels_ms": kmx, "density_max_rel_err_vs_fp64": den_err,
+                  "ddt_max_err_rel_to_max": ddt_err},
+        "mixed_vs_fp64": vmx / v64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

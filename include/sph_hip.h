@@ -1,0 +1,123 @@
+/* C-ABI of the MI355X two-loop SPH hot path (libsph_hip.so).
+ *
+ * Drop-in for the reference's step kernel: the Fortran module call
+ *   call iterate(n, store, maxconsenrg)          (playground/src/iterator.f90:44)
+ * made from main.f90:219,346 and sts_integrator.f90:101,108, plus the
+ * reference's module state it reads (statevars, kernel choice).
+ *
+ * Conventions (same as the reference):
+ *   - store: FP64 particle array, particle-major: particle i's 32 fields at
+ *     store[i*ld + (es_* - 1)], i.e. the Fortran column store(:, i+1) with
+ *     leading dimension ld >= 32 (const.f90:69-103).  Real particles first,
+ *     then fixed ones; periodic ghosts are NOT appended (images are virtual),
+ *     so the caller never needs spare capacity.
+ *   - state: the 49-slot FP64 statevars vector (const.f90:14-66, state.f90:36).
+ *   - kernel_id: 0 = M4, 1 = M6, 2 = gauss (the reference's compile-time
+ *     `make kernel=`, Makefile:14,84-86; all three are built here).
+ *   - All entry points are synchronous with respect to the host and return an
+ *     SPH_* code; a context is used by one host thread at a time.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes; each mirrors an abort of the reference. */
+#define SPH_OK 0
+#define SPH_ERR_NEIGH_TRUNCATED 1 /* neighboursearch.f90:127-132 (count incl. self > maxneibnum) */
+#define SPH_ERR_NEG_H 2           /* circuit1.f90:202-204 */
+#define SPH_ERR_Q_NAN_OR_NEG 3    /* base_m6.f90:48-53 */
+#define SPH_ERR_FLD_ANISO 4       /* circuit2.f90:189,367 */
+#define SPH_ERR_BAD_DDW 5         /* circuit2.f90:421-424 */
+#define SPH_ERR_PRECONDITION 6    /* preruncheck.f90:13-60 */
+#define SPH_ERR_BAD_ARG 7         /* invalid argument (state, sizes, process) */
+#define SPH_ERR_HIP 8             /* HIP runtime failure */
+#define SPH_ERR_NOMEM 9           /* device allocation failed */
+#define SPH_ERR_UNSUPPORTED 10    /* outside the implemented scope (e.g. cylindrical, kernel.f90:137) */
+
+typedef struct sph_ctx sph_ctx;
+
+typedef struct {
+  int precision;      /* 0 = FP64 (reference arithmetic); 1 = mixed: FP64 positions,
+                         separations, Newton update and energy sum, FP32 pair math */
+  int tensor_mode;    /* 0 = reference diffusion tensor (difcond*I or difcond*diag(b^2),
+                         circuit2.f90:194-206); 1 = kappa_par*bb + kappa_perp*(I-bb) */
+  double kappa_perp;  /* perpendicular conductivity for tensor_mode 1 */
+  int max_sweeps;     /* Newton sweeps cap; 0 -> 100 (circuit1.f90:114) */
+  double newton_tol;  /* relative h tolerance; 0 -> 1e-8 (circuit1.f90:104) FP64, 1e-6 mixed */
+  int neigh_cap;      /* neighbour cap incl. self; 0 -> reference maxneibnum(dim) */
+  int device;         /* HIP device ordinal */
+} sph_options;
+
+typedef struct {
+  int64_t n_real, n_fixed;
+  int64_t n_pairs;    /* P = sum_i |N_i| of the last step */
+  int64_t loop1_pairs;/* pairs evaluated by loop 1 over all Newton sweeps */
+  int sweeps;         /* Newton sweeps of the last step (0 for c1simple) */
+  int sorted;         /* 1 if the last step had to reorder particles */
+  double energy;      /* FP64 energy sum of loop 2 (circuit2.f90:490-492); the
+                         reference computes and discards it */
+  double ms_neigh, ms_loop1, ms_loop2, ms_total; /* device time of the last step's phases */
+  double ms_k_neigh, ms_k_loop1, ms_k_loop2;     /* device time of the neighbour kernel, all
+                                                    Newton sweep kernels, the loop-2 kernel */
+} sph_stats;
+
+/* Fill *opt with defaults (FP64, reference tensor, 100 sweeps, 1e-8). */
+int sph_default_options(sph_options *opt);
+
+/* Create a context.  Freezes what the reference freezes at first call
+ * (iterator.f90:31-42, circuit1.f90:38-51, circuit2.f90:45-56, BC.f90:46-68):
+ * dim, equation set, ddw, adden, artts, au, ics, process, box, bordersize.
+ * Replaces: initkernel (kernel.f90:91), initBorders (BC.f90:46). */
+int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt, sph_ctx **out);
+int sph_ctx_destroy(sph_ctx *ctx);
+
+/* iterate(n, store, maxconsenrg), iterator.f90:44-179, on a HOST store:
+ * n = real + fixed particles; store updated in place exactly on the rows the
+ * reference writes (wrapped positions, den, h, om, p, c, dtdx..dtdz, ax..az,
+ * dh, du, ddt, dbx..dbz, vfld).  *maxconsenrg = 0 as the reference
+ * (circuit2.f90:101); the energy sum is in sph_get_stats. */
+int sph_iterate(sph_ctx *ctx, int64_t n, double *store, int64_t ld, const double state[49],
+                double *maxconsenrg);
+
+/* Same on a DEVICE store (d_store is device memory, same layout). */
+int sph_iterate_device(sph_ctx *ctx, int64_t n, double *d_store, int64_t ld, const double state[49],
+                       double *maxconsenrg);
+
+/* Resident use: load a store once, step many times, save.  on_device selects
+ * whether `store` is host or device memory. */
+int sph_load(sph_ctx *ctx, int64_t n, const double *store, int64_t ld, int on_device);
+int sph_save(sph_ctx *ctx, double *store, int64_t ld, int on_device);
+int sph_step(sph_ctx *ctx, const double state[49], double *maxconsenrg);
+
+/* Neighbour lists of the last step (replaces xgetneighnumber/xgetneighindex,
+ * neighboursearch.f90:221-237, plus getCrossRef, BC.f90:36-44): for real
+ * particle i (original order) entries off[i]..off[i+1]-1 hold j = real/fixed
+ * index of the neighbour and code = (sx+1)+3(sy+1)+9(sz+1), the periodic image
+ * shift (13 = none).  off has n_real+1 entries; cap_pairs bounds j/code. */
+int sph_neighbours(sph_ctx *ctx, int64_t *off, int32_t *j, int8_t *code, int64_t cap_pairs,
+                   int64_t *n_pairs);
+
+/* One leapfrog step around iterate (main.f90:216-362, no STS): initial
+ * iterate, dt = min(timestep.calc dt, dtcap) (timestep.f90:40-166),
+ * predictor, iterate, corrector, all on the resident state. */
+int sph_leapfrog(sph_ctx *ctx, const double state[49], double dtcap, double *dt_used);
+
+/* One RKL2 super-time-step of T (sts_integrator.f90:87-124) with s stages on
+ * the resident state; s iterates. */
+int sph_sts(sph_ctx *ctx, const double state[49], int s, double dt);
+
+/* FP64 RKL2 coefficients (sts_integrator.f90:10-51) for stage j of s:
+ * out[0..3] = mu(j), nu(j), mu_hat(j,s), gm_hat(j,s); j >= 1. */
+int sph_sts_coeffs(int j, int s, double out[4]);
+
+int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
+const char *sph_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -8,6 +8,9 @@
 !   mode 1: one leapfrog step as main.f90:216-362 (no STS): iterate, calc,
 !           predictor, iterate, corrector; dt = min(calc dt, dtcap)
 !   mode 2: sts_integrate(n, s, store, dt)                 sts_integrator.f90:87
+!   mode 3: iterate twice, timing only the second call (steady state: the
+!           first call also runs initIterate's extra neighbour search)
+!   mode + 100: same, without writing the neighbour lists
 !
 ! Input  (stream, little endian): int32 mode, n_real, n_fixed, sts_s;
 !        real64 dtcap; real64 state(49); real64 store(32, n_real+n_fixed)
@@ -29,7 +32,8 @@ program ref_harness
 
   character(len=1024) :: fin, fout
   integer :: mode, nreal, nfixed, sts_s, ntot, dim, i, k, j, rj, code, cnt, ax
-  integer :: sgn(3)
+  integer :: sgn(3), dumpnb
+  integer(8) :: npairs
   real :: dtcap, dedt, dt, dthydro, dtdiff, dtfld, t0, t1
   real :: st(ec_total)
   real, allocatable :: store(:,:), tr(:,:), tv(:,:), ta(:,:), tdu(:), tdh(:), tddt(:), tdb(:,:)
@@ -39,6 +43,11 @@ program ref_harness
 
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
   read(10) mode, nreal, nfixed, sts_s
+  dumpnb = 1
+  if (mode >= 100) then
+    dumpnb = 0
+    mode = mode - 100
+  end if
   read(10) dtcap
   read(10) st
   ntot = nreal + nfixed
@@ -53,9 +62,10 @@ program ref_harness
   call initBorders()
 
   dt = 0.
+  if (mode == 3) call iterate(ntot, store, dedt)
   t0 = omp_get_wtime()
   select case (mode)
-  case (0)
+  case (0, 3)
     call iterate(ntot, store, dedt)
   case (1)
     ! main.f90:219 initial accelerations, then one pass of the loop body
@@ -96,6 +106,17 @@ program ref_harness
   write(11) ntot
   write(11) store(:, 1:ntot)
   write(11) dt
+  if (dumpnb == 0) then
+    ! only the pair count P = sum_i |N_i|
+    npairs = 0
+    do i = 1, nreal
+      npairs = npairs + xgetneighnumber(i)
+    end do
+    write(11) -1
+    write(11) npairs
+    close(11)
+    stop
+  end if
   write(11) nreal
   do i = 1, nreal
     cnt = xgetneighnumber(i)

@@ -62,6 +62,9 @@ def run(state, store, kernel=1, mode=0, sts_s=0, dtcap=0.0, threads=None):
     off += 8
     (nr,) = struct.unpack_from("<i", raw, off)
     off += 4
+    if nr == -1:
+        (npairs,) = struct.unpack_from("<q", raw, off)
+        return dict(store=out, dt=dt, neigh=None, n_pairs=npairs, seconds=secs, stdout=p.stdout)
     neigh = []
     for _ in range(nr):
         (cnt,) = struct.unpack_from("<i", raw, off)
@@ -70,4 +73,5 @@ def run(state, store, kernel=1, mode=0, sts_s=0, dtcap=0.0, threads=None):
         off += 8 * cnt
         a[:, 0] -= 1
         neigh.append(a)
-    return dict(store=out, dt=dt, neigh=neigh, seconds=secs, stdout=p.stdout)
+    return dict(store=out, dt=dt, neigh=neigh, n_pairs=sum(len(a) for a in neigh), seconds=secs,
+                stdout=p.stdout)

@@ -1,0 +1,971 @@
+// Host side of libsph_hip.so: context, step sequencing and the C-ABI
+// declared in include/sph_hip.h.
+//
+// sph_step follows iterate() (iterator.f90:44-179): for the process/ics chosen
+// at context creation it wraps escaped particles, builds the neighbour lists
+// (virtual periodic images instead of ghost copies), runs loop 1 (c1advanced
+// Newton sweeps or c1simple), the EOS and loop 2.  All particle data stay in
+// device memory between calls; the host only sizes the cell grid (one small
+// read-back), watches the Newton convergence count and reads error bits.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sph_hip.h"
+#include "sph_kernels.h"
+
+using namespace sph;
+
+namespace sph {
+// kernels (sph_kernels.hip)
+__global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, long long *, int *);
+__global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
+__global__ void k_check_sorted(const uint32_t *, int, int *);
+__global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
+__global__ void k_cell_range(const uint32_t *, int, int *, int *);
+__global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, int, double4 *, double *, uint16_t *,
+                       uint8_t *, double4 *, double *);
+template <int DIM>
+__global__ void k_neigh(NeighArgs);
+template <int DIM, int KT, int GRAD, typename PT>
+__global__ void k_loop1(Loop1Args<double4>);
+template <int DIM, int KT, int GRAD, typename PT>
+__global__ void k_loop1_simple(Loop1Args<double4>, Fields);
+template <typename V4>
+__global__ void k_post1(Fields, const double4 *, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
+                        V4 *, V4 *, V4 *);
+template <int DIM, int KT, typename PT, typename V4>
+__global__ void k_loop2(Loop2Args<V4>);
+__global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
+__global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
+__global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
+                               const int64_t *, int32_t *, int8_t *);
+__global__ void k_counts_by_orig(const int *, const int64_t *, const uint8_t *, int, int64_t *);
+__global__ void k_find_orig(const int64_t *, int, int64_t, int *);
+__global__ void k_predict(Fields, const uint8_t *, int, double, int, double *, int);
+__global__ void k_correct(Fields, const uint8_t *, int, double, int, const double *, int);
+__global__ void k_dt_reduce(Fields, const uint8_t *, int, long long *);
+__global__ void k_sts_stage(double *, const double *, const double *, const double *, const double *,
+                            const double *, int, double, double, double, double, double *);
+__global__ void k_axpy_copy(double *, const double *, const double *, double, int);
+}  // namespace sph
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fields
+
+// statevars slots (0-based), const.f90:14-66
+enum { S_DIM = 0, S_EQS = 1, S_DDW = 2, S_ICS = 3, S_ADDEN = 4, S_ARTTS = 5, S_COORD = 6, S_DISO = 7,
+       S_DCOND = 8, S_MUZERO = 9, S_HFAC = 12, S_XMIN = 16, S_BORD = 22, S_PAD = 23, S_REALPN = 24,
+       S_FIXEDPN = 25, S_GAMMA = 26, S_PROCESS = 30, S_AU = 31, S_EQONHYDRO = 33, S_EQONDIFF = 34,
+       S_EQONFLD = 35 };
+
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+inline double ord2d(long long v) {
+  long long b = v >= 0 ? v : (v ^ 0x7FFFFFFFFFFFFFFFLL);
+  double d;
+  memcpy(&d, &b, 8);
+  return d;
+}
+inline long long d2ord_h(double d) {
+  long long b;
+  memcpy(&b, &d, 8);
+  return b >= 0 ? b : (b ^ 0x7FFFFFFFFFFFFFFFLL);
+}
+
+}  // namespace
+
+struct sph_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int kt = 1;
+  sph_options opt{};
+  Phys frozen{};
+  Box box{};
+  int process = 0, ics = 0, do_eos = 1, eos_iso = 0;
+  int maxneib = 150;
+  // particles
+  int64_t n = 0, n_real = 0, n_fixed = 0, cap = 0, npad = 0;
+  double *fbuf[2] = {nullptr, nullptr};  // (32 + kAux) * npad doubles each
+  int cur = 0;
+  int naux = 0;
+  int64_t *orig[2] = {nullptr, nullptr};
+  uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int *cstart = nullptr, *cend = nullptr;
+  int64_t ncell_cap = 0;
+  double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
+  double *h0 = nullptr, *eps = nullptr;
+  uint16_t *band = nullptr;
+  uint8_t *real = nullptr;
+  uint32_t *nb = nullptr;
+  int nb_cap = 0;
+  int *cnt = nullptr;
+  void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *energy_part = nullptr;
+  int64_t energy_cap = 0;
+  // small device scratch: [0..7] long long reductions, then ints
+  long long *d_red = nullptr;
+  int *d_int = nullptr;  // 0 err, 1 maxcnt, 2 sorted flag, 3 pos, 8.. sweep counters (max_sweeps+2)
+  unsigned long long *d_pairs = nullptr;  // 0 neighbour pairs, 1 loop-1 pairs
+  int *h_int = nullptr;                   // pinned mirror of d_int
+  long long *h_ll = nullptr;              // pinned scratch for small host<->device values
+  double *d_staging = nullptr;
+  int64_t staging_bytes = 0;
+  double4 *final_st = nullptr;
+  Grid grid{};
+  sph_stats stats{};
+  bool have_step = false;
+  hipEvent_t ev[8]{};
+
+  Fields fields() const {
+    Fields F;
+    for (int f = 0; f < F_TOTAL; f++) F.f[f] = fbuf[cur] + (size_t)f * npad;
+    return F;
+  }
+  double *aux(int k) const { return fbuf[cur] + (size_t)(F_TOTAL + k) * npad; }
+};
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      fprintf(stderr, "sph_hip: %s failed: %s\n", #x, hipGetErrorString(e_));      \
+      return SPH_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+#define RCHK(x)             \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != SPH_OK) return r_; \
+  } while (0)
+
+static int dev_alloc(void **p, size_t bytes) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return SPH_ERR_NOMEM; }
+  return SPH_OK;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t count) { return dev_alloc((void **)p, count * sizeof(T)); }
+
+// --------------------------------------------------------------- kernel base
+static double knorm(int kt, int dim) {
+  const double pi = kPi;
+  if (kt == KT_M4) { const double k[3] = {2. / 3., 10. / (7. * pi), 1. / pi}; return k[dim - 1]; }
+  if (kt == KT_M6) { const double k[3] = {1. / 120., 7. / (478. * pi), 1. / (120. * pi)}; return k[dim - 1]; }
+  const double k[3] = {1. / sqrt(pi), 1. / pi, 1. / pow(pi, 1.5)};
+  return k[dim - 1];
+}
+static double fwc_of(int kt, int dim) {
+  if (kt == KT_M4) { const double f[3] = {6., 6.3226, 6.66400}; return f[dim - 1]; }
+  if (kt == KT_M6) { const double f[3] = {4., 9.31505 / 2.25, 9.6429 / 2.25}; return f[dim - 1]; }
+  return 4.0;
+}
+static int maxneib_of(int kt, int dim) {
+  static const int t[3][3] = {{35, 250, 150}, {50, 10000, 150}, {100, 5000, 50000}};
+  return t[kt][dim - 1];
+}
+// CFL numbers cnarr (base_m6.f90:30-33, base_m4.f90:127-130, base_gauss.f90:222-225)
+static void cn_of(int kt, int ddw, double *cnhydro, double *cnd2) {
+  static const double t[3][4] = {{0.1, 0.96, 0.15, 0.2}, {0.3, 1.1, 0.2, 0.4}, {0.55, 1.25, 0.24, 0.50}};
+  *cnhydro = t[kt][0];
+  if (ddw == 401) *cnd2 = t[kt][3];                        // n2w
+  else if (ddw == 403 || ddw == 404) *cnd2 = t[kt][1];     // 2nw
+  else *cnd2 = t[kt][2];                                   // fab, fw, default
+}
+
+// getEqComponent, state.f90:117-149
+static void eq_components(const double *s, Phys *P) {
+  P->hydro = P->magneto = P->diff = P->fld = 0;
+  switch ((int)s[S_EQS]) {
+    case 300: P->hydro = 1; break;
+    case 302: P->hydro = P->magneto = 1; break;
+    case 301: P->diff = 1; break;
+    case 306: P->fld = P->hydro = 1; break;
+    case 303: P->hydro = P->magneto = P->diff = 1; break;
+    case 304: P->hydro = P->diff = 1; break;
+    case 307:
+      P->diff = (int)s[S_EQONDIFF] == 1;
+      P->hydro = (int)s[S_EQONHYDRO] == 1;
+      P->fld = (int)s[S_EQONFLD] == 1;
+      break;
+    default: break;
+  }
+}
+
+// ------------------------------------------------------------------ buffers
+static int ensure_particles(sph_ctx *c, int64_t n) {
+  if (n <= c->cap) return SPH_OK;
+  int64_t cap = std::max<int64_t>(n, 1024);
+  int64_t npad = (cap + 63) / 64 * 64;
+  const size_t nf = F_TOTAL + kAux;
+  for (int b = 0; b < 2; b++) {
+    RCHK(dalloc(&c->fbuf[b], nf * npad));
+    RCHK(dalloc(&c->orig[b], npad));
+    RCHK(dalloc(&c->st[b], npad));
+  }
+  RCHK(dalloc(&c->keys, npad));
+  RCHK(dalloc(&c->keys2, npad));
+  RCHK(dalloc(&c->vals, npad));
+  RCHK(dalloc(&c->vals2, npad));
+  RCHK(dalloc(&c->pm, npad));
+  RCHK(dalloc(&c->h0, npad));
+  RCHK(dalloc(&c->eps, npad));
+  RCHK(dalloc(&c->band, npad));
+  RCHK(dalloc(&c->real, npad));
+  RCHK(dalloc(&c->cnt, npad));
+  size_t v4 = c->opt.precision == 1 ? sizeof(float4) : sizeof(double4);
+  for (int k = 0; k < 5; k++) RCHK(dev_alloc(&c->q[k], v4 * npad));
+  c->energy_cap = (npad + kBlock - 1) / kBlock;
+  RCHK(dalloc(&c->energy_part, c->energy_cap));
+  // radix-sort scratch for n keys
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->keys, c->keys2, c->vals, c->vals2, (int)npad, 0, 32,
+                                     c->stream);
+  RCHK(dev_alloc(&c->sort_tmp, bytes));
+  c->sort_tmp_bytes = bytes;
+  c->cap = cap;
+  c->npad = npad;
+  c->nb_cap = 0;
+  if (c->nb) { (void)hipFree(c->nb); c->nb = nullptr; }
+  return SPH_OK;
+}
+
+static int ensure_nb(sph_ctx *c, int cap) {
+  if (cap <= c->nb_cap && c->nb) return SPH_OK;
+  cap = std::max(cap, 16);
+  RCHK(dalloc(&c->nb, (size_t)cap * c->npad));
+  c->nb_cap = cap;
+  return SPH_OK;
+}
+
+static int ensure_cells(sph_ctx *c, int64_t ncell) {
+  if (ncell <= c->ncell_cap) return SPH_OK;
+  int64_t cap = std::max<int64_t>(ncell, 1024);
+  RCHK(dalloc(&c->cstart, cap));
+  RCHK(dalloc(&c->cend, cap));
+  c->ncell_cap = cap;
+  return SPH_OK;
+}
+
+static float elapsed(sph_ctx *c, hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+static int map_err(int bits) {
+  if (bits & EB_Q) return SPH_ERR_Q_NAN_OR_NEG;
+  if (bits & EB_NEG_H) return SPH_ERR_NEG_H;
+  if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
+  if (bits & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
+  return SPH_OK;
+}
+
+static int read_ints(sph_ctx *c, int first, int count) {
+  HIPCHK(hipMemcpyAsync(c->h_int + first, c->d_int + first, sizeof(int) * count, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+// ------------------------------------------------------- step: neighbours
+static int build_neighbours(sph_ctx *c) {
+  const int n = (int)c->n;
+  const int dim = c->frozen.dim;
+  Fields F = c->fields();
+  unsigned G = grid_for(n);
+  // 1. wrap escaped real particles, bounding box, max h
+  long long *init = c->h_ll;
+  for (int a = 0; a < 3; a++) { init[a] = 0x7FFFFFFFFFFFFFFFLL; init[3 + a] = -0x7FFFFFFFFFFFFFFFLL - 1; }
+  init[6] = -0x7FFFFFFFFFFFFFFFLL - 1;
+  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
+  hipLaunchKernelGGL(k_wrap_reduce, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box,
+                     dim, c->d_red, c->d_int);
+  long long *red = c->h_ll + 8;
+  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7, hipMemcpyDeviceToHost, c->stream));
+  RCHK(read_ints(c, 0, 1));
+  if (c->h_int[0] & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; a++) { lo[a] = ord2d(red[a]); hi[a] = ord2d(red[3 + a]); }
+  double hmax = c->n_real > 0 ? ord2d(red[6]) : 0.0;
+  if (!(hmax > 0.0) && c->n_real > 0) return SPH_ERR_Q_NAN_OR_NEG;
+  double rmax = c->frozen.krad * hmax * (1.0 + 1e-9);
+  if (!(rmax > 0.0)) rmax = 1.0;
+  // 2. cell grid: cells at least rmax wide; imaged axes tile [emin, emax]
+  Grid g{};
+  double cs[3];
+  for (int a = 0; a < 3; a++) {
+    g.periodic[a] = 0;
+    if (a >= dim) { g.lo[a] = lo[a]; g.nc[a] = 1; cs[a] = 0; continue; }
+    if ((c->box.ghost_mask >> a) & 1) {
+      double L = c->box.emax[a] - c->box.emin[a];
+      int64_t nc = (int64_t)std::floor(L / rmax);
+      nc = std::max<int64_t>(1, std::min<int64_t>(nc, 1 << 20));
+      g.nc[a] = (int)nc;
+      g.lo[a] = c->box.emin[a];
+      cs[a] = L / (double)nc;
+      g.periodic[a] = 1;
+    } else {
+      double ext = hi[a] - lo[a];
+      int64_t nc = (int64_t)std::floor(ext / rmax) + 1;
+      nc = std::max<int64_t>(1, std::min<int64_t>(nc, 1 << 20));
+      g.nc[a] = (int)nc;
+      g.lo[a] = lo[a];
+      cs[a] = rmax;
+    }
+  }
+  // bound the cell count (sparse sets): coarsen the largest axis
+  while ((int64_t)g.nc[0] * g.nc[1] * g.nc[2] > (int64_t)1 << 26) {
+    int a = g.nc[0] >= g.nc[1] ? (g.nc[0] >= g.nc[2] ? 0 : 2) : (g.nc[1] >= g.nc[2] ? 1 : 2);
+    if (g.periodic[a]) {
+      g.nc[a] = std::max(1, g.nc[a] / 2);
+      cs[a] = (c->box.emax[a] - c->box.emin[a]) / g.nc[a];
+    } else {
+      cs[a] *= 2;
+      g.nc[a] = (int)std::floor((hi[a] - lo[a]) / cs[a]) + 1;
+    }
+  }
+  for (int a = 0; a < 3; a++) g.inv_cs[a] = cs[a] > 0 ? 1.0 / cs[a] : 0.0;
+  c->grid = g;
+  int64_t ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+  RCHK(ensure_cells(c, ncell));
+  // 3. keys, sortedness, radix sort + reorder when needed
+  hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
+  hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
+  RCHK(read_ints(c, 2, 1));
+  c->stats.sorted = 0;
+  if (c->h_int[2]) {
+    int bits = 1;
+    while (((int64_t)1 << bits) < ncell) bits++;
+    size_t bytes = c->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, bytes, c->keys, c->keys2, c->vals, c->vals2, n, 0, bits,
+                                              c->stream));
+    int nxt = 1 - c->cur;
+    Fields src = c->fields();
+    Fields dst;
+    for (int f = 0; f < F_TOTAL; f++) dst.f[f] = c->fbuf[nxt] + (size_t)f * c->npad;
+    hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, src, dst, c->vals2, c->orig[c->cur],
+                       c->orig[nxt], n);
+    // auxiliary arrays ride along in groups of 32
+    for (int k0 = 0; k0 < c->naux; k0 += F_TOTAL) {
+      Fields as{}, ad{};
+      int m = std::min((int)F_TOTAL, c->naux - k0);
+      for (int f = 0; f < F_TOTAL; f++) {
+        int k = k0 + std::min(f, m - 1);
+        as.f[f] = c->fbuf[c->cur] + (size_t)(F_TOTAL + k) * c->npad;
+        ad.f[f] = c->fbuf[nxt] + (size_t)(F_TOTAL + k) * c->npad;
+      }
+      hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, as, ad, c->vals2, c->orig[c->cur],
+                         c->orig[nxt], n);
+    }
+    c->cur = nxt;
+    std::swap(c->keys, c->keys2);
+    F = c->fields();
+    c->stats.sorted = 1;
+  }
+  HIPCHK(hipMemsetAsync(c->cstart, 0, sizeof(int) * ncell, c->stream));
+  HIPCHK(hipMemsetAsync(c->cend, 0, sizeof(int) * ncell, c->stream));
+  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->cstart, c->cend);
+  hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, dim,
+                     c->pm, c->h0, c->band, c->real, c->st[0], c->eps);
+  // 4. neighbour lists; grow the storage and redo if any list overflowed
+  if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
+  for (int attempt = 0; attempt < 2; attempt++) {
+    HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 2, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_pairs, 0, sizeof(unsigned long long), c->stream));
+    NeighArgs a;
+    a.pm = c->pm; a.hs = c->h0; a.band = c->band; a.real = c->real; a.cstart = c->cstart; a.cend = c->cend;
+    a.g = g; a.b = c->box; a.krad = c->frozen.krad; a.n = n; a.npad = (int)c->npad; a.cap = c->nb_cap;
+    a.maxneib = c->maxneib; a.nb = c->nb; a.cnt = c->cnt; a.err = c->d_int; a.maxcnt = c->d_int + 1;
+    a.pairs = c->d_pairs;
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    if (dim == 1) hipLaunchKernelGGL(k_neigh<1>, dim3(G), dim3(kBlock), 0, c->stream, a);
+    else if (dim == 2) hipLaunchKernelGGL(k_neigh<2>, dim3(G), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_neigh<3>, dim3(G), dim3(kBlock), 0, c->stream, a);
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_ll + 16, c->d_pairs, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    RCHK(read_ints(c, 0, 2));
+    c->stats.n_pairs = (int64_t)c->h_ll[16];
+    c->stats.ms_k_neigh = elapsed(c, c->ev[5], c->ev[6]);
+    int bits = c->h_int[0];
+    if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
+    if (!(bits & EB_OVERFLOW)) return SPH_OK;
+    RCHK(ensure_nb(c, c->h_int[1] + 8));
+  }
+  return SPH_ERR_HIP;
+}
+
+// ------------------------------------------------------------ step: loops
+template <int DIM, int KT>
+static void launch_loop1(sph_ctx *c, Loop1Args<double4> &a, int grad, bool simple, Fields F) {
+  unsigned G = grid_for(a.n);
+  bool mixed = c->opt.precision == 1;
+#define L1(GR, PT)                                                                                   \
+  if (simple) hipLaunchKernelGGL((k_loop1_simple<DIM, KT, GR, PT>), dim3(G), dim3(kBlock), 0, c->stream, a, F); \
+  else hipLaunchKernelGGL((k_loop1<DIM, KT, GR, PT>), dim3(G), dim3(kBlock), 0, c->stream, a)
+  if (mixed) {
+    if (grad == 0) { L1(0, float); } else if (grad == 1) { L1(1, float); } else { L1(2, float); }
+  } else {
+    if (grad == 0) { L1(0, double); } else if (grad == 1) { L1(1, double); } else { L1(2, double); }
+  }
+#undef L1
+}
+
+template <int DIM>
+static void launch_loop1_k(sph_ctx *c, Loop1Args<double4> &a, int grad, bool simple, Fields F) {
+  if (c->kt == KT_M4) launch_loop1<DIM, KT_M4>(c, a, grad, simple, F);
+  else if (c->kt == KT_M6) launch_loop1<DIM, KT_M6>(c, a, grad, simple, F);
+  else launch_loop1<DIM, KT_GAUSS>(c, a, grad, simple, F);
+}
+
+static int run_loop1(sph_ctx *c, const Phys &P) {
+  const int n = (int)c->n;
+  Fields F = c->fields();
+  Loop1Args<double4> a{};
+  a.pm = c->pm; a.h0 = c->h0; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt; a.eps = c->eps;
+  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = c->d_int; a.pairs = c->d_pairs + 1;
+  a.b = c->box; a.p = P; a.n = n; a.npad = (int)c->npad;
+  double tol = c->opt.newton_tol > 0 ? c->opt.newton_tol : (c->opt.precision == 1 ? 1e-6 : 1e-8);
+  a.tol = tol;
+  int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
+  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_pairs + 1, 0, sizeof(unsigned long long), c->stream));
+  if (!P.adden) {
+    // c1simple: grad T only with diffusion and the 2nw operators
+    int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
+    a.cur = c->st[0];
+    a.next = c->st[1];
+    c->h_int[8] = n;
+    HIPCHK(hipMemcpyAsync(c->d_int + 8, c->h_int + 8, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    a.prev_active = c->d_int + 8;
+    a.active = c->d_int + 9;
+    if (P.dim == 1) launch_loop1_k<1>(c, a, grad, true, F);
+    else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, true, F);
+    else launch_loop1_k<3>(c, a, grad, true, F);
+    c->final_st = c->st[0];
+    c->stats.sweeps = 0;
+    RCHK(read_ints(c, 0, 1));
+    return map_err(c->h_int[0]);
+  }
+  int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
+  int nreal = (int)c->n_real;
+  for (int k = 0; k < maxsw + 2; k++) c->h_int[8 + k] = 0;
+  c->h_int[8] = nreal;
+  HIPCHK(hipMemcpyAsync(c->d_int + 8, c->h_int + 8, sizeof(int) * (maxsw + 2), hipMemcpyHostToDevice, c->stream));
+  int sweeps = 0;
+  // while (maxval(eps, eps > 0) > tol .and. iter < 100), circuit1.f90:114;
+  // each sweep counts the particles it leaves active
+  for (int it = 1; nreal > 0 && it <= maxsw; it++) {
+    a.cur = c->st[(it - 1) & 1];
+    a.next = c->st[it & 1];
+    a.prev_active = c->d_int + 8 + (it - 1);
+    a.active = c->d_int + 8 + it;
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
+    else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
+    else launch_loop1_k<3>(c, a, grad, false, F);
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    sweeps = it;
+    HIPCHK(hipMemcpyAsync(c->h_int, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RCHK(read_ints(c, 8 + it, 1));
+    c->stats.ms_k_loop1 += elapsed(c, c->ev[5], c->ev[6]);
+    if (c->h_int[0]) break;  // circuit1.f90:202-204 stops inside the sweep
+    if (c->h_int[8 + it] == 0) break;
+  }
+  c->final_st = c->st[sweeps & 1];
+  c->stats.sweeps = sweeps;
+  RCHK(read_ints(c, 0, 1));
+  HIPCHK(hipMemcpyAsync(c->h_ll + 17, c->d_pairs + 1, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->stats.loop1_pairs = (int64_t)c->h_ll[17];
+  return map_err(c->h_int[0]);
+}
+
+template <int DIM, int KT>
+static void launch_loop2(sph_ctx *c, void *args) {
+  unsigned G = grid_for(c->n);
+  if (c->opt.precision == 1)
+    hipLaunchKernelGGL((k_loop2<DIM, KT, float, float4>), dim3(G), dim3(kBlock), 0, c->stream,
+                       *(Loop2Args<float4> *)args);
+  else
+    hipLaunchKernelGGL((k_loop2<DIM, KT, double, double4>), dim3(G), dim3(kBlock), 0, c->stream,
+                       *(Loop2Args<double4> *)args);
+}
+template <int DIM>
+static void launch_loop2_k(sph_ctx *c, void *args) {
+  if (c->kt == KT_M4) launch_loop2<DIM, KT_M4>(c, args);
+  else if (c->kt == KT_M6) launch_loop2<DIM, KT_M6>(c, args);
+  else launch_loop2<DIM, KT_GAUSS>(c, args);
+}
+
+template <typename V4>
+static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
+  const int n = (int)c->n;
+  Fields F = c->fields();
+  unsigned G = grid_for(n);
+  // reference c1 for isothermal EOS reads particle 1's sound speed (eos.f90:43)
+  double c1 = 0.0;
+  if (c->do_eos && c->eos_iso && n > 0) {
+    HIPCHK(hipMemsetAsync(c->d_int + 3, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_find_orig, dim3(G), dim3(kBlock), 0, c->stream, c->orig[c->cur], n, (int64_t)0,
+                       c->d_int + 3);
+    RCHK(read_ints(c, 3, 1));
+    HIPCHK(hipMemcpy(&c1, F.f[F_C] + c->h_int[3], sizeof(double), hipMemcpyDeviceToHost));
+  }
+  const bool need_v = P.adden || P.artts || P.hydro || P.magneto;
+  const bool need_q3 = P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
+  const bool need_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
+  const bool need_q2 = P.diff || (P.artts && P.au != -1.0);
+  V4 *q0 = (V4 *)c->q[0];
+  V4 *q1 = need_v ? (V4 *)c->q[1] : nullptr;
+  V4 *q2 = need_q2 ? (V4 *)c->q[2] : nullptr;
+  V4 *q3 = need_q3 ? (V4 *)c->q[3] : nullptr;
+  V4 *q4 = need_q4 ? (V4 *)c->q[4] : nullptr;
+  hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
+                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4);
+  if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
+  if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
+  Loop2Args<V4> a{};
+  a.pm = c->pm; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
+  a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
+  a.out = F; a.energy_part = c->energy_part; a.err = c->d_int; a.b = c->box; a.p = P; a.n = n;
+  a.npad = (int)c->npad;
+  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
+  HIPCHK(hipEventRecord(c->ev[5], c->stream));
+  if (P.dim == 1) launch_loop2_k<1>(c, &a);
+  else if (P.dim == 2) launch_loop2_k<2>(c, &a);
+  else launch_loop2_k<3>(c, &a);
+  HIPCHK(hipEventRecord(c->ev[6], c->stream));
+  std::vector<double> parts(G);
+  HIPCHK(hipMemcpyAsync(parts.data(), c->energy_part, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  RCHK(read_ints(c, 0, 1));
+  double e = 0;
+  for (unsigned b = 0; b < G; b++) e += parts[b];
+  c->stats.energy = e;
+  c->stats.ms_k_loop2 = elapsed(c, c->ev[5], c->ev[6]);
+  return map_err(c->h_int[0]);
+}
+
+static Phys call_phys(const sph_ctx *c, const double *state) {
+  Phys P = c->frozen;
+  // read at every call by the reference (circuit1.f90:96, circuit2.f90:86-90)
+  P.hfac = state[S_HFAC];
+  P.dcond = state[S_DCOND];
+  P.diso = (int)state[S_DISO];
+  P.muzero = state[S_MUZERO];
+  P.gamma = state[S_GAMMA];
+  return P;
+}
+
+static int run_step(sph_ctx *c, const double *state) {
+  if (c->n <= 0) return SPH_ERR_BAD_ARG;
+  Phys P = call_phys(c, state);
+  c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  RCHK(build_neighbours(c));
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  RCHK(run_loop1(c, P));
+  HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
+  HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(hipEventSynchronize(c->ev[3]));
+  float t01 = 0, t12 = 0, t23 = 0;
+  (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
+  (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+  c->stats.ms_neigh = t01;
+  c->stats.ms_loop1 = t12;
+  c->stats.ms_loop2 = t23;
+  c->stats.ms_total = t01 + t12 + t23;
+  c->have_step = true;
+  return rc;
+}
+
+// ----------------------------------------------------------------- loading
+static int load_impl(sph_ctx *c, int64_t n, const double *store, int64_t ld, int on_device) {
+  if (n <= 0 || ld < F_TOTAL || !store) return SPH_ERR_BAD_ARG;
+  if (n > ((int64_t)1 << 27) - 1) return SPH_ERR_UNSUPPORTED;
+  if (n < c->n_real) return SPH_ERR_BAD_ARG;
+  RCHK(ensure_particles(c, n));
+  c->n = n;
+  c->n_fixed = n - c->n_real;
+  const double *src = store;
+  int64_t bytes = n * ld * (int64_t)sizeof(double);
+  if (!on_device) {
+    if (bytes > c->staging_bytes) {
+      RCHK(dev_alloc((void **)&c->d_staging, bytes));
+      c->staging_bytes = bytes;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_staging, store, bytes, hipMemcpyHostToDevice, c->stream));
+    src = c->d_staging;
+  }
+  c->cur = 0;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, ld, (int)n, c->fields(),
+                     c->orig[0]);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_step = false;
+  return SPH_OK;
+}
+
+static int save_impl(sph_ctx *c, double *store, int64_t ld, int on_device) {
+  if (!store || ld < F_TOTAL || c->n <= 0) return SPH_ERR_BAD_ARG;
+  int64_t bytes = c->n * ld * (int64_t)sizeof(double);
+  double *dst = store;
+  if (!on_device) {
+    if (bytes > c->staging_bytes) {
+      RCHK(dev_alloc((void **)&c->d_staging, bytes));
+      c->staging_bytes = bytes;
+    }
+    dst = c->d_staging;
+    if (ld > F_TOTAL) HIPCHK(hipMemcpyAsync(dst, store, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->orig[c->cur],
+                     (int)c->n, dst, ld);
+  if (!on_device) HIPCHK(hipMemcpyAsync(store, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+// ================================================================== C-ABI
+extern "C" {
+
+int sph_default_options(sph_options *o) {
+  if (!o) return SPH_ERR_BAD_ARG;
+  memset(o, 0, sizeof(*o));
+  o->precision = 0;
+  o->tensor_mode = 0;
+  o->kappa_perp = 0.0;
+  o->max_sweeps = 100;
+  o->newton_tol = 0.0;
+  o->neigh_cap = 0;
+  o->device = 0;
+  return SPH_OK;
+}
+
+int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt, sph_ctx **out) {
+  if (!state || !out || kernel_id < 0 || kernel_id > 2) return SPH_ERR_BAD_ARG;
+  *out = nullptr;
+  int dim = (int)state[S_DIM];
+  if (dim < 1 || dim > 3) return SPH_ERR_BAD_ARG;
+  if ((int)state[S_COORD] == 2) return SPH_ERR_UNSUPPORTED;  // cylindrical, kernel.f90:135-148
+  sph_ctx *c = new sph_ctx();
+  if (opt) c->opt = *opt;
+  else sph_default_options(&c->opt);
+  if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1) {
+    delete c;
+    return SPH_ERR_BAD_ARG;
+  }
+  c->device = c->opt.device;
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SPH_ERR_HIP;
+  }
+  for (auto &e : c->ev) (void)hipEventCreate(&e);
+  c->kt = kernel_id;
+  Phys &P = c->frozen;
+  P.dim = dim;
+  P.kt = kernel_id;
+  P.ddw = (int)state[S_DDW];
+  P.adden = (int)state[S_ADDEN] == 1;
+  P.artts = (int)state[S_ARTTS] == 1;
+  P.au = state[S_AU];
+  P.tensor_mode = c->opt.tensor_mode;
+  P.kappa_perp = c->opt.kappa_perp;
+  eq_components(state, &P);
+  P.wcv = knorm(kernel_id, dim);
+  P.fwc = fwc_of(kernel_id, dim);
+  P.krad = krad_of(kernel_id);
+  c->maxneib = c->opt.neigh_cap > 0 ? c->opt.neigh_cap : maxneib_of(kernel_id, dim);
+  // process / legacy-setup branch of iterate (iterator.f90:60-178)
+  c->process = (int)state[S_PROCESS];
+  c->ics = (int)state[S_ICS];
+  int wrap = 0, ghost = 0;
+  c->do_eos = 1;
+  c->eos_iso = 0;
+  if (c->process == 601) { wrap = ghost = 7; }
+  else if (c->process == 602) { }
+  else if (c->process == 600) {
+    switch (c->ics) {
+      case 102: ghost = 2; break;
+      case 103: case 104: case 111: c->do_eos = 0; break;
+      case 105: wrap = ghost = 7; c->eos_iso = 1; break;
+      case 106: wrap = ghost = 6; break;
+      case 108: wrap = ghost = 7; break;
+      default: delete c; return SPH_ERR_UNSUPPORTED;  // mti gravity terms etc.
+    }
+  } else {
+    delete c;
+    return SPH_ERR_BAD_ARG;
+  }
+  // box and ghost bands, BC.f90:46-68,210-223
+  Box &b = c->box;
+  double bs = state[S_BORD], pad = state[S_PAD];
+  for (int a = 0; a < 3; a++) {
+    b.emin[a] = state[S_XMIN + 2 * a];
+    b.emax[a] = state[S_XMIN + 2 * a + 1];
+    b.boxmax[a] = b.emax[a] - bs;
+    b.boxmin[a] = b.emin[a] + bs;
+    b.pmin[a] = b.emin[a];
+    b.pmax[a] = b.emax[a];
+    if (pad == 0) { b.pmin[a] = b.emin[a] + pad / 10; b.pmax[a] = b.emax[a] - pad / 10; }
+  }
+  int dmask = (1 << dim) - 1;
+  b.wrap_mask = wrap & dmask;
+  b.ghost_mask = ghost & dmask;
+  c->n_real = (int64_t)state[S_REALPN];
+  c->n_fixed = (int64_t)state[S_FIXEDPN];
+  if (c->n_real < 0) { delete c; return SPH_ERR_BAD_ARG; }
+  if (dev_alloc((void **)&c->d_red, 64 * sizeof(long long)) != SPH_OK ||
+      dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
+      dev_alloc((void **)&c->d_pairs, 4 * sizeof(unsigned long long)) != SPH_OK ||
+      hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_ll, 64 * sizeof(long long)) != hipSuccess) {
+    sph_ctx_destroy(c);
+    return SPH_ERR_NOMEM;
+  }
+  if (c->opt.max_sweeps > 240) c->opt.max_sweeps = 240;
+  *out = c;
+  return SPH_OK;
+}
+
+int sph_ctx_destroy(sph_ctx *c) {
+  if (!c) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
+                  c->sort_tmp, c->cstart, c->cend, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->band,
+                  c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
+                  c->d_red, c->d_int, c->d_pairs, c->d_staging};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_int) (void)hipHostFree(c->h_int);
+  if (c->h_ll) (void)hipHostFree(c->h_ll);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SPH_OK;
+}
+
+static int check_counts(sph_ctx *c, int64_t n, const double *state) {
+  if (!state) return SPH_ERR_BAD_ARG;
+  int64_t nr = (int64_t)state[S_REALPN];
+  if (nr != c->n_real || n < nr) return SPH_ERR_BAD_ARG;
+  return SPH_OK;
+}
+
+int sph_load(sph_ctx *c, int64_t n, const double *store, int64_t ld, int on_device) {
+  if (!c) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  return load_impl(c, n, store, ld, on_device);
+}
+
+int sph_save(sph_ctx *c, double *store, int64_t ld, int on_device) {
+  if (!c) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  return save_impl(c, store, ld, on_device);
+}
+
+int sph_step(sph_ctx *c, const double state[49], double *maxconsenrg) {
+  if (!c || !state) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = run_step(c, state);
+  if (maxconsenrg) *maxconsenrg = 0.0;  // circuit2.f90:101
+  return rc;
+}
+
+int sph_iterate(sph_ctx *c, int64_t n, double *store, int64_t ld, const double state[49], double *maxconsenrg) {
+  if (!c) return SPH_ERR_BAD_ARG;
+  RCHK(check_counts(c, n, state));
+  RCHK(sph_load(c, n, store, ld, 0));
+  int rc = sph_step(c, state, maxconsenrg);
+  int rs = save_impl(c, store, ld, 0);
+  return rc != SPH_OK ? rc : rs;
+}
+
+int sph_iterate_device(sph_ctx *c, int64_t n, double *d_store, int64_t ld, const double state[49],
+                       double *maxconsenrg) {
+  if (!c) return SPH_ERR_BAD_ARG;
+  RCHK(check_counts(c, n, state));
+  RCHK(sph_load(c, n, d_store, ld, 1));
+  int rc = sph_step(c, state, maxconsenrg);
+  int rs = save_impl(c, d_store, ld, 1);
+  return rc != SPH_OK ? rc : rs;
+}
+
+int sph_neighbours(sph_ctx *c, int64_t *off, int32_t *j, int8_t *code, int64_t cap_pairs, int64_t *n_pairs) {
+  if (!c || !off || !c->have_step) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  const int n = (int)c->n;
+  int64_t nr = c->n_real;
+  int64_t *d_cnt = nullptr;
+  RCHK(dalloc(&d_cnt, nr + 1));
+  HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(int64_t) * (nr + 1), c->stream));
+  hipLaunchKernelGGL(k_counts_by_orig, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->cnt, c->orig[c->cur],
+                     c->real, n, d_cnt);
+  std::vector<int64_t> h(nr + 1);
+  HIPCHK(hipMemcpyAsync(h.data(), d_cnt, sizeof(int64_t) * nr, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  off[0] = 0;
+  for (int64_t i = 0; i < nr; i++) off[i + 1] = off[i] + h[i];
+  int64_t total = off[nr];
+  if (n_pairs) *n_pairs = total;
+  int rc = SPH_OK;
+  if (j && code) {
+    if (total > cap_pairs) rc = SPH_ERR_BAD_ARG;
+    else {
+      int32_t *dj = nullptr;
+      int8_t *dc = nullptr;
+      if (dalloc(&dj, total + 1) != SPH_OK || dalloc(&dc, total + 1) != SPH_OK) rc = SPH_ERR_NOMEM;
+      else {
+        HIPCHK(hipMemcpyAsync(d_cnt, off, sizeof(int64_t) * nr, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_neigh_export, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->nb, c->cnt,
+                           c->orig[c->cur], c->real, n, (int)c->npad, d_cnt, dj, dc);
+        HIPCHK(hipMemcpyAsync(j, dj, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(code, dc, total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
+      if (dj) (void)hipFree(dj);
+      if (dc) (void)hipFree(dc);
+    }
+  }
+  (void)hipFree(d_cnt);
+  return rc;
+}
+
+// timestep.calc, timestep.f90:40-166 (no FLD branch)
+static int calc_dt(sph_ctx *c, const Phys &P, double *dt) {
+  long long *init = c->h_ll + 24;
+  const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
+  init[0] = BIG; init[1] = SMALL; init[2] = BIG; init[3] = BIG; init[4] = SMALL; init[5] = SMALL;
+  HIPCHK(hipMemcpyAsync(c->d_red + 8, init, sizeof(long long) * 6, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_dt_reduce, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
+                     c->d_red + 8);
+  long long *r = c->h_ll + 32;
+  HIPCHK(hipMemcpyAsync(r, c->d_red + 8, sizeof(long long) * 6, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double minh = ord2d(r[0]), maxc = ord2d(r[1]), minden = ord2d(r[2]), minc = ord2d(r[3]);
+  double maxkappa = ord2d(r[4]), maxb = ord2d(r[5]);
+  double cnhydro, cnd2;
+  cn_of(c->kt, P.ddw, &cnhydro, &cnd2);
+  double d = HUGE_VAL;
+  if (P.hydro) d = std::min(d, cnhydro * minh / maxc);
+  if (P.magneto) d = std::min(d, .1 * minh / maxc / maxb);
+  if (P.diff) d = std::min(d, cnd2 * minden * minc * minh * minh / maxkappa);
+  if (P.fld) return SPH_ERR_UNSUPPORTED;
+  *dt = d;
+  return SPH_OK;
+}
+
+int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_used) {
+  if (!c || !state || c->n <= 0) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  Phys P = call_phys(c, state);
+  RCHK(run_step(c, state));  // main.f90:219
+  double dt;
+  RCHK(calc_dt(c, P, &dt));
+  dt = std::min(dt, dtcap);
+  if (dt_used) *dt_used = dt;
+  c->naux = 9;
+  hipLaunchKernelGGL(k_predict, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
+                     dt, 1, c->aux(0), (int)c->npad);
+  int rc = run_step(c, state);
+  if (rc == SPH_OK)
+    hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real,
+                       (int)c->n, dt, 1, c->aux(0), (int)c->npad);
+  c->naux = 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+// sts_integrator.f90:10-51
+static double sts_b(int j) { return j <= 2 ? 1. / 3. : (j * j + j - 2.) / (2. * j * (j + 1.)); }
+static double sts_a(int j) { return 1 - sts_b(j); }
+static double sts_mu(int j) { return (2. * j - 1.) / j * sts_b(j) / sts_b(j - 1); }
+static double sts_nu(int j) { return -(j - 1.) / j * sts_b(j) / sts_b(j - 2); }
+static double sts_w(int s) { return 4. / (s * s + s - 2.); }
+static double sts_mu_hat(int j, int s) { return j == 1 ? sts_b(1) * sts_w(s) : sts_mu(j) * sts_w(s); }
+static double sts_gm_hat(int j, int s) { return -sts_a(j - 1) * sts_mu_hat(j, s); }
+
+int sph_sts_coeffs(int j, int s, double out[4]) {
+  if (!out || j < 1 || s < 2) return SPH_ERR_BAD_ARG;
+  out[0] = j >= 2 ? sts_mu(j) : 0.0;
+  out[1] = j >= 2 ? sts_nu(j) : 0.0;
+  out[2] = sts_mu_hat(j, s);
+  out[3] = j >= 2 ? sts_gm_hat(j, s) : 0.0;
+  return SPH_OK;
+}
+
+// sts_integrate, sts_integrator.f90:87-124: aux arrays 0 Y0, 1 Yjm1, 2 Yjm2,
+// 3 DY0, 4 DYjm1, 5 Yj (permuted with the particles at every re-sort).
+int sph_sts(sph_ctx *c, const double state[49], int s, double dt) {
+  if (!c || !state || c->n <= 0 || s < 2) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  const int n = (int)c->n;
+  unsigned G = grid_for(n);
+  c->naux = 6;
+  auto T = [&]() { return c->fields().f[F_T]; };
+  auto DDT = [&]() { return c->fields().f[F_DDT]; };
+  HIPCHK(hipMemcpyAsync(c->aux(0), T(), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  int rc = run_step(c, state);
+  if (rc != SPH_OK) { c->naux = 0; return rc; }
+  HIPCHK(hipMemcpyAsync(c->aux(3), DDT(), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->aux(2), c->aux(0), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_axpy_copy, dim3(G), dim3(kBlock), 0, c->stream, c->aux(1), c->aux(0), c->aux(3),
+                     sts_mu_hat(1, s) * dt, n);
+  for (int j = 2; j <= s; j++) {
+    HIPCHK(hipMemcpyAsync(T(), c->aux(1), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    rc = run_step(c, state);
+    if (rc != SPH_OK) { c->naux = 0; return rc; }
+    HIPCHK(hipMemcpyAsync(c->aux(4), DDT(), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    double mu = sts_mu(j), nu = sts_nu(j);
+    hipLaunchKernelGGL(k_sts_stage, dim3(G), dim3(kBlock), 0, c->stream, T(), c->aux(0), c->aux(1), c->aux(2),
+                       c->aux(3), c->aux(4), n, mu, nu, sts_mu_hat(j, s) * dt, sts_gm_hat(j, s) * dt, c->aux(5));
+    HIPCHK(hipMemcpyAsync(c->aux(2), c->aux(1), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->aux(1), c->aux(5), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(T(), c->aux(5), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->naux = 0;
+  return SPH_OK;
+}
+
+int sph_get_stats(sph_ctx *c, sph_stats *s) {
+  if (!c || !s) return SPH_ERR_BAD_ARG;
+  *s = c->stats;
+  s->n_real = c->n_real;
+  s->n_fixed = c->n_fixed;
+  return SPH_OK;
+}
+
+const char *sph_error_string(int code) {
+  switch (code) {
+    case SPH_OK: return "ok";
+    case SPH_ERR_NEIGH_TRUNCATED: return "# <!> KDTree neibsearch result was truncated.";
+    case SPH_ERR_NEG_H: return "Negative smoothing length in c1 advanced";
+    case SPH_ERR_Q_NAN_OR_NEG: return "q is nan or negative";
+    case SPH_ERR_FLD_ANISO: return "There is no anisotropic for FLD";
+    case SPH_ERR_BAD_DDW: return "# <!> second deriv id not found";
+    case SPH_ERR_PRECONDITION: return "pre-run check failed";
+    case SPH_ERR_BAD_ARG: return "invalid argument";
+    case SPH_ERR_HIP: return "HIP runtime error";
+    case SPH_ERR_NOMEM: return "device allocation failed";
+    case SPH_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown error";
+  }
+}
+
+}  // extern "C"

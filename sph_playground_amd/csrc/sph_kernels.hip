@@ -1,0 +1,856 @@
+// Device kernels of the two-loop hot path for gfx950.
+//
+// Data layout (HBM, "sorted order" = particles ordered by cell key, x fastest):
+//   fields     32 FP64 SoA arrays, one per store row (const.f90:69-103)
+//   pm         double4 {x, y, z, m}: the one 32-byte gather every pair makes
+//   st (x2)    loop-1 Newton state {h, den, om, T}, ping-pong per sweep
+//   q0..q4     loop-2 j-side packs (double4 for FP64, float4 for mixed)
+//   nb         neighbour entries, transposed [k][s] so lane s of a wavefront
+//              reads entry k of 64 consecutive particles in one coalesced load;
+//              entry = sorted index << 5 | periodic image code
+//
+// Every kernel maps one particle per lane (64-lane wavefronts, 256 per block)
+// and remaps blockIdx so each XCD works on a contiguous particle range (their
+// neighbours then hit that XCD's own L2).
+#include <hip/hip_runtime.h>
+
+#include "sph_kernels.h"
+
+namespace sph {
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  // blocks are dispatched round-robin over 8 XCDs; give XCD x the contiguous
+  // logical range [x*q + min(x,r), ...) (a bijection on [0, nb))
+  int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ int gtid() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
+
+// Order-preserving map of doubles onto int64 (for atomicMin/atomicMax).
+__device__ __forceinline__ long long d2ord(double v) {
+  long long b = __double_as_longlong(v);
+  return b >= 0 ? b : (b ^ 0x7FFFFFFFFFFFFFFFLL);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ long long wave_min(long long v) {
+  for (int o = 32; o > 0; o >>= 1) { long long w = __shfl_xor(v, o, 64); v = w < v ? w : v; }
+  return v;
+}
+__device__ __forceinline__ long long wave_max(long long v) {
+  for (int o = 32; o > 0; o >>= 1) { long long w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
+  return v;
+}
+
+// ---------------------------------------------------------------- set-up
+// reflectRealParticlesPeriodically (BC.f90:102-194) for real particles, then
+// the bounding box of all particles and the largest h of the real ones.
+// red[0..2] min, red[3..5] max (ordered int64), red[6] max h.
+__global__ void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, int dim,
+                              long long *red, int *err) {
+  int s = gtid();
+  long long mn[3], mx[3], hm;
+  for (int a = 0; a < 3; a++) { mn[a] = 0x7FFFFFFFFFFFFFFFLL; mx[a] = -0x7FFFFFFFFFFFFFFFLL - 1; }
+  hm = mx[0];
+  if (s < n) {
+    bool real = orig[s] < n_real;
+    double x[3] = {F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s]};
+    if (real && b.wrap_mask) {
+      double dxmin[3], dxmax[3];
+      for (int a = 0; a < 3; a++) { dxmin[a] = b.emin[a] - x[a]; dxmax[a] = x[a] - b.emax[a]; }
+      for (int a = 0; a < dim; a++) {
+        if (!((b.wrap_mask >> a) & 1)) continue;
+        double nx = x[a];
+        if (dxmax[a] > 0) nx = b.emin[a] + dxmax[a];
+        if (dxmin[a] > 0) nx = b.emax[a] - dxmin[a];
+        if (nx != x[a]) { x[a] = nx; F.f[F_RX + a][s] = nx; }
+      }
+    }
+    for (int a = 0; a < dim; a++)
+      if (((b.ghost_mask >> a) & 1) && !(x[a] >= b.emin[a] && x[a] <= b.emax[a])) atomicOr(err, EB_OUTSIDE);
+    for (int a = 0; a < 3; a++) { mn[a] = mx[a] = d2ord(x[a]); }
+    if (real) hm = d2ord(F.f[F_H][s]);
+  }
+  for (int a = 0; a < 3; a++) { mn[a] = wave_min(mn[a]); mx[a] = wave_max(mx[a]); }
+  hm = wave_max(hm);
+  if ((threadIdx.x & 63) == 0) {
+    for (int a = 0; a < 3; a++) { atomicMin(&red[a], mn[a]); atomicMax(&red[3 + a], mx[a]); }
+    atomicMax(&red[6], hm);
+  }
+}
+
+__global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) {
+  int s = gtid();
+  if (s >= n) return;
+  double x[3] = {F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s]};
+  int c[3];
+  for (int a = 0; a < 3; a++) {
+    double t = floor((x[a] - g.lo[a]) * g.inv_cs[a]);
+    int ci = (t < 0.0) ? 0 : (t >= (double)g.nc[a] ? g.nc[a] - 1 : (int)t);
+    c[a] = ci;
+  }
+  keys[s] = (uint32_t)((c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]);
+  vals[s] = (uint32_t)s;
+}
+
+__global__ void k_check_sorted(const uint32_t *keys, int n, int *flag) {
+  int s = gtid();
+  bool bad = (s + 1 < n) && keys[s] > keys[s + 1];
+  if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+// Gather every field into the new order: dst[s] = src[perm[s]].
+__global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const int64_t *orig_src,
+                          int64_t *orig_dst, int n) {
+  int s = gtid();
+  if (s >= n) return;
+  uint32_t p = perm[s];
+#pragma unroll 4
+  for (int f = 0; f < F_TOTAL; f++) dst.f[f][s] = src.f[f][p];
+  orig_dst[s] = orig_src[p];
+}
+
+__global__ void k_cell_range(const uint32_t *keys, int n, int *cstart, int *cend) {
+  int s = gtid();
+  if (s >= n) return;
+  uint32_t k = keys[s];
+  if (s == 0 || keys[s - 1] != k) cstart[k] = s;
+  if (s == n - 1 || keys[s + 1] != k) cend[k] = s + 1;
+}
+
+// Positions/masses, search h, ghost-band codes (BC.f90:196-309: strict
+// padding test for a particle's own band, the looser one a chained ghost
+// gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
+__global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, int dim, double4 *pm,
+                       double *h0, uint16_t *band, uint8_t *real, double4 *st, double *eps) {
+  int s = gtid();
+  if (s >= n) return;
+  double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
+  pm[s] = make_double4(x, y, z, F.f[F_M][s]);
+  double h = F.f[F_H][s];
+  h0[s] = h;
+  bool r = orig[s] < n_real;
+  real[s] = r ? 1 : 0;
+  st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
+  eps[s] = r ? 1.0 : 0.0;
+  int ty = (int)F.f[F_TYPE][s];
+  uint32_t code = 0;
+  if (ty == 1 || ty == 2 || ty == 21) {
+    double xs[3] = {x, y, z};
+    for (int a = 0; a < dim; a++) {
+      if (!((b.ghost_mask >> a) & 1)) continue;
+      double v = xs[a];
+      uint32_t st_ = 0, lo = 0;
+      if (v < b.boxmin[a]) {
+        if (v > b.pmin[a]) st_ = 1;
+        lo = 1;
+      } else if (v > b.boxmax[a]) {
+        if (v < b.pmax[a]) st_ = 2;
+        lo = 2;
+      }
+      code |= (st_ << (2 * a)) | (lo << (6 + 2 * a));
+    }
+  }
+  band[s] = (uint16_t)code;
+}
+
+// Does the reference hold a ghost of j shifted by (sx,sy,sz)?  The first
+// shifted axis uses j's own (strict) band, later axes the chained test.
+__device__ __forceinline__ bool image_exists(uint32_t band, int sx, int sy, int sz) {
+  int sh[3] = {sx, sy, sz};
+  bool first = true;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (sh[a] == 0) continue;
+    uint32_t want = sh[a] > 0 ? 1u : 2u;
+    uint32_t got = first ? (band >> (2 * a)) & 3u : (band >> (6 + 2 * a)) & 3u;
+    if (got != want) return false;
+    first = false;
+  }
+  return true;
+}
+
+// Squared separation exactly as kdtree2.f90:1673-1677 rounds it:
+// sd = ((0 + dx^2) + dy^2) + dz^2 with dx = x_j - x_i, no fused multiply-add.
+__device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, double xi, double yi, double zi) {
+#pragma clang fp contract(off)
+  double d = xj - xi;
+  double sd = d * d;
+  d = yj - yi;
+  sd = sd + d * d;
+  d = zj - zi;
+  sd = sd + d * d;
+  return sd;
+}
+
+// findneighboursKDT (neighboursearch.f90:74-194) as a cell-list search:
+// every real i tests the particles of the 3^DIM surrounding cells (cells wrap
+// with an image shift along imaged axes) with the kd-tree's FP64 inclusion
+// rule sd <= (krad*h_i)^2; self (unshifted) is dropped.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
+  int s = gtid();
+  int count = 0;
+  bool live = s < a.n && a.real[s];
+  if (live) {
+    double4 pi = a.pm[s];
+    double rr = a.krad * a.hs[s];
+    double r2 = rr * rr;
+    int ci[3];
+    {
+      double x[3] = {pi.x, pi.y, pi.z};
+      for (int d = 0; d < 3; d++) {
+        double t = floor((x[d] - a.g.lo[d]) * a.g.inv_cs[d]);
+        ci[d] = (t < 0.0) ? 0 : (t >= (double)a.g.nc[d] ? a.g.nc[d] - 1 : (int)t);
+      }
+    }
+    const int zlo = DIM == 3 ? -1 : 0, zhi = DIM == 3 ? 1 : 0;
+    const int ylo = DIM >= 2 ? -1 : 0, yhi = DIM >= 2 ? 1 : 0;
+    for (int dz = zlo; dz <= zhi; dz++) {
+      int cz = ci[2] + dz, sz = 0;
+      if (a.g.periodic[2]) {
+        if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
+        else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
+      } else if (cz < 0 || cz >= a.g.nc[2]) continue;
+      for (int dy = ylo; dy <= yhi; dy++) {
+        int cy = ci[1] + dy, sy = 0;
+        if (a.g.periodic[1]) {
+          if (cy < 0) { cy += a.g.nc[1]; sy = -1; }
+          else if (cy >= a.g.nc[1]) { cy -= a.g.nc[1]; sy = 1; }
+        } else if (cy < 0 || cy >= a.g.nc[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          int cx = ci[0] + dx, sx = 0;
+          if (a.g.periodic[0]) {
+            if (cx < 0) { cx += a.g.nc[0]; sx = -1; }
+            else if (cx >= a.g.nc[0]) { cx -= a.g.nc[0]; sx = 1; }
+          } else if (cx < 0 || cx >= a.g.nc[0]) continue;
+          int c = (cz * a.g.nc[1] + cy) * a.g.nc[0] + cx;
+          uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+          int t0 = a.cstart[c], t1 = a.cend[c];
+          if (code == kNoImage) {
+            for (int t = t0; t < t1; t++) {
+              double4 pj = a.pm[t];
+              double sd = kd_sqdist(pj.x, pj.y, pj.z, pi.x, pi.y, pi.z);
+              if (sd <= r2 && t != s) {
+                if (count < a.cap) a.nb[(size_t)count * a.npad + s] = nb_pack((uint32_t)t, code);
+                count++;
+              }
+            }
+          } else {
+            for (int t = t0; t < t1; t++) {
+              if (!image_exists(a.band[t], sx, sy, sz)) continue;
+              double4 pj = a.pm[t];
+              double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
+              double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
+              double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
+              double sd = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z);
+              if (sd <= r2) {
+                if (count < a.cap) a.nb[(size_t)count * a.npad + s] = nb_pack((uint32_t)t, code);
+                count++;
+              }
+            }
+          }
+        }
+      }
+    }
+    a.cnt[s] = count;
+    if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
+    if (count > a.cap) atomicOr(a.err, EB_OVERFLOW);
+  } else if (s < a.n) {
+    a.cnt[s] = 0;
+  }
+  int m = count;
+  for (int o = 32; o > 0; o >>= 1) { int w = __shfl_xor(m, o, 64); m = w > m ? w : m; }
+  unsigned long long tot = wave_sum((unsigned long long)count);
+  if ((threadIdx.x & 63) == 0) {
+    if (m) atomicMax(a.maxcnt, m);
+    if (tot) atomicAdd(a.pairs, tot);
+  }
+}
+
+// ---------------------------------------------------------------- loop 1
+// One Jacobi sweep of c1advanced (circuit1.f90:114-211) for the particles
+// still active (eps > tol); frozen ones carry their state to the next buffer.
+// GRAD: 0 no grad T, 1 standard (:178-179), 2 the 2nw_sd form (:174-176).
+// PT is the pair arithmetic type; the Newton update is always FP64.
+template <int DIM, int KT, int GRAD, typename PT>
+__global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
+  if (*a.prev_active == 0) return;  // converged earlier: nothing to do
+  int s = gtid();
+  bool inb = s < a.n;
+  double4 c = inb ? a.cur[s] : make_double4(0, 0, 0, 0);
+  bool act = inb && a.real[s] && a.eps[s] > a.tol;
+  bool still = false;
+  if (inb && !act) a.next[s] = c;
+  if (act) {
+    using K = KernelFn<KT, PT>;
+    const double ha = c.x, rhoa = c.y, oma = c.z, ta = c.w;
+    const double4 pa = a.pm[s];
+    const double ma = pa.w;
+    const PT inv_ha = (PT)(1.0 / ha);
+    PT dsum = 0, osum = 0;
+    PT g0 = 0, g1 = 0, g2 = 0, h0x = 0, h0y = 0, h0z = 0;
+    bool qbad = !(ha > 0.0);
+    const int nn = a.cnt[s];
+    const uint32_t *e = a.nb + s;
+    for (int k = 0; k < nn; k++) {
+      uint32_t ent = e[(size_t)k * a.npad];
+      uint32_t j = nb_index(ent), code = nb_code(ent);
+      double4 pb = a.pm[j];
+      double xb = pb.x, yb = pb.y, zb = pb.z;
+      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+      PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
+      PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
+      PT dr = sqrt(rsq);
+      PT q = dr * inv_ha;
+      PT f, df;
+      K::f_df(q, f, df);
+      PT mb = (PT)pb.w;
+      dsum += mb * f;
+      osum += mb * (PT(DIM) * f + q * df);
+      if constexpr (GRAD != 0) {
+        double4 sb = a.cur[j];  // hb, rhob, omb, tb of the previous sweep
+        PT inv_dr = PT(1) / dr;
+        if constexpr (GRAD == 1) {
+          PT w = mb * ((PT)sb.w * (PT)sb.y - (PT)ta * (PT)rhoa) * df * inv_dr;
+          g0 += w * r0; g1 += w * r1; g2 += w * r2;
+        } else {
+          PT inv_hb = (PT)(1.0 / sb.x);
+          PT qb = dr * inv_hb;
+          PT dfb = K::df(qb);
+          PT wa = mb * df * inv_dr;
+          PT wb = mb * (PT)sb.w * (PT)rhoa / ((PT)sb.z * (PT)sb.y * (PT)sb.y) * dfb * inv_dr *
+                  ipow<DIM + 1>(inv_hb);
+          g0 += wa * r0; g1 += wa * r1; g2 += wa * r2;
+          h0x += wb * r0; h0y += wb * r1; h0z += wb * r2;
+        }
+      }
+      qbad |= !(q >= PT(0));
+    }
+    const double wcv = a.p.wcv;
+    const double ih = 1.0 / ha;
+    const double cW = wcv * ipow<DIM>(ih);
+    const double cH = -wcv * ipow<DIM + 1>(ih);
+    double f0, df0;
+    KernelFn<KT, double>::f_df(0.0, f0, df0);
+    double den = cW * (double)dsum + ma * (cW * f0);
+    double osm = cH * (double)osum + ma * (cH * (DIM * f0));
+    osm = 1. - osm * (-ha / (DIM * den));
+    double dfdh = -DIM * den * osm / ha;
+    double fh = ma * ipow<DIM>(a.p.hfac / ha) - den;
+    double hn = ha - fh / dfdh;
+    if (hn <= 0.) atomicOr(a.err, EB_NEG_H);
+    if (qbad) atomicOr(a.err, EB_Q);
+    double ep = fabs(hn - ha) / a.h0[s];
+    a.eps[s] = ep;
+    still = ep > a.tol;
+    a.next[s] = make_double4(hn, den, osm, ta);
+    double gx = 0, gy = 0, gz = 0;
+    if constexpr (GRAD == 1) {
+      double cN = wcv * ipow<DIM + 1>(ih) / rhoa / oma;
+      gx = cN * (double)g0; gy = cN * (double)g1; gz = cN * (double)g2;
+    } else if constexpr (GRAD == 2) {
+      double cA = wcv * ipow<DIM + 1>(ih) * ta / (oma * rhoa);
+      gx = cA * (double)g0 + wcv * (double)h0x;
+      gy = cA * (double)g1 + wcv * (double)h0y;
+      gz = cA * (double)g2 + wcv * (double)h0z;
+    }
+    a.dtdx[s] = gx; a.dtdy[s] = gy; a.dtdz[s] = gz;
+  }
+  // wave-aggregated counts (every lane of the wavefront reaches this point)
+  unsigned long long m = __ballot(still);
+  unsigned long long pairs = wave_sum((unsigned long long)(act ? a.cnt[s] : 0));
+  if ((threadIdx.x & 63) == 0) {
+    if (m) atomicAdd(a.active, __popcll(m));
+    if (pairs) atomicAdd(a.pairs, pairs);
+  }
+}
+
+// c1simple (circuit1.f90:227-305): direct summation, h = hfac (m/rho)^(1/dim),
+// grad T only for 2nw_ds (:282-284) / 2nw_sd (:285-289), om = 1 (:62,301).
+template <int DIM, int KT, int GRAD, typename PT>
+__global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fields F) {
+  int s = gtid();
+  if (s >= a.n) return;
+  F.f[F_OM][s] = 1.0;
+  if (!a.real[s]) return;
+  using K = KernelFn<KT, PT>;
+  const double4 c = a.cur[s];
+  const double ha = c.x, rhoa = c.y, ta = c.w;
+  const double4 pa = a.pm[s];
+  const PT inv_ha = (PT)(1.0 / ha);
+  PT dsum = 0, g0 = 0, g1 = 0, g2 = 0, h0x = 0, h0y = 0, h0z = 0;
+  bool qbad = !(ha > 0.0);
+  const int nn = a.cnt[s];
+  for (int k = 0; k < nn; k++) {
+    uint32_t ent = a.nb[(size_t)k * a.npad + s];
+    uint32_t j = nb_index(ent), code = nb_code(ent);
+    double4 pb = a.pm[j];
+    double xb = pb.x, yb = pb.y, zb = pb.z;
+    if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+    PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
+    PT dr = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    PT q = dr * inv_ha;
+    PT mb = (PT)pb.w;
+    if constexpr (GRAD == 0) {
+      dsum += mb * K::f(q);
+    } else {
+      PT f, df;
+      K::f_df(q, f, df);
+      dsum += mb * f;
+      double4 sb = a.cur[j];
+      PT inv_dr = PT(1) / dr;
+      if constexpr (GRAD == 1) {  // 2nw_ds: m_b/rho_a (T_b - T_a) nabla W_a
+        PT w = mb * ((PT)sb.w - (PT)ta) * df * inv_dr;
+        g0 += w * r0; g1 += w * r1; g2 += w * r2;
+      } else {  // 2nw_sd: m_b (T_a/rho_a^2 nabla W_a + T_b/rho_b^2 nabla W_b)
+        PT inv_hb = (PT)(1.0 / sb.x);
+        PT dfb = K::df(dr * inv_hb);
+        PT wa = mb * df * inv_dr;
+        PT wb = mb * (PT)sb.w / ((PT)sb.y * (PT)sb.y) * dfb * inv_dr * ipow<DIM + 1>(inv_hb);
+        g0 += wa * r0; g1 += wa * r1; g2 += wa * r2;
+        h0x += wb * r0; h0y += wb * r1; h0z += wb * r2;
+      }
+    }
+    qbad |= !(q >= PT(0));
+  }
+  if (qbad) atomicOr(a.err, EB_Q);
+  const double wcv = a.p.wcv, ih = 1.0 / ha;
+  double f0 = KernelFn<KT, double>::f(0.0);
+  double den = wcv * ipow<DIM>(ih) * (double)dsum + pa.w * (wcv * ipow<DIM>(ih) * f0);
+  F.f[F_DEN][s] = den;
+  F.f[F_H][s] = a.p.hfac * pow(pa.w / den, 1. / DIM);
+  double gx = 0, gy = 0, gz = 0;
+  if constexpr (GRAD == 1) {
+    double cN = wcv * ipow<DIM + 1>(ih) / rhoa;
+    gx = cN * (double)g0; gy = cN * (double)g1; gz = cN * (double)g2;
+  } else if constexpr (GRAD == 2) {
+    double cA = wcv * ipow<DIM + 1>(ih) * ta / (rhoa * rhoa);
+    gx = cA * (double)g0 + wcv * (double)h0x;
+    gy = cA * (double)g1 + wcv * (double)h0y;
+    gz = cA * (double)g2 + wcv * (double)h0z;
+  }
+  F.f[F_DTDX][s] = gx; F.f[F_DTDY][s] = gy; F.f[F_DTDZ][s] = gz;
+}
+
+// After loop 1: write back h/den/om of real particles (circuit1.f90:213-215),
+// the EOS (eos.f90:21-45) and the loop-2 j-side packs.
+template <typename V4>
+__global__ void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n, int advanced, int do_eos,
+                        int iso, double c1, double gamma, V4 *q0, V4 *q1, V4 *q2, V4 *q3, V4 *q4) {
+  int s = gtid();
+  if (s >= n) return;
+  using T = decltype(V4().x);
+  bool r = real[s];
+  if (r && advanced) {
+    double4 v = fin[s];
+    F.f[F_H][s] = v.x; F.f[F_DEN][s] = v.y; F.f[F_OM][s] = v.z;
+  }
+  double den = F.f[F_DEN][s];
+  if (r && do_eos) {
+    if (iso) {
+      F.f[F_P][s] = den * c1 * c1;
+    } else {
+      double p = (gamma - 1.) * den * F.f[F_U][s];
+      F.f[F_P][s] = p;
+      F.f[F_C][s] = sqrt(gamma * p / den);
+    }
+  }
+  q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_P][s]};
+  if (q1) q1[s] = V4{(T)F.f[F_VX][s], (T)F.f[F_VY][s], (T)F.f[F_VZ][s], (T)F.f[F_C][s]};
+  if (q2) q2[s] = V4{(T)F.f[F_U][s], (T)F.f[F_T][s], (T)F.f[F_KAPPA][s], (T)0};
+  if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
+  if (q4) q4[s] = V4{(T)F.f[F_BX][s], (T)F.f[F_BY][s], (T)F.f[F_BZ][s], (T)0};
+}
+
+// ---------------------------------------------------------------- loop 2
+template <typename T>
+struct Sym3 {
+  T xx, yy, zz, xy, xz, yz;
+};
+
+// Diffusion tensor of one particle from its unit field direction:
+// reference modes (circuit2.f90:194-206, 370-374) or the full tensor.
+template <typename T>
+__device__ __forceinline__ Sym3<T> cond_tensor(const Phys &p, T ux, T uy, T uz) {
+  Sym3<T> k;
+  if (p.diso == 1) {
+    k.xx = k.yy = k.zz = (T)p.dcond; k.xy = k.xz = k.yz = 0;
+  } else if (p.tensor_mode == 1) {
+    T kp = (T)p.dcond, kq = (T)p.kappa_perp, d = kp - kq;
+    k.xx = d * ux * ux + kq; k.yy = d * uy * uy + kq; k.zz = d * uz * uz + kq;
+    k.xy = d * ux * uy; k.xz = d * ux * uz; k.yz = d * uy * uz;
+  } else {
+    T kp = (T)p.dcond;
+    k.xx = kp * ux * ux; k.yy = kp * uy * uy; k.zz = kp * uz * uz; k.xy = k.xz = k.yz = 0;
+  }
+  return k;
+}
+
+// c2 (circuit2.f90:58-505) for one real particle per lane over the loop-1
+// neighbour list, post-loop-1 h/den/om and post-EOS p/c (Q2 of the spec).
+template <int DIM, int KT, typename PT, typename V4>
+__global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
+  int s = gtid();
+  double e_part = 0.0;
+  if (s < a.n && a.real[s]) {
+    using K = KernelFn<KT, PT>;
+    const Phys &P = a.p;
+    const bool need_v = P.adden || P.artts || P.hydro || P.magneto;
+    const bool need_q3 = P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
+    const bool need_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
+    const bool need_q2 = P.diff || (P.artts && P.au != -1.0);
+    const bool hess = P.diff && (P.ddw == 400 || P.ddw == 401 || P.ddw == 402);
+    const double4 pa = a.pm[s];
+    // i-side values straight from the FP64 fields (coalesced, read once)
+    const double *const *Fi = a.out.f;
+    const double ha_d = Fi[F_H][s];
+    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)Fi[F_DEN][s], oma = (PT)Fi[F_OM][s], pa_ = (PT)Fi[F_P][s];
+    const PT vax = (PT)Fi[F_VX][s], vay = (PT)Fi[F_VY][s], vaz = (PT)Fi[F_VZ][s], ca = (PT)Fi[F_C][s];
+    const PT ua = (PT)Fi[F_U][s], ta = (PT)Fi[F_T][s], kappaa = (PT)Fi[F_KAPPA][s];
+    const PT gax = (PT)Fi[F_DTDX][s], gay = (PT)Fi[F_DTDY][s], gaz = (PT)Fi[F_DTDZ][s];
+    const PT bax = (PT)Fi[F_BX][s], bay = (PT)Fi[F_BY][s], baz = (PT)Fi[F_BZ][s];
+    // i-side constants
+    const PT wcv = (PT)P.wcv;
+    const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
+    const PT rho2oma = rhoa * rhoa * oma;
+    PT ub2 = bax * bax + bay * bay + baz * baz;
+    PT ubax = 0, ubay = 0, ubaz = 0;
+    if (ub2 != PT(0)) { PT r = PT(1) / sqrt(ub2); ubax = bax * r; ubay = bay * r; ubaz = baz * r; }
+    Sym3<PT> kta = {0, 0, 0, 0, 0, 0}, ktb_fixed = {0, 0, 0, 0, 0, 0};
+    PT mu0inv = P.magneto ? (PT)(1.0 / P.muzero) : PT(0);
+    if (P.diff) {
+      if (P.fld) {
+        PT gn = sqrt(gax * gax + gay * gay + gaz * gaz);
+        PT fr = gn / (kappaa * rhoa * ta * rhoa);
+        PT lam = (PT(2) + fr) / (PT(6) + PT(3) * fr + fr * fr);
+        PT d = (PT)kLightspeed * lam / kappaa / rhoa;
+        kta.xx = kta.yy = kta.zz = d;
+        a.out.f[F_VFLD][s] = (double)(d / ((ta * rhoa) / gn));
+      } else {
+        kta = cond_tensor<PT>(P, ubax, ubay, ubaz);
+        if (P.diso == 1) ktb_fixed = kta;
+      }
+    }
+    // kappa_a grad T_a for the 2nw forms (circuit2.f90:401-403)
+    PT kdax = kta.xx * gax + kta.xy * gay + kta.xz * gaz;
+    PT kday = kta.xy * gax + kta.yy * gay + kta.yz * gaz;
+    PT kdaz = kta.xz * gax + kta.yz * gay + kta.zz * gaz;
+    PT dvx = 0, dvy = 0, dvz = 0, du = 0, ddt = 0, dbx = 0, dby = 0, dbz = 0, drho = 0;
+    bool qbad = !(inv_ha > PT(0));
+    const int nn = a.cnt[s];
+    for (int k = 0; k < nn; k++) {
+      uint32_t ent = a.nb[(size_t)k * a.npad + s];
+      uint32_t j = nb_index(ent), code = nb_code(ent);
+      double4 pb = a.pm[j];
+      double xb = pb.x, yb = pb.y, zb = pb.z;
+      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+      const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
+      const PT mb = (PT)pb.w;
+      const V4 B0 = a.q0[j];
+      const PT inv_hb = (PT)B0.x, rhob = (PT)B0.y, omb = (PT)B0.z, pbb = (PT)B0.w;
+      const PT r2 = rx * rx + ry * ry + rz * rz;
+      const PT dr = sqrt(r2);
+      const PT inv_dr = PT(1) / dr;
+      const PT qa = dr * inv_ha, qb = dr * inv_hb;
+      PT dfa, dfb;
+      PT ddfa = 0, fa = 0;
+      if (hess && P.ddw == 401) K::df_ddf(qa, dfa, ddfa);
+      else if (hess && P.ddw == 402) K::f_df(qa, fa, dfa);
+      else dfa = K::df(qa);
+      dfb = K::df(qb);
+      qbad |= !(qb >= PT(0));
+      // nabla W = wcv df r / (h^(dim+1) dr)   (kernel.f90:189-197)
+      const PT ga = wcv * dfa * ihd1 * inv_dr;
+      const PT gb = wcv * dfb * ipow<DIM + 1>(inv_hb) * inv_dr;
+      const PT nwax = ga * rx, nway = ga * ry, nwaz = ga * rz;
+      const PT nwbx = gb * rx, nwby = gb * ry, nwbz = gb * rz;
+      PT vabx = 0, vaby = 0, vabz = 0, cb = 0;
+      if (need_v) {
+        V4 t = a.q1[j];
+        vabx = vax - (PT)t.x; vaby = vay - (PT)t.y; vabz = vaz - (PT)t.z; cb = (PT)t.w;
+      }
+      const PT vnwa = vabx * nwax + vaby * nway + vabz * nwaz;
+      if (P.adden) drho += mb * vnwa;
+      PT ub = 0, tb = 0, kappab = 0;
+      if (need_q2) { V4 t = a.q2[j]; ub = (PT)t.x; tb = (PT)t.y; kappab = (PT)t.z; }
+      if (P.artts) {
+        // Fab = wcv df dr / h^(dim+2) / q = wcv df / h^(dim+1)   (kernel.f90:251-259)
+        const PT faba = wcv * dfa * ihd1;
+        const PT fabb = wcv * dfb * ipow<DIM + 1>(inv_hb);
+        const PT dvr = (vabx * rx + vaby * ry + vabz * rz) * inv_dr;
+        PT qva = 0, qvb = 0;
+        if (dvr < PT(0)) {  // art_viscosity, circuit2.f90:558-575
+          qva = PT(-0.5) * rhoa * (ca + PT(2) * fabs(dvr)) * dvr;
+          qvb = PT(-0.5) * rhob * (cb + PT(2) * fabs(dvr)) * dvr;
+        }
+        const PT ta_ = qva / (rhoa * rhoa) / oma, tb_ = qvb / (rhob * rhob) / omb;
+        dvx -= mb * (ta_ * nwax + tb_ * nwbx);
+        dvy -= mb * (ta_ * nway + tb_ * nwby);
+        dvz -= mb * (ta_ * nwaz + tb_ * nwbz);
+        PT qc = 0;
+        if (P.au != -1.0) {  // art_termcond, circuit2.f90:520-529
+          PT vs = sqrt(fabs(pa_ - pbb) / (PT(0.5) * (rhoa + rhob)));
+          qc = (PT)P.au * (vs * (ua - ub) * PT(0.5) * (faba / oma / rhoa + fabb / omb / rhob));
+        }
+        du += mb * (dvr * qva * faba / oma / rhoa + qc);
+      }
+      if (P.hydro) {
+        const PT ta_ = pa_ / rho2oma, tb_ = pbb / (rhob * rhob * omb);
+        dvx -= mb * (ta_ * nwax + tb_ * nwbx);
+        dvy -= mb * (ta_ * nway + tb_ * nwby);
+        dvz -= mb * (ta_ * nwaz + tb_ * nwbz);
+        du += mb * pa_ * vnwa / (rhoa * rhoa) / oma;
+      }
+      PT bbx = 0, bby = 0, bbz = 0;
+      if (need_q4) { V4 t = a.q4[j]; bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z; }
+      if (P.diff) {
+        Sym3<PT> ktb = ktb_fixed;
+        PT gbx = 0, gby = 0, gbz = 0;
+        if (need_q3) { V4 t = a.q3[j]; gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z; }
+        if (P.fld) {
+          PT gn = sqrt(gbx * gbx + gby * gby + gbz * gbz);
+          PT fr = gn / (kappab * rhob * rhob * tb);
+          PT lam = (PT(2) + fr) / (PT(6) + PT(3) * fr + fr * fr);
+          PT d = (PT)kLightspeed * lam / kappab / rhob;
+          ktb.xx = ktb.yy = ktb.zz = d; ktb.xy = ktb.xz = ktb.yz = 0;
+        } else if (P.diso == 0) {
+          PT b2 = bbx * bbx + bby * bby + bbz * bbz;
+          PT ux = 0, uy = 0, uz = 0;
+          if (b2 != PT(0)) { PT r = PT(1) / sqrt(b2); ux = bbx * r; uy = bby * r; uz = bbz * r; }
+          ktb = cond_tensor<PT>(P, ux, uy, uz);
+        }
+        if (hess) {
+          // sum_lm ((kta+ktb)/2)_lm H_lm over the dim x dim block
+          Sym3<PT> kt;
+          kt.xx = (kta.xx + ktb.xx) * PT(0.5); kt.yy = (kta.yy + ktb.yy) * PT(0.5); kt.zz = (kta.zz + ktb.zz) * PT(0.5);
+          kt.xy = (kta.xy + ktb.xy) * PT(0.5); kt.xz = (kta.xz + ktb.xz) * PT(0.5); kt.yz = (kta.yz + ktb.yz) * PT(0.5);
+          PT S = kt.xx * rx * rx, Tr = kt.xx;
+          if (DIM >= 2) { S += kt.yy * ry * ry + PT(2) * kt.xy * rx * ry; Tr += kt.yy; }
+          if (DIM == 3) { S += kt.zz * rz * rz + PT(2) * (kt.xz * rx * rz + kt.yz * ry * rz); Tr += kt.zz; }
+          S = S / r2;
+          PT contr;
+          if (P.ddw == 401) {
+            // hessian_ddw_cart, kernel.f90:377-417
+            PT cst = wcv * ihd1 * inv_ha;
+            PT dfq = dfa / qa;
+            contr = cst * ((ddfa - dfq) * S + dfq * Tr);
+          } else {
+            // hessian_fab_cart / hessian_fw_cart, kernel.f90:460-539
+            PT half = (P.ddw == 402) ? PT(0.5) * (PT)P.fwc * wcv * fa * ihd1 * inv_ha
+                                     : -wcv * dfa * ihd1 * inv_dr;  // 0.5*Gab
+            contr = half * (PT(DIM + 2) * S - Tr);
+          }
+          ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
+        } else {
+          PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
+          PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
+          PT kdbz = ktb.xz * gbx + ktb.yz * gby + ktb.zz * gbz;
+          if (P.ddw == 403) {
+            PT odda = PT(1) / oma / rhoa / rhoa, oddb = PT(1) / omb / rhob / rhob;
+            ddt += mb * ((kdax * nwax + kday * nway + kdaz * nwaz) * odda +
+                         (kdbx * nwbx + kdby * nwby + kdbz * nwbz) * oddb);
+          } else {
+            ddt += mb / oma / rhoa / rhoa * ((kdbx - kdax) * nwax + (kdby - kday) * nway + (kdbz - kdaz) * nwaz);
+          }
+        }
+      }
+      if (P.magneto) {
+        // stress tensors (circuit2.f90:166-175, 433-441), force (:443-454), induction (:458-461)
+        PT b2a = ub2, b2b = bbx * bbx + bby * bby + bbz * bbz;
+        PT ma_x = ((PT(0.5) * b2a - bax * bax) * nwax - bax * bay * nway - bax * baz * nwaz) * mu0inv;
+        PT ma_y = (-bay * bax * nwax + (PT(0.5) * b2a - bay * bay) * nway - bay * baz * nwaz) * mu0inv;
+        PT ma_z = (-baz * bax * nwax - baz * bay * nway + (PT(0.5) * b2a - baz * baz) * nwaz) * mu0inv;
+        PT mb_x = ((PT(0.5) * b2b - bbx * bbx) * nwbx - bbx * bby * nwby - bbx * bbz * nwbz) * mu0inv;
+        PT mb_y = (-bby * bbx * nwbx + (PT(0.5) * b2b - bby * bby) * nwby - bby * bbz * nwbz) * mu0inv;
+        PT mb_z = (-bbz * bbx * nwbx - bbz * bby * nwby + (PT(0.5) * b2b - bbz * bbz) * nwbz) * mu0inv;
+        PT ib = PT(1) / (rhob * rhob * omb);
+        dvx -= mb * (ma_x / rho2oma + mb_x * ib);
+        dvy -= mb * (ma_y / rho2oma + mb_y * ib);
+        dvz -= mb * (ma_z / rho2oma + mb_z * ib);
+        PT bnwa = bax * nwax + bay * nway + baz * nwaz;
+        PT f = PT(1) / (rhoa * oma) * mb;
+        dbx -= f * (vabx * bnwa - bbx * vnwa);
+        dby -= f * (vaby * bnwa - bby * vnwa);
+        dbz -= f * (vabz * bnwa - bbz * vnwa);
+      }
+    }
+    if (qbad) atomicOr(a.err, EB_Q);
+    double dh = 0.0;
+    if (P.adden) dh = -DIM * ha_d / (double)rhoa * (double)drho / (double)oma;
+    a.out.f[F_AX][s] = (double)dvx; a.out.f[F_AY][s] = (double)dvy; a.out.f[F_AZ][s] = (double)dvz;
+    a.out.f[F_DH][s] = dh;
+    a.out.f[F_DDT][s] = (double)ddt;
+    a.out.f[F_DBX][s] = (double)dbx; a.out.f[F_DBY][s] = (double)dby; a.out.f[F_DBZ][s] = (double)dbz;
+    a.out.f[F_DU][s] = (double)du;
+    // FP64 energy sum (circuit2.f90:490-492)
+    double vx = (double)vax, vy = (double)vay, vz = (double)vaz;
+    double bx = (double)bax, by = (double)bay, bz = (double)baz, rh = (double)rhoa;
+    e_part = pa.w * (vx * (double)dvx + vy * (double)dvy + vz * (double)dvz + (double)du +
+                     (bx * (double)dbx + by * (double)dby + bz * (double)dbz) / rh -
+                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * (double)drho / (double)oma);
+  }
+  __shared__ double red[4];
+  double w = wave_sum(e_part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i];
+    a.energy_part[xcd_block(blockIdx.x, gridDim.x)] = t;
+  }
+}
+
+// ------------------------------------------------------------- transfers
+// AoS store rows (any leading dimension) <-> SoA fields in sorted order.
+__global__ void k_aos_to_soa(const double *aos, int64_t ld, int n, Fields F, int64_t *orig) {
+  int s = gtid();
+  if (s >= n) return;
+  const double *row = aos + (size_t)s * ld;
+#pragma unroll 4
+  for (int f = 0; f < F_TOTAL; f++) F.f[f][s] = row[f];
+  orig[s] = s;
+}
+
+__global__ void k_soa_to_aos(Fields F, const int64_t *orig, int n, double *aos, int64_t ld) {
+  int s = gtid();
+  if (s >= n) return;
+  double *row = aos + (size_t)orig[s] * ld;
+#pragma unroll 4
+  for (int f = 0; f < F_TOTAL; f++) row[f] = F.f[f][s];
+}
+
+// Neighbour lists back in original numbering (for sph_neighbours).
+__global__ void k_neigh_export(const uint32_t *nb, const int *cnt, const int64_t *orig, const uint8_t *real,
+                               int n, int npad, const int64_t *row_off_by_orig, int32_t *jout, int8_t *cout) {
+  int s = gtid();
+  if (s >= n || !real[s]) return;
+  int64_t o = row_off_by_orig[orig[s]];
+  for (int k = 0; k < cnt[s]; k++) {
+    uint32_t e = nb[(size_t)k * npad + s];
+    jout[o + k] = (int32_t)orig[nb_index(e)];
+    cout[o + k] = (int8_t)nb_code(e);
+  }
+}
+
+__global__ void k_counts_by_orig(const int *cnt, const int64_t *orig, const uint8_t *real, int n, int64_t *out) {
+  int s = gtid();
+  if (s >= n || !real[s]) return;
+  out[orig[s]] = cnt[s];
+}
+
+__global__ void k_find_orig(const int64_t *orig, int n, int64_t target, int *pos) {
+  int s = gtid();
+  if (s < n && orig[s] == target) *pos = s;
+}
+
+// ----------------------------------------------------- step-driver updates
+// Leapfrog predictor (main.f90:323-342) on real particles; saves the old
+// derivatives in `sv` (ax, ay, az, du, dh, ddt, dbx, dby, dbz) for the corrector.
+__global__ void k_predict(Fields F, const uint8_t *real, int n, double dt, int do_t, double *sv, int npad) {
+  int s = gtid();
+  if (s >= n || !real[s]) return;
+  const int src[9] = {F_AX, F_AY, F_AZ, F_DU, F_DH, F_DDT, F_DBX, F_DBY, F_DBZ};
+  double o[9];
+  for (int k = 0; k < 9; k++) { o[k] = F.f[src[k]][s]; sv[(size_t)k * npad + s] = o[k]; }
+  for (int a = 0; a < 3; a++) {
+    double v = F.f[F_VX + a][s];
+    F.f[F_RX + a][s] = F.f[F_RX + a][s] + dt * v + 0.5 * dt * dt * o[a];
+    F.f[F_VX + a][s] = v + dt * o[a];
+    F.f[F_BX + a][s] = F.f[F_BX + a][s] + dt * o[6 + a];
+  }
+  F.f[F_U][s] = F.f[F_U][s] + dt * o[3];
+  F.f[F_H][s] = F.f[F_H][s] + dt * o[4];
+  if (do_t) F.f[F_T][s] = F.f[F_T][s] + dt * o[5];
+}
+
+// Leapfrog corrector (main.f90:348-362).
+__global__ void k_correct(Fields F, const uint8_t *real, int n, double dt, int do_t, const double *sv, int npad) {
+  int s = gtid();
+  if (s >= n || !real[s]) return;
+  double o[9];
+  for (int k = 0; k < 9; k++) o[k] = sv[(size_t)k * npad + s];
+  for (int a = 0; a < 3; a++) {
+    F.f[F_VX + a][s] = F.f[F_VX + a][s] + 0.5 * dt * (F.f[F_AX + a][s] - o[a]);
+    F.f[F_BX + a][s] = F.f[F_BX + a][s] + 0.5 * dt * (F.f[F_DBX + a][s] - o[6 + a]);
+  }
+  F.f[F_U][s] = F.f[F_U][s] + 0.5 * dt * (F.f[F_DU][s] - o[3]);
+  F.f[F_H][s] = F.f[F_H][s] + 0.5 * dt * (F.f[F_DH][s] - o[4]);
+  if (do_t) F.f[F_T][s] = F.f[F_T][s] + 0.5 * dt * (F.f[F_DDT][s] - o[5]);
+}
+
+// Per-real-particle minima/maxima for timestep.calc (timestep.f90:62-81):
+// red (ordered int64): 0 min h, 1 max c, 2 min den, 3 min c, 4 max kappa, 5 max |B_l|
+__global__ void k_dt_reduce(Fields F, const uint8_t *real, int n, long long *red) {
+  int s = gtid();
+  const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
+  long long v[6] = {BIG, SMALL, BIG, BIG, SMALL, SMALL};
+  if (s < n && real[s]) {
+    v[0] = d2ord(F.f[F_H][s]);
+    v[1] = d2ord(F.f[F_C][s]);
+    v[2] = d2ord(F.f[F_DEN][s]);
+    v[3] = v[1];
+    v[4] = d2ord(F.f[F_KAPPA][s]);
+    double bm = fmax(fmax(F.f[F_BX][s], F.f[F_BY][s]), F.f[F_BZ][s]);
+    v[5] = d2ord(bm);
+  }
+  v[0] = wave_min(v[0]); v[2] = wave_min(v[2]); v[3] = wave_min(v[3]);
+  v[1] = wave_max(v[1]); v[4] = wave_max(v[4]); v[5] = wave_max(v[5]);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&red[0], v[0]); atomicMax(&red[1], v[1]); atomicMin(&red[2], v[2]);
+    atomicMin(&red[3], v[3]); atomicMax(&red[4], v[4]); atomicMax(&red[5], v[5]);
+  }
+}
+
+// RKL2 stage combination on T (sts_integrator.f90:111-115) over every
+// particle slot, as the reference updates whole store rows.
+__global__ void k_sts_stage(double *T, const double *y0, const double *yjm1, const double *yjm2,
+                            const double *dy0, const double *dyjm1, int n, double mu, double nu,
+                            double muh_dt, double gmh_dt, double *yj) {
+  int s = gtid();
+  if (s >= n) return;
+  yj[s] = mu * yjm1[s] + nu * yjm2[s] + (1. - mu - nu) * y0[s] + muh_dt * dyjm1[s] + gmh_dt * dy0[s];
+  (void)T;
+}
+
+__global__ void k_axpy_copy(double *dst, const double *x, const double *y, double a, int n) {
+  int s = gtid();
+  if (s >= n) return;
+  dst[s] = x[s] + a * y[s];
+}
+
+// ------------------------------------------------------------ instantiation
+#define SPH_INST_DK(DIM, KT)                                                                 \
+  template __global__ void k_loop1<DIM, KT, 0, double>(Loop1Args<double4>);                  \
+  template __global__ void k_loop1<DIM, KT, 1, double>(Loop1Args<double4>);                  \
+  template __global__ void k_loop1<DIM, KT, 2, double>(Loop1Args<double4>);                  \
+  template __global__ void k_loop1<DIM, KT, 0, float>(Loop1Args<double4>);                   \
+  template __global__ void k_loop1<DIM, KT, 1, float>(Loop1Args<double4>);                   \
+  template __global__ void k_loop1<DIM, KT, 2, float>(Loop1Args<double4>);                   \
+  template __global__ void k_loop1_simple<DIM, KT, 0, double>(Loop1Args<double4>, Fields);   \
+  template __global__ void k_loop1_simple<DIM, KT, 1, double>(Loop1Args<double4>, Fields);   \
+  template __global__ void k_loop1_simple<DIM, KT, 2, double>(Loop1Args<double4>, Fields);   \
+  template __global__ void k_loop1_simple<DIM, KT, 0, float>(Loop1Args<double4>, Fields);    \
+  template __global__ void k_loop1_simple<DIM, KT, 1, float>(Loop1Args<double4>, Fields);    \
+  template __global__ void k_loop1_simple<DIM, KT, 2, float>(Loop1Args<double4>, Fields);    \
+  template __global__ void k_loop2<DIM, KT, double, double4>(Loop2Args<double4>);            \
+  template __global__ void k_loop2<DIM, KT, float, float4>(Loop2Args<float4>);
+
+SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
+SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
+SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
+
+template __global__ void k_neigh<1>(NeighArgs);
+template __global__ void k_neigh<2>(NeighArgs);
+template __global__ void k_neigh<3>(NeighArgs);
+template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
+                                          double, double4 *, double4 *, double4 *, double4 *, double4 *);
+template __global__ void k_post1<float4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
+                                         double, float4 *, float4 *, float4 *, float4 *, float4 *);
+
+}  // namespace sph

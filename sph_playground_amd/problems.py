@@ -174,6 +174,25 @@ def hydro2d(n_side=64, seed=1, jitter=0.01, hfac=1.0, artts="yes", adden="yes",
     return state, st
 
 
+def hydro1d(n=64, seed=2, jitter=0.01, hfac=1.2, kernel=C.KERNEL_M6, velocity=0.05):
+    """1D hydro on a jittered line in periodic [-1,1] with a sine velocity."""
+    box = (-1.0, 1.0, 0.0, 0.0, 0.0, 0.0)
+    pos, dx = lattice_positions(1, n, box)
+    pos[:, 0] += jitter * dx * gaussian(seed, n)
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_vx - 1] = velocity * np.sin(math.pi * pos[:, 0])
+    st[:, C.es_m - 1] = 2.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = dx * hfac
+    st[:, C.es_u - 1] = 1.0
+    gamma = 5.0 / 3.0
+    _eos_fill(st, gamma)
+    state = make_state(dim=1, equations="hydro", hfac=hfac, box=box, spacing=dx,
+                       n_real=n, gamma=gamma, kernel=kernel, au=1.0)
+    return state, st
+
+
 def diffusion3d(n_side=128, hfac=1.0, ddw="fab", b=(1.0, 1.0, 0.0),
                 disotropic="no", dcond=1.0, kernel=C.KERNEL_M6, kind="staggered",
                 adden="yes", sts=True, h_scale=1.0):

@@ -1,0 +1,96 @@
+"""Write the golden vectors under tests/golden/ from the reference Fortran build.
+
+Runs the reference's own hot-path modules (compiled from
+/root/reference/playground/src by oracle/ref.mk with amdflang, FP64,
+-O2 -ffp-contract=off, no fast-math) through oracle/ref_harness.f90 on seeded
+inputs and stores inputs + outputs as compressed .npz fixtures:
+
+  state      (49,)   statevars
+  store_in   (n,32)  store before the call (row i = Fortran column i+1)
+  store_out  (n,32)  store after the call
+  nb_off/nb_j/nb_code  neighbour lists of the last iterate: real index of j
+                     (0-based) and image code (sx+1)+3(sy+1)+9(sz+1)
+  meta       json: kernel, mode (0 iterate, 1 leapfrog step, 2 STS), sts_s,
+             dtcap, dt_used, generator call
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import refrun  # noqa: E402
+from sph_playground_amd import consts as C  # noqa: E402
+from sph_playground_amd import problems as P  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _slab(**kw):
+    kw.setdefault("hfac", 1.0)
+    kw.setdefault("velocity_seed", 9)
+    return P.random_slab3d(2000, **kw)
+
+
+def cases():
+    """(name, kernel, mode, sts_s, dtcap, builder, state overrides)."""
+    c = []
+    c.append(("hydro2d_m6_32", 1, 0, 0, 0.0, lambda: P.hydro2d(32), {}))
+    c.append(("hydro2d_m6_32_leapfrog", 1, 1, 0, 1e-3, lambda: P.hydro2d(32), {}))
+    c.append(("hydro2d_simple_16", 1, 0, 0, 0.0, lambda: P.hydro2d(16, artts="no", adden="no"), {}))
+    c.append(("hydro2d_m4_16", 0, 0, 0, 0.0, lambda: P.hydro2d(16, kernel=0, hfac=1.2), {}))
+    c.append(("hydro2d_gauss_16", 2, 0, 0, 0.0, lambda: P.hydro2d(16, kernel=2, hfac=0.8), {}))
+    c.append(("hydro1d_m6_64", 1, 0, 0, 0.0, lambda: P.hydro1d(64), {}))
+    for ddw, tag in (("fab", "fab"), ("n2w", "n2w"), ("fw", "fw"), ("2nw-+", "2nwds"), ("2nw+-", "2nwsd")):
+        for iso in ("yes", "no"):
+            c.append(("diff2d_%s_%s_16" % (tag, "iso" if iso == "yes" else "aniso"), 1, 0, 0, 0.0,
+                      (lambda ddw=ddw, iso=iso: P.diffusion2d(16, ddw=ddw, disotropic=iso, b=(1.0, 0.5, 0.0),
+                                                              seed=5, jitter=0.05)), {}))
+    for ddw, tag in (("2nw-+", "2nwds"), ("2nw+-", "2nwsd")):
+        c.append(("diff2d_simple_%s_16" % tag, 1, 0, 0, 0.0,
+                  (lambda ddw=ddw: P.diffusion2d(16, ddw=ddw, adden="no", seed=5, jitter=0.05)), {}))
+    c.append(("diff2d_fab_sts3_16", 1, 2, 3, 2e-4,
+              lambda: P.diffusion2d(16, ddw="fab", disotropic="no", b=(1.0, 0.5, 0.0), seed=5, jitter=0.05), {}))
+    c.append(("diff3d_fab_aniso_8", 1, 0, 0, 0.0, lambda: P.diffusion3d(8, ddw="fab"), {}))
+    c.append(("diff3d_fab_iso_8", 1, 0, 0, 0.0, lambda: P.diffusion3d(8, ddw="fab", disotropic="yes"), {}))
+    c.append(("mhd3d_periodic_2000", 1, 0, 0, 0.0, lambda: _slab(seed=3), {}))
+    c.append(("mhd3d_hydroshock_2000", 1, 0, 0, 0.0, lambda: _slab(seed=3, process="backcompatibility"),
+              {C.ec_ics: C.ett_hydroshock}))
+    c.append(("mhd3d_anisodiff_2000", 1, 0, 0, 0.0, lambda: _slab(seed=4, equations="mhd"),
+              {C.ec_disotropic: 0.0}))
+    c.append(("mhd3d_anisodiff_2nwsd_noau_2000", 1, 0, 0, 0.0, lambda: _slab(seed=4, equations="mhd", au=-1.0),
+              {C.ec_disotropic: 0.0, C.ec_ddw: C.esd_2nw_sd}))
+    c.append(("fld3d_fab_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, fld=True), {}))
+    c.append(("fld3d_2nwds_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, fld=True), {C.ec_ddw: C.esd_2nw_ds}))
+    c.append(("hyrad3d_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, equations="hyrad"), {}))
+    return c
+
+
+def main():
+    if not refrun.available(1):
+        sys.exit("reference harness not built: make -C oracle ref")
+    for name, kernel, mode, sts_s, dtcap, build, over in cases():
+        state, st = build()
+        for k, v in over.items():
+            state[k - 1] = v
+        r = refrun.run(state, st, kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, threads=1)
+        nr = int(state[C.ec_realpn - 1])
+        off = np.zeros(nr + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(a) for a in r["neigh"]])
+        nb = np.concatenate(r["neigh"]) if off[-1] else np.zeros((0, 2), np.int32)
+        meta = dict(kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, dt_used=r["dt"],
+                    reference_build="amdflang -fdefault-real-8 -fdefault-double-8 -O2 -fopenmp "
+                                    "-ffp-contract=off, OMP_NUM_THREADS=1")
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), state=state, store_in=st,
+                            store_out=r["store"], nb_off=off, nb_j=nb[:, 0].astype(np.int32),
+                            nb_code=nb[:, 1].astype(np.int8), meta=json.dumps(meta))
+        print("%-36s n=%5d pairs=%7d dt=%.3e" % (name, st.shape[0], off[-1], r["dt"]))
+
+
+if __name__ == "__main__":
+    main()

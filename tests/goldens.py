@@ -1,0 +1,89 @@
+"""Helpers to load the reference golden vectors and compare results to them."""
+import glob
+import json
+import os
+
+import numpy as np
+
+from sph_playground_amd import consts as C
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KRAD = {0: 2.0, 1: 3.0, 2: 10.0}
+
+# store rows the hot path writes (compared against the reference)
+OUT_FIELDS = ["es_rx", "es_ry", "es_rz", "es_den", "es_h", "es_om", "es_p", "es_c", "es_ax", "es_ay",
+              "es_az", "es_du", "es_dh", "es_ddt", "es_dtdx", "es_dtdy", "es_dtdz", "es_dbx", "es_dby",
+              "es_dbz", "es_vfld", "es_t", "es_vx", "es_vy", "es_vz", "es_u", "es_bx", "es_by", "es_bz"]
+
+
+def names(mode=None):
+    out = []
+    for p in sorted(glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))):
+        g = load(os.path.basename(p)[:-4])
+        if mode is None or g["meta"]["mode"] == mode:
+            out.append(os.path.basename(p)[:-4])
+    return out
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    d["meta"] = json.loads(str(d["meta"]))
+    d["name"] = name
+    return d
+
+
+def field_errors(ref, got, nr):
+    """Per store row: max |got - ref| / max |ref| over real particles (the
+    normwise relative error; exact-zero fields report the absolute error)."""
+    errs = {}
+    for f in OUT_FIELDS:
+        k = getattr(C, f) - 1
+        a = ref[:nr, k]
+        b = got[:nr, k]
+        scale = np.abs(a).max()
+        d = np.abs(a - b).max()
+        errs[f] = d / scale if scale > 0 else d
+    return errs
+
+
+def pair_sets(off, j, code):
+    return [set(zip(j[off[i]:off[i + 1]].tolist(), code[off[i]:off[i + 1]].tolist()))
+            for i in range(len(off) - 1)]
+
+
+def _image(x, s, lo, hi):
+    if s > 0:
+        return hi + (x - lo)
+    if s < 0:
+        return lo + (x - hi)
+    return x
+
+
+def is_exact_tie(state, store_in, store_out, kernel, i, j, code):
+    """True if pair (i, j, image code) sits exactly on the search radius:
+    ((0+dx^2)+dy^2)+dz^2 == (krad*h_i)^2 in FP64 (kdtree2.f90:1673-1686), the
+    only case where the kd-tree's box pruning may differ from brute force."""
+    lo = [state[C.ec_xmin - 1], state[C.ec_ymin - 1], state[C.ec_zmin - 1]]
+    hi = [state[C.ec_xmax - 1], state[C.ec_ymax - 1], state[C.ec_zmax - 1]]
+    s = [code % 3 - 1, (code // 3) % 3 - 1, code // 9 - 1]
+    xi = store_out[i, 0:3]
+    xj = [_image(store_out[j, a], s[a], lo[a], hi[a]) for a in range(3)]
+    sd = np.float64(0.0)
+    for a in range(3):
+        d = np.float64(xj[a]) - np.float64(xi[a])
+        sd = sd + d * d
+    r = np.float64(KRAD[kernel]) * np.float64(store_in[i, C.es_h - 1])
+    return bool(sd == r * r)
+
+
+def compare_neighbours(g, off, j, code):
+    """Return the list of (i, pair) mismatches that are NOT exact ties."""
+    ref = pair_sets(g["nb_off"], g["nb_j"], g["nb_code"])
+    got = pair_sets(off, j, code)
+    bad = []
+    for i, (a, b) in enumerate(zip(ref, got)):
+        for (jj, cc) in a ^ b:
+            if not is_exact_tie(g["state"], g["store_in"], g["store_out"], g["meta"]["kernel"], i, jj, cc):
+                bad.append((i, (jj, cc)))
+    return bad

@@ -1,0 +1,188 @@
+"""HIP path parity: against the reference golden vectors and the C oracle.
+
+FP64 path: neighbour sets identical as (j, image) sets per particle (only
+exact FP64 ties on the radius may differ from the kd-tree), every written
+store row within 1e-12 relative (max |d| / max |ref| per row).
+Mixed path: density within 1e-5 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sph_playground_amd import consts as C
+from sph_playground_amd import problems as P
+from tests import goldens as G
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _engine(state, kernel, **kw):
+    from sph_playground_amd.engine import Engine
+    return Engine(state, kernel=kernel, **kw)
+
+
+@pytest.mark.parametrize("name", G.names(mode=0))
+def test_iterate_matches_reference_golden(gpu, name):
+    g = G.load(name)
+    print("case", name, flush=True)
+    nr = int(g["state"][C.ec_realpn - 1])
+    eng = _engine(g["state"], g["meta"]["kernel"])
+    st = g["store_in"].copy()
+    assert eng.iterate(st) == 0.0
+    off, j, code = eng.neighbours()
+    bad = G.compare_neighbours(g, off, j, code)
+    assert not bad, bad[:5]
+    errs = G.field_errors(g["store_out"], st, nr)
+    assert max(errs.values()) <= FP64_TOL, errs
+    eng.close()
+
+
+def test_leapfrog_matches_reference_golden(gpu):
+    g = G.load("hydro2d_m6_32_leapfrog")
+    nr = int(g["state"][C.ec_realpn - 1])
+    eng = _engine(g["state"], 1)
+    eng.load(g["store_in"])
+    dt = eng.leapfrog(g["meta"]["dtcap"])
+    assert abs(dt - g["meta"]["dt_used"]) <= 1e-15 * abs(g["meta"]["dt_used"])
+    out = eng.save()
+    errs = G.field_errors(g["store_out"], out, nr)
+    assert max(errs.values()) <= FP64_TOL, errs
+    off, j, code = eng.neighbours()
+    assert not G.compare_neighbours(g, off, j, code)
+
+
+def test_sts_matches_reference_golden(gpu):
+    g = G.load("diff2d_fab_sts3_16")
+    nr = int(g["state"][C.ec_realpn - 1])
+    eng = _engine(g["state"], 1)
+    eng.load(g["store_in"])
+    eng.sts(g["meta"]["sts_s"], g["meta"]["dtcap"])
+    out = eng.save()
+    errs = G.field_errors(g["store_out"], out, nr)
+    assert max(errs.values()) <= FP64_TOL, errs
+
+
+def _oracle_compare(state, st, kernel=1, tol=FP64_TOL, fields=None, **okw):
+    nr = int(state[C.ec_realpn - 1])
+    ref = oracle.iterate(state, st, kernel=kernel, **okw)
+    ekw = {}
+    if okw.get("tensor_mode"):
+        ekw = dict(tensor="full", kappa_perp=okw.get("kappa_perp", 0.0))
+    eng = _engine(state, kernel, **ekw)
+    got = st.copy()
+    eng.iterate(got)
+    off, j, code = eng.neighbours()
+    a = G.pair_sets(ref["off"], ref["j"], ref["code"])
+    b = G.pair_sets(off, j, code)
+    assert sum(1 for x, y in zip(a, b) if x != y) == 0
+    errs = G.field_errors(ref["store"], got, nr)
+    if fields:
+        errs = {k: v for k, v in errs.items() if k in fields}
+    assert max(errs.values()) <= tol, errs
+    st_ = eng.stats()
+    assert st_["n_pairs"] == ref["off"][-1]
+    assert st_["sweeps"] == ref["sweeps"]
+    return eng, ref, got
+
+
+def test_hydro2d_64_one_step_vs_oracle(gpu):
+    # the 64x64 2D hydro workload (jittered lattice, M6, AV, Newton h)
+    state, st = P.hydro2d(64)
+    eng, ref, got = _oracle_compare(state, st)
+    # energy sum of loop 2 (circuit2.f90:490-492), FP64
+    assert abs(eng.stats()["energy"] - ref["energy"]) <= 1e-12 * max(1.0, abs(ref["energy"]))
+
+
+def test_diffusion3d_full_tensor_vs_oracle(gpu):
+    # general kappa_par bb + kappa_perp (I - bb) tensor, b = (1,1,0)/sqrt2
+    state, st = P.diffusion3d(16, ddw="fab", b=(1.0, 1.0, 0.0))
+    _oracle_compare(state, st, tensor_mode=1, kappa_perp=1e-3)
+
+
+@pytest.mark.parametrize("ddw", ["fab", "n2w", "fw", "2nw-+", "2nw+-"])
+def test_diffusion3d_operators_vs_oracle(gpu, ddw):
+    state, st = P.diffusion3d(12, ddw=ddw, b=(1.0, 0.3, 0.2), kind="square", h_scale=1.05)
+    _oracle_compare(state, st, tensor_mode=1, kappa_perp=0.1)
+
+
+def test_mhd_fld_slab_vs_oracle(gpu):
+    state, st = P.random_slab3d(4000, seed=5, hfac=1.0, fld=True, velocity_seed=2)
+    _oracle_compare(state, st)
+    state, st = P.random_slab3d(4000, seed=4, hfac=1.0, equations="mhd", velocity_seed=2)
+    state[C.ec_disotropic - 1] = 0.0
+    _oracle_compare(state, st)
+
+
+def test_repeat_is_bitwise_deterministic(gpu):
+    state, st = P.hydro2d(64)
+    eng = _engine(state, 1)
+    a = st.copy()
+    b = st.copy()
+    eng.iterate(a)
+    eng.iterate(b)
+    assert np.array_equal(a, b)
+
+
+def test_resident_steps_equal_host_iterate(gpu):
+    state, st = P.diffusion2d(32, ddw="fab", disotropic="no", b=(1.0, 0.5, 0.0), seed=5, jitter=0.05)
+    eng = _engine(state, 1)
+    a = st.copy()
+    eng.iterate(a)
+    eng.iterate(a)
+    eng2 = _engine(state, 1)
+    eng2.load(st)
+    eng2.step()
+    eng2.step()
+    b = eng2.save()
+    assert np.array_equal(a, b)
+
+
+def test_device_store_path(gpu):
+    import torch
+    state, st = P.hydro2d(32)
+    eng = _engine(state, 1)
+    host = st.copy()
+    eng.iterate(host)
+    d = torch.from_numpy(st.copy()).cuda()
+    eng2 = _engine(state, 1)
+    from sph_playground_amd._lib import lib, check
+    import ctypes as ct
+    s = np.ascontiguousarray(state)
+    e = ct.c_double(1.0)
+    torch.cuda.synchronize()
+    check(lib().sph_iterate_device(eng2._h, d.shape[0], ct.c_void_p(d.data_ptr()), 32,
+                                   s.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(e)))
+    assert e.value == 0.0
+    assert np.array_equal(d.cpu().numpy(), host)
+
+
+def test_mixed_precision_density_within_1e5(gpu):
+    state, st = P.diffusion3d(24, ddw="fab", b=(1.0, 1.0, 0.0), h_scale=1.02)
+    nr = st.shape[0]
+    ref = oracle.iterate(state, st, kernel=1, tensor_mode=1, kappa_perp=1e-3)
+    eng = _engine(state, 1, precision="mixed", tensor="full", kappa_perp=1e-3)
+    got = st.copy()
+    eng.iterate(got)
+    den_err = np.abs(got[:nr, C.es_den - 1] - ref["store"][:nr, C.es_den - 1]) / ref["store"][:nr, C.es_den - 1]
+    assert den_err.max() <= 1e-5
+    ddt = ref["store"][:nr, C.es_ddt - 1]
+    assert np.abs(got[:nr, C.es_ddt - 1] - ddt).max() <= 1e-3 * np.abs(ddt).max()
+
+
+def test_error_codes(gpu):
+    from sph_playground_amd._lib import SphError
+    state, st = P.hydro2d(16, kernel=2, hfac=0.3)
+    with pytest.raises(SphError) as e:
+        _engine(state, 2).iterate(st.copy())
+    assert e.value.code == 2  # NEG_H, circuit1.f90:202-204
+    state, st = P.random_slab3d(2000, seed=3, hfac=1.2)
+    with pytest.raises(SphError) as e:
+        _engine(state, 1).iterate(st.copy())
+    assert e.value.code == 1  # neighbour cap, neighboursearch.f90:127-132
+    state, st = P.random_slab3d(2000, seed=5, hfac=1.0, fld=True)
+    state[C.ec_disotropic - 1] = 0.0
+    with pytest.raises(SphError) as e:
+        _engine(state, 1).iterate(st.copy())
+    assert e.value.code == 4  # anisotropic FLD, circuit2.f90:189
