@@ -48,3 +48,15 @@ clean:
 	rm -rf $(OUT)
 
 .PHONY: all clean
+
+# Side-by-side program: reference iterate() and the HIP drop-in (needs libsph_hip.so)
+HIPLIB := sph_playground_amd/libsph_hip.so
+$(OUT)/ref_vs_hip: $(OUT)/stamp oracle/ref_vs_hip.f90 sph_playground_amd/fortran/sph_hip_capi.f90 \
+                   sph_playground_amd/fortran/iterator_hip.f90 $(HIPLIB)
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) -c sph_playground_amd/fortran/sph_hip_capi.f90 -o $(OUT)/sph_hip_capi.o
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) -c sph_playground_amd/fortran/iterator_hip.f90 -o $(OUT)/iterator_hip.o
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) oracle/ref_vs_hip.f90 $(OBJS) $(OUT)/sph_hip_capi.o \
+	  $(OUT)/iterator_hip.o -L sph_playground_amd -lsph_hip -Wl,-rpath,'$$ORIGIN/../../../sph_playground_amd' -o $@
+
+dropin: $(OUT)/ref_vs_hip
+.PHONY: dropin

@@ -24,6 +24,17 @@ def available(kernel=1):
     return os.path.exists(harness_path(kernel))
 
 
+def write_input(path, state, store, mode=0, sts_s=0, dtcap=0.0):
+    """Write a problem in the ref_harness.f90 input format."""
+    store = np.ascontiguousarray(store, dtype=np.float64)
+    n_real = int(state[24])
+    with open(path, "wb") as f:
+        f.write(struct.pack("<4i", mode, n_real, store.shape[0] - n_real, sts_s))
+        f.write(struct.pack("<d", dtcap))
+        f.write(np.asarray(state, dtype="<f8").tobytes())
+        f.write(store.astype("<f8").tobytes())
+
+
 def run(state, store, kernel=1, mode=0, sts_s=0, dtcap=0.0, threads=None):
     """Run the reference on ``(state, store)``.
 

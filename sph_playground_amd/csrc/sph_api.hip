@@ -28,10 +28,10 @@ __global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, l
 __global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
 __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
-__global__ void k_cell_range(const uint32_t *, int, int *, int *);
+__global__ void k_cell_range(const uint32_t *, int, Grid, int *, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, int, double4 *, double *, uint16_t *,
                        uint8_t *, double4 *, double *);
-template <int DIM>
+template <int DIM, int RS>
 __global__ void k_neigh(NeighArgs);
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1(Loop1Args<double4>);
@@ -39,8 +39,8 @@ template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1_simple(Loop1Args<double4>, Fields);
 template <typename V4>
 __global__ void k_post1(Fields, const double4 *, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
-                        V4 *, V4 *, V4 *);
-template <int DIM, int KT, typename PT, typename V4>
+                        V4 *, V4 *, V4 *, long long *);
+template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ void k_loop2(Loop2Args<V4>);
 __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
 __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
@@ -51,8 +51,8 @@ __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
 __global__ void k_predict(Fields, const uint8_t *, int, double, int, double *, int);
 __global__ void k_correct(Fields, const uint8_t *, int, double, int, const double *, int);
 __global__ void k_dt_reduce(Fields, const uint8_t *, int, long long *);
-__global__ void k_sts_stage(double *, const double *, const double *, const double *, const double *,
-                            const double *, int, double, double, double, double, double *);
+__global__ void k_sts_stage(const double *, const double *, const double *, const double *, const double *, int,
+                            double, double, double, double, double *);
 __global__ void k_axpy_copy(double *, const double *, const double *, double, int);
 }  // namespace sph
 
@@ -116,9 +116,9 @@ struct sph_ctx {
   // small device scratch: [0..7] long long reductions, then ints
   long long *d_red = nullptr;
   int *d_int = nullptr;  // 0 err, 1 maxcnt, 2 sorted flag, 3 pos, 8.. sweep counters (max_sweeps+2)
-  unsigned long long *d_pairs = nullptr;  // 0 neighbour pairs, 1 loop-1 pairs
+  long long *d_stripes = nullptr;         // striped counters (kStripes slots per value)
   int *h_int = nullptr;                   // pinned mirror of d_int
-  long long *h_ll = nullptr;              // pinned scratch for small host<->device values
+  long long *h_ll = nullptr;              // pinned scratch for small host<->device values (1024)
   double *d_staging = nullptr;
   int64_t staging_bytes = 0;
   double4 *final_st = nullptr;
@@ -245,7 +245,7 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
 
 static int ensure_nb(sph_ctx *c, int cap) {
   if (cap <= c->nb_cap && c->nb) return SPH_OK;
-  cap = std::max(cap, 16);
+  cap = (std::max(cap, 16) + 7) / 8 * 8;  // whole chunks of 8 (nb_slot)
   RCHK(dalloc(&c->nb, (size_t)cap * c->npad));
   c->nb_cap = cap;
   return SPH_OK;
@@ -281,51 +281,83 @@ static int read_ints(sph_ctx *c, int first, int count) {
 }
 
 // ------------------------------------------------------- step: neighbours
+// Sum / max / or of the kStripes slots of value v in a pinned copy.
+static long long stripe_sum(const long long *h, int v) {
+  long long t = 0;
+  for (int k = 0; k < kStripes; k++) t += h[v * kStripes + k];
+  return t;
+}
+static long long stripe_max(const long long *h, int v) {
+  long long t = h[v * kStripes];
+  for (int k = 1; k < kStripes; k++) t = std::max(t, h[v * kStripes + k]);
+  return t;
+}
+static long long stripe_min(const long long *h, int v) {
+  long long t = h[v * kStripes];
+  for (int k = 1; k < kStripes; k++) t = std::min(t, h[v * kStripes + k]);
+  return t;
+}
+
 static int build_neighbours(sph_ctx *c) {
   const int n = (int)c->n;
   const int dim = c->frozen.dim;
   Fields F = c->fields();
   unsigned G = grid_for(n);
-  // 1. wrap escaped real particles, bounding box, max h
+  const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
+  // 1. wrap escaped real particles, bounding box, max h (striped reductions)
   long long *init = c->h_ll;
-  for (int a = 0; a < 3; a++) { init[a] = 0x7FFFFFFFFFFFFFFFLL; init[3 + a] = -0x7FFFFFFFFFFFFFFFLL - 1; }
-  init[6] = -0x7FFFFFFFFFFFFFFFLL - 1;
-  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7, hipMemcpyHostToDevice, c->stream));
+  for (int v = 0; v < 7; v++)
+    for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = v < 3 ? BIG : SMALL;
+  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7 * kStripes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
   hipLaunchKernelGGL(k_wrap_reduce, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box,
                      dim, c->d_red, c->d_int);
-  long long *red = c->h_ll + 8;
-  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7, hipMemcpyDeviceToHost, c->stream));
+  long long *red = c->h_ll + 512;
+  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7 * kStripes, hipMemcpyDeviceToHost, c->stream));
   RCHK(read_ints(c, 0, 1));
   if (c->h_int[0] & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
   double lo[3], hi[3];
-  for (int a = 0; a < 3; a++) { lo[a] = ord2d(red[a]); hi[a] = ord2d(red[3 + a]); }
-  double hmax = c->n_real > 0 ? ord2d(red[6]) : 0.0;
+  for (int a = 0; a < 3; a++) { lo[a] = ord2d(stripe_min(red, a)); hi[a] = ord2d(stripe_max(red, 3 + a)); }
+  double hmax = c->n_real > 0 ? ord2d(stripe_max(red, 6)) : 0.0;
   if (!(hmax > 0.0) && c->n_real > 0) return SPH_ERR_Q_NAN_OR_NEG;
   double rmax = c->frozen.krad * hmax * (1.0 + 1e-9);
   if (!(rmax > 0.0)) rmax = 1.0;
-  // 2. cell grid: cells at least rmax wide; imaged axes tile [emin, emax]
+  // 2. cell grid: cells at least rmax/RS wide, searched over +-RS cells;
+  //    imaged axes tile [emin, emax] and need 2 RS + 1 cells for every
+  //    (cell, image) pair to be distinct, else RS drops to 1.  Per-axis cell
+  //    counts stay within the 30-bit Morton key.
+  const int axis_cap = dim == 3 ? 1024 : (dim == 2 ? 32768 : (1 << 30));
   Grid g{};
   double cs[3];
-  for (int a = 0; a < 3; a++) {
-    g.periodic[a] = 0;
-    if (a >= dim) { g.lo[a] = lo[a]; g.nc[a] = 1; cs[a] = 0; continue; }
-    if ((c->box.ghost_mask >> a) & 1) {
-      double L = c->box.emax[a] - c->box.emin[a];
-      int64_t nc = (int64_t)std::floor(L / rmax);
-      nc = std::max<int64_t>(1, std::min<int64_t>(nc, 1 << 20));
-      g.nc[a] = (int)nc;
-      g.lo[a] = c->box.emin[a];
-      cs[a] = L / (double)nc;
-      g.periodic[a] = 1;
-    } else {
-      double ext = hi[a] - lo[a];
-      int64_t nc = (int64_t)std::floor(ext / rmax) + 1;
-      nc = std::max<int64_t>(1, std::min<int64_t>(nc, 1 << 20));
-      g.nc[a] = (int)nc;
-      g.lo[a] = lo[a];
-      cs[a] = rmax;
+  int RS = 2;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    g = Grid{};
+    g.mdim = dim;
+    const double cw = rmax / RS;
+    bool ok = true;
+    for (int a = 0; a < 3; a++) {
+      g.periodic[a] = 0;
+      if (a >= dim) { g.lo[a] = lo[a]; g.nc[a] = 1; cs[a] = 0; continue; }
+      if ((c->box.ghost_mask >> a) & 1) {
+        double L = c->box.emax[a] - c->box.emin[a];
+        int64_t nc = (int64_t)std::floor(L / cw);
+        nc = std::max<int64_t>(1, std::min<int64_t>(nc, axis_cap));
+        g.nc[a] = (int)nc;
+        g.lo[a] = c->box.emin[a];
+        cs[a] = L / (double)nc;
+        g.periodic[a] = 1;
+        if (RS > 1 && nc < 2 * RS + 1) ok = false;
+      } else {
+        double ext = hi[a] - lo[a];
+        cs[a] = cw;
+        if (ext / cs[a] + 1 > axis_cap) cs[a] = ext / (axis_cap - 1) * (1.0 + 1e-9);
+        g.nc[a] = (int)std::min<int64_t>((int64_t)std::floor(ext / cs[a]) + 1, axis_cap);
+        g.lo[a] = lo[a];
+      }
+      if (RS > 1 && cs[a] * RS < rmax) ok = false;  // capped axis: cells too small for the stencil
     }
+    if (ok || RS == 1) break;
+    RS = 1;
   }
   // bound the cell count (sparse sets): coarsen the largest axis
   while ((int64_t)g.nc[0] * g.nc[1] * g.nc[2] > (int64_t)1 << 26) {
@@ -333,6 +365,7 @@ static int build_neighbours(sph_ctx *c) {
     if (g.periodic[a]) {
       g.nc[a] = std::max(1, g.nc[a] / 2);
       cs[a] = (c->box.emax[a] - c->box.emin[a]) / g.nc[a];
+      if (RS > 1 && g.nc[a] < 2 * RS + 1) RS = 1;
     } else {
       cs[a] *= 2;
       g.nc[a] = (int)std::floor((hi[a] - lo[a]) / cs[a]) + 1;
@@ -342,17 +375,18 @@ static int build_neighbours(sph_ctx *c) {
   c->grid = g;
   int64_t ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
   RCHK(ensure_cells(c, ncell));
-  // 3. keys, sortedness, radix sort + reorder when needed
+  int axis_bits = 1;
+  while ((1 << axis_bits) < std::max(g.nc[0], std::max(g.nc[1], g.nc[2]))) axis_bits++;
+  const int key_bits = std::min(32, axis_bits * dim);
+  // 3. Morton keys, sortedness, radix sort + reorder when needed
   hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
   hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
   RCHK(read_ints(c, 2, 1));
   c->stats.sorted = 0;
   if (c->h_int[2]) {
-    int bits = 1;
-    while (((int64_t)1 << bits) < ncell) bits++;
     size_t bytes = c->sort_tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, bytes, c->keys, c->keys2, c->vals, c->vals2, n, 0, bits,
-                                              c->stream));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, bytes, c->keys, c->keys2, c->vals, c->vals2, n, 0,
+                                              key_bits, c->stream));
     int nxt = 1 - c->cur;
     Fields src = c->fields();
     Fields dst;
@@ -378,32 +412,39 @@ static int build_neighbours(sph_ctx *c) {
   }
   HIPCHK(hipMemsetAsync(c->cstart, 0, sizeof(int) * ncell, c->stream));
   HIPCHK(hipMemsetAsync(c->cend, 0, sizeof(int) * ncell, c->stream));
-  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->cstart, c->cend);
+  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->cstart, c->cend);
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, dim,
                      c->pm, c->h0, c->band, c->real, c->st[0], c->eps);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
     HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 2, c->stream));
-    HIPCHK(hipMemsetAsync(c->d_pairs, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     NeighArgs a;
     a.pm = c->pm; a.hs = c->h0; a.band = c->band; a.real = c->real; a.cstart = c->cstart; a.cend = c->cend;
     a.g = g; a.b = c->box; a.krad = c->frozen.krad; a.n = n; a.npad = (int)c->npad; a.cap = c->nb_cap;
-    a.maxneib = c->maxneib; a.nb = c->nb; a.cnt = c->cnt; a.err = c->d_int; a.maxcnt = c->d_int + 1;
-    a.pairs = c->d_pairs;
+    a.maxneib = c->maxneib; a.nb = c->nb; a.cnt = c->cnt; a.err = c->d_int; a.stripes = c->d_stripes;
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    if (dim == 1) hipLaunchKernelGGL(k_neigh<1>, dim3(G), dim3(kBlock), 0, c->stream, a);
-    else if (dim == 2) hipLaunchKernelGGL(k_neigh<2>, dim3(G), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_neigh<3>, dim3(G), dim3(kBlock), 0, c->stream, a);
+    if (RS == 2) {
+      if (dim == 1) hipLaunchKernelGGL((k_neigh<1, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      else if (dim == 2) hipLaunchKernelGGL((k_neigh<2, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      else hipLaunchKernelGGL((k_neigh<3, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
+    } else {
+      if (dim == 1) hipLaunchKernelGGL((k_neigh<1, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      else if (dim == 2) hipLaunchKernelGGL((k_neigh<2, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      else hipLaunchKernelGGL((k_neigh<3, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
+    }
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_ll + 16, c->d_pairs, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-    RCHK(read_ints(c, 0, 2));
-    c->stats.n_pairs = (int64_t)c->h_ll[16];
+    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
+                          c->stream));
+    RCHK(read_ints(c, 0, 1));
+    c->stats.n_pairs = stripe_sum(c->h_ll, 0);
+    int maxcnt = (int)stripe_max(c->h_ll, 1);
     c->stats.ms_k_neigh = elapsed(c, c->ev[5], c->ev[6]);
     int bits = c->h_int[0];
     if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
     if (!(bits & EB_OVERFLOW)) return SPH_OK;
-    RCHK(ensure_nb(c, c->h_int[1] + 8));
+    RCHK(ensure_nb(c, maxcnt + 8));
   }
   return SPH_ERR_HIP;
 }
@@ -436,79 +477,78 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
   Fields F = c->fields();
   Loop1Args<double4> a{};
   a.pm = c->pm; a.h0 = c->h0; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt; a.eps = c->eps;
-  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = c->d_int; a.pairs = c->d_pairs + 1;
+  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = c->d_int; a.stripes = c->d_stripes;
   a.b = c->box; a.p = P; a.n = n; a.npad = (int)c->npad;
   double tol = c->opt.newton_tol > 0 ? c->opt.newton_tol : (c->opt.precision == 1 ? 1e-6 : 1e-8);
   a.tol = tol;
   int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
-  HIPCHK(hipMemsetAsync(c->d_pairs + 1, 0, sizeof(unsigned long long), c->stream));
+  c->stats.loop1_pairs = 0;
   if (!P.adden) {
     // c1simple: grad T only with diffusion and the 2nw operators
     int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
     a.cur = c->st[0];
     a.next = c->st[1];
-    c->h_int[8] = n;
-    HIPCHK(hipMemcpyAsync(c->d_int + 8, c->h_int + 8, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    a.prev_active = c->d_int + 8;
-    a.active = c->d_int + 9;
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, true, F);
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, true, F);
     else launch_loop1_k<3>(c, a, grad, true, F);
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->final_st = c->st[0];
     c->stats.sweeps = 0;
     RCHK(read_ints(c, 0, 1));
+    c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
+    c->stats.loop1_pairs = c->stats.n_pairs;
     return map_err(c->h_int[0]);
   }
   int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
-  int nreal = (int)c->n_real;
-  for (int k = 0; k < maxsw + 2; k++) c->h_int[8 + k] = 0;
-  c->h_int[8] = nreal;
-  HIPCHK(hipMemcpyAsync(c->d_int + 8, c->h_int + 8, sizeof(int) * (maxsw + 2), hipMemcpyHostToDevice, c->stream));
   int sweeps = 0;
   // while (maxval(eps, eps > 0) > tol .and. iter < 100), circuit1.f90:114;
   // each sweep counts the particles it leaves active
-  for (int it = 1; nreal > 0 && it <= maxsw; it++) {
+  for (int it = 1; c->n_real > 0 && it <= maxsw; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
-    a.prev_active = c->d_int + 8 + (it - 1);
-    a.active = c->d_int + 8 + it;
+    HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
     else launch_loop1_k<3>(c, a, grad, false, F);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     sweeps = it;
-    HIPCHK(hipMemcpyAsync(c->h_int, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    RCHK(read_ints(c, 8 + it, 1));
+    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
+                          c->stream));
+    RCHK(read_ints(c, 0, 1));
     c->stats.ms_k_loop1 += elapsed(c, c->ev[5], c->ev[6]);
+    c->stats.loop1_pairs += stripe_sum(c->h_ll, 1);
     if (c->h_int[0]) break;  // circuit1.f90:202-204 stops inside the sweep
-    if (c->h_int[8 + it] == 0) break;
+    if (stripe_sum(c->h_ll, 0) == 0) break;
   }
   c->final_st = c->st[sweeps & 1];
   c->stats.sweeps = sweeps;
-  RCHK(read_ints(c, 0, 1));
-  HIPCHK(hipMemcpyAsync(c->h_ll + 17, c->d_pairs + 1, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  c->stats.loop1_pairs = (int64_t)c->h_ll[17];
   return map_err(c->h_int[0]);
 }
 
-template <int DIM, int KT>
+template <int DIM, int KT, int MODE>
 static void launch_loop2(sph_ctx *c, void *args) {
   unsigned G = grid_for(c->n);
   if (c->opt.precision == 1)
-    hipLaunchKernelGGL((k_loop2<DIM, KT, float, float4>), dim3(G), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_loop2<DIM, KT, float, float4, MODE>), dim3(G), dim3(kBlock), 0, c->stream,
                        *(Loop2Args<float4> *)args);
   else
-    hipLaunchKernelGGL((k_loop2<DIM, KT, double, double4>), dim3(G), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_loop2<DIM, KT, double, double4, MODE>), dim3(G), dim3(kBlock), 0, c->stream,
                        *(Loop2Args<double4> *)args);
 }
-template <int DIM>
+template <int DIM, int MODE>
 static void launch_loop2_k(sph_ctx *c, void *args) {
-  if (c->kt == KT_M4) launch_loop2<DIM, KT_M4>(c, args);
-  else if (c->kt == KT_M6) launch_loop2<DIM, KT_M6>(c, args);
-  else launch_loop2<DIM, KT_GAUSS>(c, args);
+  if (c->kt == KT_M4) launch_loop2<DIM, KT_M4, MODE>(c, args);
+  else if (c->kt == KT_M6) launch_loop2<DIM, KT_M6, MODE>(c, args);
+  else launch_loop2<DIM, KT_GAUSS, MODE>(c, args);
+}
+template <int MODE>
+static void launch_loop2_d(sph_ctx *c, int dim, void *args) {
+  if (dim == 1) launch_loop2_k<1, MODE>(c, args);
+  else if (dim == 2) launch_loop2_k<2, MODE>(c, args);
+  else launch_loop2_k<3, MODE>(c, args);
 }
 
 template <typename V4>
@@ -523,31 +563,42 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
     hipLaunchKernelGGL(k_find_orig, dim3(G), dim3(kBlock), 0, c->stream, c->orig[c->cur], n, (int64_t)0,
                        c->d_int + 3);
     RCHK(read_ints(c, 3, 1));
-    HIPCHK(hipMemcpy(&c1, F.f[F_C] + c->h_int[3], sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(c->h_ll + 4, F.f[F_C] + c->h_int[3], sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memcpy(&c1, c->h_ll + 4, sizeof(double));
   }
-  const bool need_v = P.adden || P.artts || P.hydro || P.magneto;
-  const bool need_q3 = P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
-  const bool need_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
-  const bool need_q2 = P.diff || (P.artts && P.au != -1.0);
+  // which j-side packs loop 2 may gather (must match k_loop2)
+  const bool generic = !(P.diff && !P.hydro && !P.artts && !P.magneto && !P.fld &&
+                         (P.ddw == 400 || P.ddw == 401 || P.ddw == 402));
+  const bool want_v = P.adden || P.artts || P.hydro || P.magneto;
+  const bool want_q2 = generic && ((P.diff && P.fld) || P.artts);
+  const bool want_q3 = generic && P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
+  const bool want_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
   V4 *q0 = (V4 *)c->q[0];
-  V4 *q1 = need_v ? (V4 *)c->q[1] : nullptr;
-  V4 *q2 = need_q2 ? (V4 *)c->q[2] : nullptr;
-  V4 *q3 = need_q3 ? (V4 *)c->q[3] : nullptr;
-  V4 *q4 = need_q4 ? (V4 *)c->q[4] : nullptr;
+  V4 *q1 = want_v ? (V4 *)c->q[1] : nullptr;
+  V4 *q2 = want_q2 ? (V4 *)c->q[2] : nullptr;
+  V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
+  V4 *q4 = want_q4 ? (V4 *)c->q[4] : nullptr;
+  HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
-                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4);
+                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, c->d_stripes);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
+  HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * kStripes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  long long fl = 0;
+  for (int k = 0; k < kStripes; k++) fl |= c->h_ll[k];
   Loop2Args<V4> a{};
   a.pm = c->pm; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
+  a.v_nonzero = (fl & 1) ? 1 : 0;
+  a.b_uniform = (fl & 2) ? 0 : 1;
   a.out = F; a.energy_part = c->energy_part; a.err = c->d_int; a.b = c->box; a.p = P; a.n = n;
   a.npad = (int)c->npad;
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
   HIPCHK(hipEventRecord(c->ev[5], c->stream));
-  if (P.dim == 1) launch_loop2_k<1>(c, &a);
-  else if (P.dim == 2) launch_loop2_k<2>(c, &a);
-  else launch_loop2_k<3>(c, &a);
+  if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
+  else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
   HIPCHK(hipEventRecord(c->ev[6], c->stream));
   std::vector<double> parts(G);
   HIPCHK(hipMemcpyAsync(parts.data(), c->energy_part, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
@@ -728,11 +779,11 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   c->n_real = (int64_t)state[S_REALPN];
   c->n_fixed = (int64_t)state[S_FIXEDPN];
   if (c->n_real < 0) { delete c; return SPH_ERR_BAD_ARG; }
-  if (dev_alloc((void **)&c->d_red, 64 * sizeof(long long)) != SPH_OK ||
+  if (dev_alloc((void **)&c->d_red, 16 * kStripes * sizeof(long long)) != SPH_OK ||
       dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
-      dev_alloc((void **)&c->d_pairs, 4 * sizeof(unsigned long long)) != SPH_OK ||
+      dev_alloc((void **)&c->d_stripes, 8 * kStripes * sizeof(long long)) != SPH_OK ||
       hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_ll, 64 * sizeof(long long)) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_ll, 1024 * sizeof(long long)) != hipSuccess) {
     sph_ctx_destroy(c);
     return SPH_ERR_NOMEM;
   }
@@ -748,7 +799,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
                   c->sort_tmp, c->cstart, c->cend, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->band,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
-                  c->d_red, c->d_int, c->d_pairs, c->d_staging};
+                  c->d_red, c->d_int, c->d_stripes, c->d_staging};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
@@ -848,17 +899,20 @@ int sph_neighbours(sph_ctx *c, int64_t *off, int32_t *j, int8_t *code, int64_t c
 
 // timestep.calc, timestep.f90:40-166 (no FLD branch)
 static int calc_dt(sph_ctx *c, const Phys &P, double *dt) {
-  long long *init = c->h_ll + 24;
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
-  init[0] = BIG; init[1] = SMALL; init[2] = BIG; init[3] = BIG; init[4] = SMALL; init[5] = SMALL;
-  HIPCHK(hipMemcpyAsync(c->d_red + 8, init, sizeof(long long) * 6, hipMemcpyHostToDevice, c->stream));
+  const bool is_min[6] = {true, false, true, true, false, false};
+  long long *init = c->h_ll;
+  for (int v = 0; v < 6; v++)
+    for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = is_min[v] ? BIG : SMALL;
+  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 6 * kStripes, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_dt_reduce, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
-                     c->d_red + 8);
-  long long *r = c->h_ll + 32;
-  HIPCHK(hipMemcpyAsync(r, c->d_red + 8, sizeof(long long) * 6, hipMemcpyDeviceToHost, c->stream));
+                     c->d_red);
+  long long *r = c->h_ll + 512;
+  HIPCHK(hipMemcpyAsync(r, c->d_red, sizeof(long long) * 6 * kStripes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  double minh = ord2d(r[0]), maxc = ord2d(r[1]), minden = ord2d(r[2]), minc = ord2d(r[3]);
-  double maxkappa = ord2d(r[4]), maxb = ord2d(r[5]);
+  double v[6];
+  for (int k = 0; k < 6; k++) v[k] = ord2d(is_min[k] ? stripe_min(r, k) : stripe_max(r, k));
+  double minh = v[0], maxc = v[1], minden = v[2], minc = v[3], maxkappa = v[4], maxb = v[5];
   double cnhydro, cnd2;
   cn_of(c->kt, P.ddw, &cnhydro, &cnd2);
   double d = HUGE_VAL;
@@ -932,7 +986,7 @@ int sph_sts(sph_ctx *c, const double state[49], int s, double dt) {
     if (rc != SPH_OK) { c->naux = 0; return rc; }
     HIPCHK(hipMemcpyAsync(c->aux(4), DDT(), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     double mu = sts_mu(j), nu = sts_nu(j);
-    hipLaunchKernelGGL(k_sts_stage, dim3(G), dim3(kBlock), 0, c->stream, T(), c->aux(0), c->aux(1), c->aux(2),
+    hipLaunchKernelGGL(k_sts_stage, dim3(G), dim3(kBlock), 0, c->stream, c->aux(0), c->aux(1), c->aux(2),
                        c->aux(3), c->aux(4), n, mu, nu, sts_mu_hat(j, s) * dt, sts_gm_hat(j, s) * dt, c->aux(5));
     HIPCHK(hipMemcpyAsync(c->aux(2), c->aux(1), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->aux(1), c->aux(5), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));

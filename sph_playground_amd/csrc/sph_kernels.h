@@ -28,6 +28,21 @@ struct Fields {
   double *f[F_TOTAL];
 };
 
+// Neighbour entries are stored in chunks of 8 per particle:
+//   nb[(k >> 3) * npad * 8 + s * 8 + (k & 7)]
+// so each lane writes (and later reads, as two 16-byte loads) its own
+// 32-byte sector, and a wavefront reads 64 consecutive sectors.
+__host__ __device__ inline size_t nb_slot(int k, int s, int npad) {
+  return (size_t)(k >> 3) * (size_t)npad * 8 + (size_t)s * 8 + (size_t)(k & 7);
+}
+
+constexpr int kStripes = 64;
+
+// Loop-2 kernel variants.  MODE_GENERIC: every term under run-time flags.
+// MODE_DIFF: diffusion with a Hessian operator (fab/fw/n2w) and no
+// hydro/AV/MHD/FLD terms, the density+force step of the diffusion workloads.
+enum { MODE_GENERIC = 0, MODE_DIFF = 1 };  // counters are striped over 64 slots to avoid same-address atomics
+
 struct NeighArgs {
   const double4 *pm;       // x, y, z, m (sorted)
   const double *hs;        // h used for the search radius (pre-loop-1 h)
@@ -38,11 +53,10 @@ struct NeighArgs {
   Box b;
   double krad;
   int n, npad, cap, maxneib;
-  uint32_t *nb;            // [cap][npad]
+  uint32_t *nb;            // chunked, see nb_slot
   int *cnt;                // [n]
   int *err;                // error bits
-  int *maxcnt;
-  unsigned long long *pairs;
+  long long *stripes;      // [kStripes] pair counts, [kStripes..2k) max counts
 };
 
 template <typename V4>
@@ -56,10 +70,8 @@ struct Loop1Args {
   V4 *next;                // sweep k+1
   double *eps;
   double *dtdx, *dtdy, *dtdz;
-  const int *prev_active;  // active count after the previous sweep (skip if 0)
-  int *active;             // active count after this sweep
+  long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs
   int *err;
-  unsigned long long *pairs;
   Box b;
   Phys p;
   double tol;
@@ -72,11 +84,13 @@ struct Loop2Args {
   const uint8_t *real;
   const uint32_t *nb;
   const int *cnt;
-  const V4 *q0;  // 1/h, den, om, p
-  const V4 *q1;  // vx, vy, vz, c
-  const V4 *q2;  // u, T, kappa, -
+  const V4 *q0;  // 1/h, den, om, T
+  const V4 *q1;  // vx, vy, vz, p
+  const V4 *q2;  // c, u, kappa, -
   const V4 *q3;  // dtdx, dtdy, dtdz, -
   const V4 *q4;  // bx, by, bz, -
+  int v_nonzero;  // 0: every velocity is exactly 0 (v_ab = 0 for all pairs)
+  int b_uniform;  // 1: every particle carries the same B bit for bit (kappa_b = kappa_a)
   Fields out;
   double *energy_part;   // one FP64 partial per block
   int *err;

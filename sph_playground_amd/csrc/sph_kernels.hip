@@ -1,17 +1,21 @@
 // Device kernels of the two-loop hot path for gfx950.
 //
-// Data layout (HBM, "sorted order" = particles ordered by cell key, x fastest):
+// Data layout (HBM, "sorted order" = particles ordered by the Morton key of
+// their cell, so a wavefront's 64 particles and their neighbours form compact
+// 3D blobs):
 //   fields     32 FP64 SoA arrays, one per store row (const.f90:69-103)
-//   pm         double4 {x, y, z, m}: the one 32-byte gather every pair makes
+//   pm         double4 {x, y, z, m}: the 32-byte gather every pair makes
 //   st (x2)    loop-1 Newton state {h, den, om, T}, ping-pong per sweep
 //   q0..q4     loop-2 j-side packs (double4 for FP64, float4 for mixed)
-//   nb         neighbour entries, transposed [k][s] so lane s of a wavefront
-//              reads entry k of 64 consecutive particles in one coalesced load;
-//              entry = sorted index << 5 | periodic image code
+//   nb         neighbour entries (sorted index << 5 | periodic image code) in
+//              chunks of 8 per particle (nb_slot): one lane owns a 32-byte
+//              sector per chunk; a wavefront reads a chunk with two
+//              coalesced 16-byte loads per lane
 //
 // Every kernel maps one particle per lane (64-lane wavefronts, 256 per block)
-// and remaps blockIdx so each XCD works on a contiguous particle range (their
-// neighbours then hit that XCD's own L2).
+// and remaps blockIdx so each XCD works on a contiguous range of the Morton
+// curve (its particles' neighbours then stay in that XCD's L2).  Counters are
+// reduced per block and added to one of 64 stripes, never all to one address.
 #include <hip/hip_runtime.h>
 
 #include "sph_kernels.h"
@@ -46,17 +50,99 @@ __device__ __forceinline__ long long wave_max(long long v) {
   for (int o = 32; o > 0; o >>= 1) { long long w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
   return v;
 }
+__device__ __forceinline__ int wave_or(int v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide reductions (256 threads = 4 wavefronts); every thread must call.
+enum { R_SUM = 0, R_MIN = 1, R_MAX = 2, R_OR = 3 };
+template <int OP>
+__device__ __forceinline__ long long block_reduce(long long v, long long *lds) {
+  if (OP == R_SUM) v = wave_sum(v);
+  else if (OP == R_MIN) v = wave_min(v);
+  else if (OP == R_MAX) v = wave_max(v);
+  else v = wave_or((int)v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  long long t = lds[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+    long long u = lds[w];
+    if (OP == R_SUM) t += u;
+    else if (OP == R_MIN) t = u < t ? u : t;
+    else if (OP == R_MAX) t = u > t ? u : t;
+    else t |= u;
+  }
+  return t;
+}
+__device__ __forceinline__ int stripe() { return blockIdx.x & (kStripes - 1); }
+
+// Morton interleave of up to 10 (3D), 15 (2D) or 30 (1D) bits per axis.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x3FFu;
+  return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {
+  v &= 0x7FFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ __forceinline__ uint32_t compact2(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+  v = (v | (v >> 4)) & 0x00FF00FFu;
+  v = (v | (v >> 8)) & 0x7FFFu;
+  return v;
+}
+__device__ __forceinline__ uint32_t morton_key(const int c[3], int mdim) {
+  if (mdim == 3) return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+  if (mdim == 2) return spread2(c[0]) | (spread2(c[1]) << 1);
+  return (uint32_t)c[0];
+}
+__device__ __forceinline__ void morton_cell(uint32_t k, int mdim, int c[3]) {
+  if (mdim == 3) { c[0] = compact3(k); c[1] = compact3(k >> 1); c[2] = compact3(k >> 2); }
+  else if (mdim == 2) { c[0] = compact2(k); c[1] = compact2(k >> 1); c[2] = 0; }
+  else { c[0] = (int)k; c[1] = 0; c[2] = 0; }
+}
+
+__device__ __forceinline__ void cell_of(const Grid &g, double x, double y, double z, int c[3]) {
+  double p[3] = {x, y, z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double t = floor((p[a] - g.lo[a]) * g.inv_cs[a]);
+    c[a] = (t < 0.0) ? 0 : (t >= (double)g.nc[a] ? g.nc[a] - 1 : (int)t);
+  }
+}
 
 // ---------------------------------------------------------------- set-up
 // reflectRealParticlesPeriodically (BC.f90:102-194) for real particles, then
 // the bounding box of all particles and the largest h of the real ones.
-// red[0..2] min, red[3..5] max (ordered int64), red[6] max h.
-__global__ void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, int dim,
-                              long long *red, int *err) {
+// red[v * kStripes + stripe]: v = 0..2 min, 3..5 max (ordered int64), 6 max h.
+__global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_real, Box b,
+                                                     int dim, long long *red, int *err) {
+  __shared__ long long lds[4];
   int s = gtid();
   long long mn[3], mx[3], hm;
-  for (int a = 0; a < 3; a++) { mn[a] = 0x7FFFFFFFFFFFFFFFLL; mx[a] = -0x7FFFFFFFFFFFFFFFLL - 1; }
-  hm = mx[0];
+  const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
+  for (int a = 0; a < 3; a++) { mn[a] = BIG; mx[a] = SMALL; }
+  hm = SMALL;
   if (s < n) {
     bool real = orig[s] < n_real;
     double x[3] = {F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s]};
@@ -76,25 +162,22 @@ __global__ void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_re
     for (int a = 0; a < 3; a++) { mn[a] = mx[a] = d2ord(x[a]); }
     if (real) hm = d2ord(F.f[F_H][s]);
   }
-  for (int a = 0; a < 3; a++) { mn[a] = wave_min(mn[a]); mx[a] = wave_max(mx[a]); }
-  hm = wave_max(hm);
-  if ((threadIdx.x & 63) == 0) {
-    for (int a = 0; a < 3; a++) { atomicMin(&red[a], mn[a]); atomicMax(&red[3 + a], mx[a]); }
-    atomicMax(&red[6], hm);
+  for (int a = 0; a < 3; a++) {
+    long long v = block_reduce<R_MIN>(mn[a], lds);
+    if (threadIdx.x == 0) atomicMin(&red[a * kStripes + stripe()], v);
+    v = block_reduce<R_MAX>(mx[a], lds);
+    if (threadIdx.x == 0) atomicMax(&red[(3 + a) * kStripes + stripe()], v);
   }
+  long long v = block_reduce<R_MAX>(hm, lds);
+  if (threadIdx.x == 0) atomicMax(&red[6 * kStripes + stripe()], v);
 }
 
 __global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) {
   int s = gtid();
   if (s >= n) return;
-  double x[3] = {F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s]};
   int c[3];
-  for (int a = 0; a < 3; a++) {
-    double t = floor((x[a] - g.lo[a]) * g.inv_cs[a]);
-    int ci = (t < 0.0) ? 0 : (t >= (double)g.nc[a] ? g.nc[a] - 1 : (int)t);
-    c[a] = ci;
-  }
-  keys[s] = (uint32_t)((c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]);
+  cell_of(g, F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], c);
+  keys[s] = morton_key(c, g.mdim);
   vals[s] = (uint32_t)s;
 }
 
@@ -115,12 +198,18 @@ __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const in
   orig_dst[s] = orig_src[p];
 }
 
-__global__ void k_cell_range(const uint32_t *keys, int n, int *cstart, int *cend) {
+// Particle range of every occupied cell (cells indexed row-major).
+__global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int *cstart, int *cend) {
   int s = gtid();
   if (s >= n) return;
   uint32_t k = keys[s];
-  if (s == 0 || keys[s - 1] != k) cstart[k] = s;
-  if (s == n - 1 || keys[s + 1] != k) cend[k] = s + 1;
+  bool first = (s == 0 || keys[s - 1] != k), last = (s == n - 1 || keys[s + 1] != k);
+  if (!first && !last) return;
+  int c[3];
+  morton_cell(k, g.mdim, c);
+  int rm = (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
+  if (first) cstart[rm] = s;
+  if (last) cend[rm] = s + 1;
 }
 
 // Positions/masses, search h, ghost-band codes (BC.f90:196-309: strict
@@ -189,68 +278,89 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 }
 
 // findneighboursKDT (neighboursearch.f90:74-194) as a cell-list search:
-// every real i tests the particles of the 3^DIM surrounding cells (cells wrap
-// with an image shift along imaged axes) with the kd-tree's FP64 inclusion
-// rule sd <= (krad*h_i)^2; self (unshifted) is dropped.
-template <int DIM>
+// every real i tests the particles of the (2 RS + 1)^DIM surrounding cells
+// (cells of about r_max / RS; cells wrap with an image shift along imaged
+// axes) with the kd-tree's FP64 inclusion rule sd <= (krad*h_i)^2; self
+// (unshifted) is dropped.  Cells whose box lies farther than the radius
+// (with a safety margin) are skipped; candidates are loaded four at a time.
+template <int DIM, int RS>
 __global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
+  __shared__ long long lds[4];
   int s = gtid();
   int count = 0;
   bool live = s < a.n && a.real[s];
   if (live) {
-    double4 pi = a.pm[s];
-    double rr = a.krad * a.hs[s];
-    double r2 = rr * rr;
+    const double4 pi = a.pm[s];
+    const double rr = a.krad * a.hs[s];
+    const double r2 = rr * rr;
+    const double r2m = r2 * (1.0 + 1e-6);  // pruning bound, never tighter than the exact test
     int ci[3];
-    {
-      double x[3] = {pi.x, pi.y, pi.z};
-      for (int d = 0; d < 3; d++) {
-        double t = floor((x[d] - a.g.lo[d]) * a.g.inv_cs[d]);
-        ci[d] = (t < 0.0) ? 0 : (t >= (double)a.g.nc[d] ? a.g.nc[d] - 1 : (int)t);
-      }
-    }
-    const int zlo = DIM == 3 ? -1 : 0, zhi = DIM == 3 ? 1 : 0;
-    const int ylo = DIM >= 2 ? -1 : 0, yhi = DIM >= 2 ? 1 : 0;
-    for (int dz = zlo; dz <= zhi; dz++) {
-      int cz = ci[2] + dz, sz = 0;
+    cell_of(a.g, pi.x, pi.y, pi.z, ci);
+    const double cs[3] = {a.g.inv_cs[0] > 0 ? 1.0 / a.g.inv_cs[0] : 0.0, a.g.inv_cs[1] > 0 ? 1.0 / a.g.inv_cs[1] : 0.0,
+                          a.g.inv_cs[2] > 0 ? 1.0 / a.g.inv_cs[2] : 0.0};
+    const double px[3] = {pi.x, pi.y, pi.z};
+    const int rz = DIM == 3 ? RS : 0, ry = DIM >= 2 ? RS : 0;
+    for (int dz = -rz; dz <= rz; dz++) {
+      int czr = ci[2] + dz, cz = czr, sz = 0;
       if (a.g.periodic[2]) {
         if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
         else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
       } else if (cz < 0 || cz >= a.g.nc[2]) continue;
-      for (int dy = ylo; dy <= yhi; dy++) {
-        int cy = ci[1] + dy, sy = 0;
+      double gz = 0.0;
+      if (DIM == 3) {
+        double lo = a.g.lo[2] + czr * cs[2];
+        gz = fmax(fmax(lo - px[2], px[2] - (lo + cs[2])), 0.0);
+        gz *= gz;
+        if (gz > r2m) continue;
+      }
+      for (int dy = -ry; dy <= ry; dy++) {
+        int cyr = ci[1] + dy, cy = cyr, sy = 0;
         if (a.g.periodic[1]) {
           if (cy < 0) { cy += a.g.nc[1]; sy = -1; }
           else if (cy >= a.g.nc[1]) { cy -= a.g.nc[1]; sy = 1; }
         } else if (cy < 0 || cy >= a.g.nc[1]) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          int cx = ci[0] + dx, sx = 0;
+        double gy = 0.0;
+        if (DIM >= 2) {
+          double lo = a.g.lo[1] + cyr * cs[1];
+          gy = fmax(fmax(lo - px[1], px[1] - (lo + cs[1])), 0.0);
+          gy = gy * gy + gz;
+          if (gy > r2m) continue;
+        }
+        for (int dx = -RS; dx <= RS; dx++) {
+          int cxr = ci[0] + dx, cx = cxr, sx = 0;
           if (a.g.periodic[0]) {
             if (cx < 0) { cx += a.g.nc[0]; sx = -1; }
             else if (cx >= a.g.nc[0]) { cx -= a.g.nc[0]; sx = 1; }
           } else if (cx < 0 || cx >= a.g.nc[0]) continue;
-          int c = (cz * a.g.nc[1] + cy) * a.g.nc[0] + cx;
-          uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
-          int t0 = a.cstart[c], t1 = a.cend[c];
-          if (code == kNoImage) {
-            for (int t = t0; t < t1; t++) {
-              double4 pj = a.pm[t];
-              double sd = kd_sqdist(pj.x, pj.y, pj.z, pi.x, pi.y, pi.z);
-              if (sd <= r2 && t != s) {
-                if (count < a.cap) a.nb[(size_t)count * a.npad + s] = nb_pack((uint32_t)t, code);
-                count++;
+          {
+            double lo = a.g.lo[0] + cxr * cs[0];
+            double gx = fmax(fmax(lo - px[0], px[0] - (lo + cs[0])), 0.0);
+            if (gx * gx + gy > r2m) continue;
+          }
+          const int c = (cz * a.g.nc[1] + cy) * a.g.nc[0] + cx;
+          const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+          const int t0 = a.cstart[c], t1 = a.cend[c];
+          for (int t = t0; t < t1; t += 4) {
+            double4 pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) pj[u] = a.pm[min(t + u, t1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int tt = t + u;
+              if (tt >= t1) break;
+              double sd;
+              if (code == kNoImage) {
+                if (tt == s) continue;
+                sd = kd_sqdist(pj[u].x, pj[u].y, pj[u].z, pi.x, pi.y, pi.z);
+              } else {
+                if (!image_exists(a.band[tt], sx, sy, sz)) continue;
+                double xj = image_coord(pj[u].x, sx, a.b.emin[0], a.b.emax[0]);
+                double yj = image_coord(pj[u].y, sy, a.b.emin[1], a.b.emax[1]);
+                double zj = image_coord(pj[u].z, sz, a.b.emin[2], a.b.emax[2]);
+                sd = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z);
               }
-            }
-          } else {
-            for (int t = t0; t < t1; t++) {
-              if (!image_exists(a.band[t], sx, sy, sz)) continue;
-              double4 pj = a.pm[t];
-              double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
-              double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
-              double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
-              double sd = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z);
               if (sd <= r2) {
-                if (count < a.cap) a.nb[(size_t)count * a.npad + s] = nb_pack((uint32_t)t, code);
+                if (count < a.cap) a.nb[nb_slot(count, s, a.npad)] = nb_pack((uint32_t)tt, code);
                 count++;
               }
             }
@@ -264,14 +374,38 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
   } else if (s < a.n) {
     a.cnt[s] = 0;
   }
-  int m = count;
-  for (int o = 32; o > 0; o >>= 1) { int w = __shfl_xor(m, o, 64); m = w > m ? w : m; }
-  unsigned long long tot = wave_sum((unsigned long long)count);
-  if ((threadIdx.x & 63) == 0) {
-    if (m) atomicMax(a.maxcnt, m);
-    if (tot) atomicAdd(a.pairs, tot);
+  long long tot = block_reduce<R_SUM>(count, lds);
+  if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long *)&a.stripes[stripe()], (unsigned long long)tot);
+  long long mx = block_reduce<R_MAX>(count, lds);
+  if (threadIdx.x == 0 && mx) atomicMax(&a.stripes[kStripes + stripe()], mx);
+}
+
+// Neighbour iteration with batched gathers: entries come 8 per coalesced
+// chunk; for every NB of them `load(entry)` issues all gathers first (slots
+// past the list end reload particle s, always a valid address), then
+// `body(data)` runs on the valid ones.  NB gathers are then in flight at once.
+template <int NB, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbours(const uint32_t *nb, int s, int npad, int nn, Load load, Body body) {
+  const uint32_t self = nb_pack((uint32_t)s, kNoImage);
+  for (int k0 = 0; k0 < nn; k0 += 8) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(nb + nb_slot(k0, s, npad));
+    const uint4 lo = p[0], hi = p[1];
+    const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    const int m = min(8, nn - k0);
+#pragma unroll
+    for (int h = 0; h < 8; h += NB) {
+      if (h < m) {
+        decltype(load(0u)) jd[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) jd[u] = load(h + u < m ? e8[h + u] : self);
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+          if (h + u < m) body(jd[u]);
+      }
+    }
   }
 }
+
 
 // ---------------------------------------------------------------- loop 1
 // One Jacobi sweep of c1advanced (circuit1.f90:114-211) for the particles
@@ -280,12 +414,13 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
 // PT is the pair arithmetic type; the Newton update is always FP64.
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
-  if (*a.prev_active == 0) return;  // converged earlier: nothing to do
+  __shared__ long long lds[4];
   int s = gtid();
   bool inb = s < a.n;
   double4 c = inb ? a.cur[s] : make_double4(0, 0, 0, 0);
   bool act = inb && a.real[s] && a.eps[s] > a.tol;
   bool still = false;
+  int nn = 0;
   if (inb && !act) a.next[s] = c;
   if (act) {
     using K = KernelFn<KT, PT>;
@@ -296,25 +431,30 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     PT dsum = 0, osum = 0;
     PT g0 = 0, g1 = 0, g2 = 0, h0x = 0, h0y = 0, h0z = 0;
     bool qbad = !(ha > 0.0);
-    const int nn = a.cnt[s];
-    const uint32_t *e = a.nb + s;
-    for (int k = 0; k < nn; k++) {
-      uint32_t ent = e[(size_t)k * a.npad];
-      uint32_t j = nb_index(ent), code = nb_code(ent);
-      double4 pb = a.pm[j];
-      double xb = pb.x, yb = pb.y, zb = pb.z;
-      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+    nn = a.cnt[s];
+    struct J { double4 pb, sb; uint32_t code; };
+    auto load = [&](uint32_t ent) {
+      J d;
+      const uint32_t j = nb_index(ent);
+      d.code = nb_code(ent);
+      d.pb = a.pm[j];
+      if constexpr (GRAD != 0) d.sb = a.cur[j];  // hb, rhob, omb, tb of the previous sweep
+      return d;
+    };
+    for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
+      double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
+      if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
       PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
       PT dr = sqrt(rsq);
       PT q = dr * inv_ha;
       PT f, df;
       K::f_df(q, f, df);
-      PT mb = (PT)pb.w;
+      PT mb = (PT)d.pb.w;
       dsum += mb * f;
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
-        double4 sb = a.cur[j];  // hb, rhob, omb, tb of the previous sweep
+        const double4 sb = d.sb;
         PT inv_dr = PT(1) / dr;
         if constexpr (GRAD == 1) {
           PT w = mb * ((PT)sb.w * (PT)sb.y - (PT)ta * (PT)rhoa) * df * inv_dr;
@@ -331,7 +471,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
         }
       }
       qbad |= !(q >= PT(0));
-    }
+    });
     const double wcv = a.p.wcv;
     const double ih = 1.0 / ha;
     const double cW = wcv * ipow<DIM>(ih);
@@ -362,13 +502,10 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     }
     a.dtdx[s] = gx; a.dtdy[s] = gy; a.dtdz[s] = gz;
   }
-  // wave-aggregated counts (every lane of the wavefront reaches this point)
-  unsigned long long m = __ballot(still);
-  unsigned long long pairs = wave_sum((unsigned long long)(act ? a.cnt[s] : 0));
-  if ((threadIdx.x & 63) == 0) {
-    if (m) atomicAdd(a.active, __popcll(m));
-    if (pairs) atomicAdd(a.pairs, pairs);
-  }
+  long long act_n = block_reduce<R_SUM>(still ? 1 : 0, lds);
+  if (threadIdx.x == 0 && act_n) atomicAdd((unsigned long long *)&a.stripes[stripe()], (unsigned long long)act_n);
+  long long pairs = block_reduce<R_SUM>(nn, lds);
+  if (threadIdx.x == 0 && pairs) atomicAdd((unsigned long long *)&a.stripes[kStripes + stripe()], (unsigned long long)pairs);
 }
 
 // c1simple (circuit1.f90:227-305): direct summation, h = hfac (m/rho)^(1/dim),
@@ -387,23 +524,29 @@ __global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fiel
   PT dsum = 0, g0 = 0, g1 = 0, g2 = 0, h0x = 0, h0y = 0, h0z = 0;
   bool qbad = !(ha > 0.0);
   const int nn = a.cnt[s];
-  for (int k = 0; k < nn; k++) {
-    uint32_t ent = a.nb[(size_t)k * a.npad + s];
-    uint32_t j = nb_index(ent), code = nb_code(ent);
-    double4 pb = a.pm[j];
-    double xb = pb.x, yb = pb.y, zb = pb.z;
-    if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+  struct J { double4 pb, sb; uint32_t code; };
+  auto load = [&](uint32_t ent) {
+    J d;
+    const uint32_t j = nb_index(ent);
+    d.code = nb_code(ent);
+    d.pb = a.pm[j];
+    if constexpr (GRAD != 0) d.sb = a.cur[j];
+    return d;
+  };
+  for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
+    double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
+    if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
     PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
     PT dr = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
     PT q = dr * inv_ha;
-    PT mb = (PT)pb.w;
+    PT mb = (PT)d.pb.w;
     if constexpr (GRAD == 0) {
       dsum += mb * K::f(q);
     } else {
       PT f, df;
       K::f_df(q, f, df);
       dsum += mb * f;
-      double4 sb = a.cur[j];
+      const double4 sb = d.sb;
       PT inv_dr = PT(1) / dr;
       if constexpr (GRAD == 1) {  // 2nw_ds: m_b/rho_a (T_b - T_a) nabla W_a
         PT w = mb * ((PT)sb.w - (PT)ta) * df * inv_dr;
@@ -418,7 +561,7 @@ __global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fiel
       }
     }
     qbad |= !(q >= PT(0));
-  }
+  });
   if (qbad) atomicOr(a.err, EB_Q);
   const double wcv = a.p.wcv, ih = 1.0 / ha;
   double f0 = KernelFn<KT, double>::f(0.0);
@@ -439,33 +582,44 @@ __global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fiel
 }
 
 // After loop 1: write back h/den/om of real particles (circuit1.f90:213-215),
-// the EOS (eos.f90:21-45) and the loop-2 j-side packs.
+// the EOS (eos.f90:21-45) and the loop-2 j-side packs; flags[stripe] bit 1 =
+// some velocity is non-zero, bit 2 = some B differs from particle 0's.
 template <typename V4>
-__global__ void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n, int advanced, int do_eos,
-                        int iso, double c1, double gamma, V4 *q0, V4 *q1, V4 *q2, V4 *q3, V4 *q4) {
-  int s = gtid();
-  if (s >= n) return;
+__global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n,
+                                               int advanced, int do_eos, int iso, double c1, double gamma, V4 *q0,
+                                               V4 *q1, V4 *q2, V4 *q3, V4 *q4, long long *flags) {
+  __shared__ long long lds[4];
   using T = decltype(V4().x);
-  bool r = real[s];
-  if (r && advanced) {
-    double4 v = fin[s];
-    F.f[F_H][s] = v.x; F.f[F_DEN][s] = v.y; F.f[F_OM][s] = v.z;
-  }
-  double den = F.f[F_DEN][s];
-  if (r && do_eos) {
-    if (iso) {
-      F.f[F_P][s] = den * c1 * c1;
-    } else {
-      double p = (gamma - 1.) * den * F.f[F_U][s];
-      F.f[F_P][s] = p;
-      F.f[F_C][s] = sqrt(gamma * p / den);
+  int s = gtid();
+  int fl = 0;
+  if (s < n) {
+    bool r = real[s];
+    if (r && advanced) {
+      double4 v = fin[s];
+      F.f[F_H][s] = v.x; F.f[F_DEN][s] = v.y; F.f[F_OM][s] = v.z;
     }
+    double den = F.f[F_DEN][s];
+    if (r && do_eos) {
+      if (iso) {
+        F.f[F_P][s] = den * c1 * c1;
+      } else {
+        double p = (gamma - 1.) * den * F.f[F_U][s];
+        F.f[F_P][s] = p;
+        F.f[F_C][s] = sqrt(gamma * p / den);
+      }
+    }
+    double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
+    double bx = F.f[F_BX][s], by = F.f[F_BY][s], bz = F.f[F_BZ][s];
+    if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
+    if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;
+    q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_T][s]};
+    if (q1) q1[s] = V4{(T)vx, (T)vy, (T)vz, (T)F.f[F_P][s]};
+    if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
+    if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
+    if (q4) q4[s] = V4{(T)bx, (T)by, (T)bz, (T)0};
   }
-  q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_P][s]};
-  if (q1) q1[s] = V4{(T)F.f[F_VX][s], (T)F.f[F_VY][s], (T)F.f[F_VZ][s], (T)F.f[F_C][s]};
-  if (q2) q2[s] = V4{(T)F.f[F_U][s], (T)F.f[F_T][s], (T)F.f[F_KAPPA][s], (T)0};
-  if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
-  if (q4) q4[s] = V4{(T)F.f[F_BX][s], (T)F.f[F_BY][s], (T)F.f[F_BZ][s], (T)0};
+  long long t = block_reduce<R_OR>(fl, lds);
+  if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
 }
 
 // ---------------------------------------------------------------- loop 2
@@ -492,20 +646,46 @@ __device__ __forceinline__ Sym3<T> cond_tensor(const Phys &p, T ux, T uy, T uz) 
   return k;
 }
 
+template <typename T>
+__device__ __forceinline__ Sym3<T> tensor_of_b(const Phys &p, T bx, T by, T bz) {
+  T b2 = bx * bx + by * by + bz * bz;
+  T ux = 0, uy = 0, uz = 0;
+  if (b2 != T(0)) { T r = T(1) / sqrt(b2); ux = bx * r; uy = by * r; uz = bz * r; }
+  return cond_tensor<T>(p, ux, uy, uz);
+}
+
+// sum_lm kt_lm r_l r_m / r^2 (S) and trace (Tr) over the dim x dim block
+template <int DIM, typename T>
+__device__ __forceinline__ void tensor_contract(const Sym3<T> &kt, T rx, T ry, T rz, T inv_r2, T &S, T &Tr) {
+  S = kt.xx * rx * rx;
+  Tr = kt.xx;
+  if (DIM >= 2) { S += kt.yy * ry * ry + T(2) * kt.xy * rx * ry; Tr += kt.yy; }
+  if (DIM == 3) { S += kt.zz * rz * rz + T(2) * (kt.xz * rx * rz + kt.yz * ry * rz); Tr += kt.zz; }
+  S *= inv_r2;
+}
+
+
 // c2 (circuit2.f90:58-505) for one real particle per lane over the loop-1
-// neighbour list, post-loop-1 h/den/om and post-EOS p/c (Q2 of the spec).
-template <int DIM, int KT, typename PT, typename V4>
+// neighbour list, post-loop-1 h/den/om and post-EOS p/c.
+template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
   int s = gtid();
   double e_part = 0.0;
   if (s < a.n && a.real[s]) {
     using K = KernelFn<KT, PT>;
     const Phys &P = a.p;
-    const bool need_v = P.adden || P.artts || P.hydro || P.magneto;
-    const bool need_q3 = P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
-    const bool need_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
-    const bool need_q2 = P.diff || (P.artts && P.au != -1.0);
-    const bool hess = P.diff && (P.ddw == 400 || P.ddw == 401 || P.ddw == 402);
+    const bool hydro = MODE == MODE_GENERIC && P.hydro, artts = MODE == MODE_GENERIC && P.artts;
+    const bool magneto = MODE == MODE_GENERIC && P.magneto, fld = MODE == MODE_GENERIC && P.fld;
+    const bool diff = MODE == MODE_DIFF || P.diff;
+    // optional j-side packs; a pack the host did not build is never read
+    const bool need_v = (P.adden || artts || hydro || magneto) && (a.v_nonzero || artts || hydro || magneto) &&
+                        a.q1 != nullptr;
+    const bool need_q2 = MODE == MODE_GENERIC && ((diff && fld) || artts) && a.q2 != nullptr;
+    const bool need_q3 = MODE == MODE_GENERIC && diff && (fld || P.ddw == 403 || P.ddw == 404) && a.q3 != nullptr;
+    const bool tensor_b = diff && !fld && P.diso == 0 && !a.b_uniform && a.q4 != nullptr;
+    const bool need_q4 = (magneto || tensor_b) && a.q4 != nullptr;
+    const bool hess = diff && (P.ddw == 400 || P.ddw == 401 || P.ddw == 402);
+    const bool need_dfb = MODE == MODE_GENERIC && (hydro || artts || magneto || P.ddw == 403);
     const double4 pa = a.pm[s];
     // i-side values straight from the FP64 fields (coalesced, read once)
     const double *const *Fi = a.out.f;
@@ -519,13 +699,11 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
     const PT rho2oma = rhoa * rhoa * oma;
-    PT ub2 = bax * bax + bay * bay + baz * baz;
-    PT ubax = 0, ubay = 0, ubaz = 0;
-    if (ub2 != PT(0)) { PT r = PT(1) / sqrt(ub2); ubax = bax * r; ubay = bay * r; ubaz = baz * r; }
+    const PT ub2 = bax * bax + bay * bay + baz * baz;
     Sym3<PT> kta = {0, 0, 0, 0, 0, 0}, ktb_fixed = {0, 0, 0, 0, 0, 0};
-    PT mu0inv = P.magneto ? (PT)(1.0 / P.muzero) : PT(0);
-    if (P.diff) {
-      if (P.fld) {
+    const PT mu0inv = magneto ? (PT)(1.0 / P.muzero) : PT(0);
+    if (diff) {
+      if (fld) {
         PT gn = sqrt(gax * gax + gay * gay + gaz * gaz);
         PT fr = gn / (kappaa * rhoa * ta * rhoa);
         PT lam = (PT(2) + fr) / (PT(6) + PT(3) * fr + fr * fr);
@@ -533,53 +711,68 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
         kta.xx = kta.yy = kta.zz = d;
         a.out.f[F_VFLD][s] = (double)(d / ((ta * rhoa) / gn));
       } else {
-        kta = cond_tensor<PT>(P, ubax, ubay, ubaz);
-        if (P.diso == 1) ktb_fixed = kta;
+        kta = tensor_of_b<PT>(P, bax, bay, baz);
+        ktb_fixed = kta;  // isotropic, or every particle has the same B
       }
     }
-    // kappa_a grad T_a for the 2nw forms (circuit2.f90:401-403)
+    // tensor and operator constants of the i side
     PT kdax = kta.xx * gax + kta.xy * gay + kta.xz * gaz;
     PT kday = kta.xy * gax + kta.yy * gay + kta.yz * gaz;
     PT kdaz = kta.xz * gax + kta.yz * gay + kta.zz * gaz;
     PT dvx = 0, dvy = 0, dvz = 0, du = 0, ddt = 0, dbx = 0, dby = 0, dbz = 0, drho = 0;
     bool qbad = !(inv_ha > PT(0));
     const int nn = a.cnt[s];
-    for (int k = 0; k < nn; k++) {
-      uint32_t ent = a.nb[(size_t)k * a.npad + s];
-      uint32_t j = nb_index(ent), code = nb_code(ent);
-      double4 pb = a.pm[j];
+    // batched: positions/mass and {1/h, den, om, T}; the optional packs
+    // (v, c/u/kappa, grad T, B) are gathered in the body when enabled
+    struct J { double4 pb; V4 q0; uint32_t j, code; };
+    auto load = [&](uint32_t ent) {
+      J d;
+      d.j = nb_index(ent);
+      d.code = nb_code(ent);
+      d.pb = a.pm[d.j];
+      d.q0 = a.q0[d.j];
+      return d;
+    };
+    constexpr int NBATCH = MODE == MODE_DIFF ? 4 : 1;
+    for_neighbours<NBATCH>(a.nb, s, a.npad, nn, load, [&](const J &d) {
+      const double4 pb = d.pb;
+      const uint32_t code = d.code;
       double xb = pb.x, yb = pb.y, zb = pb.z;
       if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
       const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
       const PT mb = (PT)pb.w;
-      const V4 B0 = a.q0[j];
-      const PT inv_hb = (PT)B0.x, rhob = (PT)B0.y, omb = (PT)B0.z, pbb = (PT)B0.w;
+      const V4 B0 = d.q0;
+      const PT inv_hb = (PT)B0.x, rhob = (PT)B0.y, omb = (PT)B0.z, tb = (PT)B0.w;
       const PT r2 = rx * rx + ry * ry + rz * rz;
+      const PT inv_r2 = PT(1) / r2;
       const PT dr = sqrt(r2);
-      const PT inv_dr = PT(1) / dr;
-      const PT qa = dr * inv_ha, qb = dr * inv_hb;
-      PT dfa, dfb;
-      PT ddfa = 0, fa = 0;
+      const PT inv_dr = dr * inv_r2;
+      const PT qa = dr * inv_ha;
+      PT dfa, ddfa = 0, fa = 0;
       if (hess && P.ddw == 401) K::df_ddf(qa, dfa, ddfa);
       else if (hess && P.ddw == 402) K::f_df(qa, fa, dfa);
       else dfa = K::df(qa);
-      dfb = K::df(qb);
-      qbad |= !(qb >= PT(0));
-      // nabla W = wcv df r / (h^(dim+1) dr)   (kernel.f90:189-197)
+      qbad |= !(qa >= PT(0));
+      // nabla W_a = wcv df r / (h^(dim+1) dr)   (kernel.f90:189-197)
       const PT ga = wcv * dfa * ihd1 * inv_dr;
-      const PT gb = wcv * dfb * ipow<DIM + 1>(inv_hb) * inv_dr;
       const PT nwax = ga * rx, nway = ga * ry, nwaz = ga * rz;
+      PT gb = 0, dfb = 0;
+      if (need_dfb) {
+        const PT qb = dr * inv_hb;
+        dfb = K::df(qb);
+        gb = wcv * dfb * ipow<DIM + 1>(inv_hb) * inv_dr;
+      }
       const PT nwbx = gb * rx, nwby = gb * ry, nwbz = gb * rz;
-      PT vabx = 0, vaby = 0, vabz = 0, cb = 0;
+      PT vabx = 0, vaby = 0, vabz = 0, pbb = 0;
       if (need_v) {
-        V4 t = a.q1[j];
-        vabx = vax - (PT)t.x; vaby = vay - (PT)t.y; vabz = vaz - (PT)t.z; cb = (PT)t.w;
+        const V4 t = a.q1[d.j];
+        vabx = vax - (PT)t.x; vaby = vay - (PT)t.y; vabz = vaz - (PT)t.z; pbb = (PT)t.w;
       }
       const PT vnwa = vabx * nwax + vaby * nway + vabz * nwaz;
       if (P.adden) drho += mb * vnwa;
-      PT ub = 0, tb = 0, kappab = 0;
-      if (need_q2) { V4 t = a.q2[j]; ub = (PT)t.x; tb = (PT)t.y; kappab = (PT)t.z; }
-      if (P.artts) {
+      PT cb = 0, ub = 0, kappab = 0;
+      if (need_q2) { const V4 t = a.q2[d.j]; cb = (PT)t.x; ub = (PT)t.y; kappab = (PT)t.z; }
+      if (artts) {
         // Fab = wcv df dr / h^(dim+2) / q = wcv df / h^(dim+1)   (kernel.f90:251-259)
         const PT faba = wcv * dfa * ihd1;
         const PT fabb = wcv * dfb * ipow<DIM + 1>(inv_hb);
@@ -600,7 +793,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
         }
         du += mb * (dvr * qva * faba / oma / rhoa + qc);
       }
-      if (P.hydro) {
+      if (hydro) {
         const PT ta_ = pa_ / rho2oma, tb_ = pbb / (rhob * rhob * omb);
         dvx -= mb * (ta_ * nwax + tb_ * nwbx);
         dvy -= mb * (ta_ * nway + tb_ * nwby);
@@ -608,32 +801,32 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
         du += mb * pa_ * vnwa / (rhoa * rhoa) / oma;
       }
       PT bbx = 0, bby = 0, bbz = 0;
-      if (need_q4) { V4 t = a.q4[j]; bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z; }
-      if (P.diff) {
+      if (need_q4) { const V4 t = a.q4[d.j]; bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z; }
+      if (diff) {
         Sym3<PT> ktb = ktb_fixed;
         PT gbx = 0, gby = 0, gbz = 0;
-        if (need_q3) { V4 t = a.q3[j]; gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z; }
-        if (P.fld) {
+        if (need_q3) { const V4 t = a.q3[d.j]; gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z; }
+        if (fld) {
           PT gn = sqrt(gbx * gbx + gby * gby + gbz * gbz);
           PT fr = gn / (kappab * rhob * rhob * tb);
           PT lam = (PT(2) + fr) / (PT(6) + PT(3) * fr + fr * fr);
-          PT d = (PT)kLightspeed * lam / kappab / rhob;
-          ktb.xx = ktb.yy = ktb.zz = d; ktb.xy = ktb.xz = ktb.yz = 0;
-        } else if (P.diso == 0) {
-          PT b2 = bbx * bbx + bby * bby + bbz * bbz;
-          PT ux = 0, uy = 0, uz = 0;
-          if (b2 != PT(0)) { PT r = PT(1) / sqrt(b2); ux = bbx * r; uy = bby * r; uz = bbz * r; }
-          ktb = cond_tensor<PT>(P, ux, uy, uz);
+          PT dfl = (PT)kLightspeed * lam / kappab / rhob;
+          ktb.xx = ktb.yy = ktb.zz = dfl; ktb.xy = ktb.xz = ktb.yz = 0;
+        } else if (tensor_b) {
+          ktb = tensor_of_b<PT>(P, bbx, bby, bbz);
         }
         if (hess) {
           // sum_lm ((kta+ktb)/2)_lm H_lm over the dim x dim block
-          Sym3<PT> kt;
-          kt.xx = (kta.xx + ktb.xx) * PT(0.5); kt.yy = (kta.yy + ktb.yy) * PT(0.5); kt.zz = (kta.zz + ktb.zz) * PT(0.5);
-          kt.xy = (kta.xy + ktb.xy) * PT(0.5); kt.xz = (kta.xz + ktb.xz) * PT(0.5); kt.yz = (kta.yz + ktb.yz) * PT(0.5);
-          PT S = kt.xx * rx * rx, Tr = kt.xx;
-          if (DIM >= 2) { S += kt.yy * ry * ry + PT(2) * kt.xy * rx * ry; Tr += kt.yy; }
-          if (DIM == 3) { S += kt.zz * rz * rz + PT(2) * (kt.xz * rx * rz + kt.yz * ry * rz); Tr += kt.zz; }
-          S = S / r2;
+          PT S, Tr;
+          if (!tensor_b && !fld) {
+            tensor_contract<DIM, PT>(kta, rx, ry, rz, inv_r2, S, Tr);
+          } else {
+            Sym3<PT> kt;
+            kt.xx = (kta.xx + ktb.xx) * PT(0.5); kt.yy = (kta.yy + ktb.yy) * PT(0.5);
+            kt.zz = (kta.zz + ktb.zz) * PT(0.5); kt.xy = (kta.xy + ktb.xy) * PT(0.5);
+            kt.xz = (kta.xz + ktb.xz) * PT(0.5); kt.yz = (kta.yz + ktb.yz) * PT(0.5);
+            tensor_contract<DIM, PT>(kt, rx, ry, rz, inv_r2, S, Tr);
+          }
           PT contr;
           if (P.ddw == 401) {
             // hessian_ddw_cart, kernel.f90:377-417
@@ -647,7 +840,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
             contr = half * (PT(DIM + 2) * S - Tr);
           }
           ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
-        } else {
+        } else if (MODE == MODE_GENERIC) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
           PT kdbz = ktb.xz * gbx + ktb.yz * gby + ktb.zz * gbz;
@@ -660,7 +853,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
           }
         }
       }
-      if (P.magneto) {
+      if (magneto) {
         // stress tensors (circuit2.f90:166-175, 433-441), force (:443-454), induction (:458-461)
         PT b2a = ub2, b2b = bbx * bbx + bby * bby + bbz * bbz;
         PT ma_x = ((PT(0.5) * b2a - bax * bax) * nwax - bax * bay * nway - bax * baz * nwaz) * mu0inv;
@@ -679,7 +872,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
         dby -= f * (vaby * bnwa - bby * vnwa);
         dbz -= f * (vabz * bnwa - bbz * vnwa);
       }
-    }
+    });
     if (qbad) atomicOr(a.err, EB_Q);
     double dh = 0.0;
     if (P.adden) dh = -DIM * ha_d / (double)rhoa * (double)drho / (double)oma;
@@ -689,11 +882,11 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
     a.out.f[F_DBX][s] = (double)dbx; a.out.f[F_DBY][s] = (double)dby; a.out.f[F_DBZ][s] = (double)dbz;
     a.out.f[F_DU][s] = (double)du;
     // FP64 energy sum (circuit2.f90:490-492)
-    double vx = (double)vax, vy = (double)vay, vz = (double)vaz;
-    double bx = (double)bax, by = (double)bay, bz = (double)baz, rh = (double)rhoa;
+    double vx = Fi[F_VX][s], vy = Fi[F_VY][s], vz = Fi[F_VZ][s];
+    double bx = Fi[F_BX][s], by = Fi[F_BY][s], bz = Fi[F_BZ][s], rh = Fi[F_DEN][s];
     e_part = pa.w * (vx * (double)dvx + vy * (double)dvy + vz * (double)dvz + (double)du +
                      (bx * (double)dbx + by * (double)dby + bz * (double)dbz) / rh -
-                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * (double)drho / (double)oma);
+                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * (double)drho / Fi[F_OM][s]);
   }
   __shared__ double red[4];
   double w = wave_sum(e_part);
@@ -732,7 +925,7 @@ __global__ void k_neigh_export(const uint32_t *nb, const int *cnt, const int64_t
   if (s >= n || !real[s]) return;
   int64_t o = row_off_by_orig[orig[s]];
   for (int k = 0; k < cnt[s]; k++) {
-    uint32_t e = nb[(size_t)k * npad + s];
+    uint32_t e = nb[nb_slot(k, s, npad)];
     jout[o + k] = (int32_t)orig[nb_index(e)];
     cout[o + k] = (int8_t)nb_code(e);
   }
@@ -785,8 +978,10 @@ __global__ void k_correct(Fields F, const uint8_t *real, int n, double dt, int d
 }
 
 // Per-real-particle minima/maxima for timestep.calc (timestep.f90:62-81):
-// red (ordered int64): 0 min h, 1 max c, 2 min den, 3 min c, 4 max kappa, 5 max |B_l|
-__global__ void k_dt_reduce(Fields F, const uint8_t *real, int n, long long *red) {
+// red[v * kStripes + stripe] (ordered int64): 0 min h, 1 max c, 2 min den,
+// 3 min c, 4 max kappa, 5 max of the B components
+__global__ __launch_bounds__(256) void k_dt_reduce(Fields F, const uint8_t *real, int n, long long *red) {
+  __shared__ long long lds[4];
   int s = gtid();
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
   long long v[6] = {BIG, SMALL, BIG, BIG, SMALL, SMALL};
@@ -799,23 +994,24 @@ __global__ void k_dt_reduce(Fields F, const uint8_t *real, int n, long long *red
     double bm = fmax(fmax(F.f[F_BX][s], F.f[F_BY][s]), F.f[F_BZ][s]);
     v[5] = d2ord(bm);
   }
-  v[0] = wave_min(v[0]); v[2] = wave_min(v[2]); v[3] = wave_min(v[3]);
-  v[1] = wave_max(v[1]); v[4] = wave_max(v[4]); v[5] = wave_max(v[5]);
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&red[0], v[0]); atomicMax(&red[1], v[1]); atomicMin(&red[2], v[2]);
-    atomicMin(&red[3], v[3]); atomicMax(&red[4], v[4]); atomicMax(&red[5], v[5]);
+  const bool is_min[6] = {true, false, true, true, false, false};
+  for (int k = 0; k < 6; k++) {
+    long long t = is_min[k] ? block_reduce<R_MIN>(v[k], lds) : block_reduce<R_MAX>(v[k], lds);
+    if (threadIdx.x == 0) {
+      if (is_min[k]) atomicMin(&red[k * kStripes + stripe()], t);
+      else atomicMax(&red[k * kStripes + stripe()], t);
+    }
   }
 }
 
 // RKL2 stage combination on T (sts_integrator.f90:111-115) over every
 // particle slot, as the reference updates whole store rows.
-__global__ void k_sts_stage(double *T, const double *y0, const double *yjm1, const double *yjm2,
-                            const double *dy0, const double *dyjm1, int n, double mu, double nu,
-                            double muh_dt, double gmh_dt, double *yj) {
+__global__ void k_sts_stage(const double *y0, const double *yjm1, const double *yjm2, const double *dy0,
+                            const double *dyjm1, int n, double mu, double nu, double muh_dt, double gmh_dt,
+                            double *yj) {
   int s = gtid();
   if (s >= n) return;
   yj[s] = mu * yjm1[s] + nu * yjm2[s] + (1. - mu - nu) * y0[s] + muh_dt * dyjm1[s] + gmh_dt * dy0[s];
-  (void)T;
 }
 
 __global__ void k_axpy_copy(double *dst, const double *x, const double *y, double a, int n) {
@@ -838,19 +1034,25 @@ __global__ void k_axpy_copy(double *dst, const double *x, const double *y, doubl
   template __global__ void k_loop1_simple<DIM, KT, 0, float>(Loop1Args<double4>, Fields);    \
   template __global__ void k_loop1_simple<DIM, KT, 1, float>(Loop1Args<double4>, Fields);    \
   template __global__ void k_loop1_simple<DIM, KT, 2, float>(Loop1Args<double4>, Fields);    \
-  template __global__ void k_loop2<DIM, KT, double, double4>(Loop2Args<double4>);            \
-  template __global__ void k_loop2<DIM, KT, float, float4>(Loop2Args<float4>);
+  template __global__ void k_loop2<DIM, KT, double, double4, MODE_GENERIC>(Loop2Args<double4>); \
+  template __global__ void k_loop2<DIM, KT, float, float4, MODE_GENERIC>(Loop2Args<float4>);    \
+  template __global__ void k_loop2<DIM, KT, double, double4, MODE_DIFF>(Loop2Args<double4>);    \
+  template __global__ void k_loop2<DIM, KT, float, float4, MODE_DIFF>(Loop2Args<float4>);
 
 SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
 SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
 
-template __global__ void k_neigh<1>(NeighArgs);
-template __global__ void k_neigh<2>(NeighArgs);
-template __global__ void k_neigh<3>(NeighArgs);
+template __global__ void k_neigh<1, 1>(NeighArgs);
+template __global__ void k_neigh<2, 1>(NeighArgs);
+template __global__ void k_neigh<3, 1>(NeighArgs);
+template __global__ void k_neigh<1, 2>(NeighArgs);
+template __global__ void k_neigh<2, 2>(NeighArgs);
+template __global__ void k_neigh<3, 2>(NeighArgs);
 template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
-                                          double, double4 *, double4 *, double4 *, double4 *, double4 *);
+                                          double, double4 *, double4 *, double4 *, double4 *, double4 *,
+                                          long long *);
 template __global__ void k_post1<float4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
-                                         double, float4 *, float4 *, float4 *, float4 *, float4 *);
+                                         double, float4 *, float4 *, float4 *, float4 *, float4 *, long long *);
 
 }  // namespace sph

@@ -19,6 +19,7 @@ struct Grid {
   double inv_cs[3];
   int nc[3];
   int periodic[3];  // 1 if the axis is imaged (cells wrap with a shift)
+  int mdim;         // axes interleaved into the Morton sort key (= dim)
 };
 
 // Per-call physics flags and constants (circuit1.f90:38-51, circuit2.f90:45-56,86-90).
