@@ -55,6 +55,47 @@ def run_path(state, st, precision, steps, warmup, torch):
     return eng, per, t1 - t0
 
 
+def adjoint_cost(size=64, s=5, nsteps=20, every=5, reps=3):
+    """Adjoint/forward cost of the differentiable diffusion: 64^3 particles,
+    20 RKL2 super-steps of s stages, checkpoints every 5 steps, loss
+    0.5 |T - T*|^2 with a seeded target; gradients w.r.t. kappa_par,
+    kappa_perp, the field angles and T0."""
+    import math
+    import numpy as np
+    import torch
+    from sph_playground_amd import autograd as AG
+    from sph_playground_amd import consts as C
+    from sph_playground_amd import problems as P
+    from sph_playground_amd.engine import Engine
+    state, st = P.diffusion3d(size, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    eng = Engine(state, kernel="m6", tensor="full", kappa_perp=1e-3)
+    eng.load(st)
+    AG.prepare(eng)
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda().requires_grad_(True)
+    target = torch.from_numpy(P.gaussian(5, st.shape[0]) * 0.1).cuda()
+    params = [torch.tensor(v, dtype=torch.float64, requires_grad=True)
+              for v in (1.0, 1e-3, math.pi / 2, math.pi / 4)]
+    dt = 1e-6
+    fw, bw = [], []
+    for _ in range(reps):
+        K = AG.conduction_tensor(*params)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        T = AG.sts(T0, K, eng, s, dt, nsteps, every)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = 0.5 * torch.sum((T - target) ** 2)
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fw.append(t1 - t0)
+        bw.append(t2 - t1)
+    f, b = float(np.median(fw)), float(np.median(bw))
+    return {"workload": "%d^3 = %d particles, %d super-steps x %d stages, checkpoint every %d, FP64"
+                        % (size, size ** 3, nsteps, s, every),
+            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "adjoint_over_forward": b / f}
+
+
 def cpu_baseline(size):
     """Reference Fortran hot path (oracle/_ref, built from the reference
     sources) on a bounded sample of the same workload, all host cores."""
@@ -86,6 +127,7 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -144,6 +186,11 @@ def main():
                     peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=None,
                     bytes_per_launch=byts, ms_per_launch=kmx[dom])
 
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint cost")
+        adj = adjoint_cost()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -153,7 +200,7 @@ def main():
             cpu = dict(value=None, unit="pair-interactions/s", cores=None, kind=None, sample="failed: %s" % ex)
 
     line = {
-        "metric": "pair-interactions/s, 2M-particle density+force step; mixed vs FP64",
+        "metric": "pair-interactions/s, 2M-particle density+force step; mixed vs FP64; adjoint/fwd",
         "value": vmx,
         "unit": "pair-interactions/s",
         "n_gpus": 1,
@@ -172,6 +219,7 @@ def main():
         "mixed": {"value": vmx, "kernels_ms": kmx, "density_max_rel_err_vs_fp64": den_err,
                   "ddt_max_err_rel_to_max": ddt_err},
         "mixed_vs_fp64": vmx / v64,
+        "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -114,6 +114,31 @@ int sph_sts(sph_ctx *ctx, const double state[49], int s, double dt);
  * out[0..3] = mu(j), nu(j), mu_hat(j,s), gm_hat(j,s); j >= 1. */
 int sph_sts_coeffs(int j, int s, double out[4]);
 
+/* ---- Differentiable anisotropic diffusion (no reference counterpart: the
+ * reference has no derivatives).  For particles at rest the loop-2 diffusion
+ * term with a global tensor K = (xx, yy, zz, xy, xz, yz) is a linear operator
+ * dT/dt = A(K) T (circuit2.f90:376-397 with kta = ktb = K); these entry points
+ * apply it, its transpose, and differentiate RKL2 super-steps through it.
+ * Vectors are device pointers of n_real doubles in original particle order.
+ *
+ * sph_diff_prepare: one iterate on the resident particles (neighbours, Newton
+ * h, EOS), after which h, rho and Omega are frozen; builds reverse lists.
+ * Needs FP64 precision, no fixed particles, a diffusion equation set with a
+ * Hessian operator (fab, fw, n2w), no FLD. */
+int sph_diff_prepare(sph_ctx *ctx, const double state[49]);
+/* d_out = A(K) d_T */
+int sph_diff_apply(sph_ctx *ctx, const double K[6], const double *d_T, double *d_out);
+/* d_out = A(K)^T d_lam; gA[k] = lam^T dA/dK_k Ya, gB[k] = lam^T dA/dK_k Yb */
+int sph_diff_apply_adjoint(sph_ctx *ctx, const double K[6], const double *d_lam, const double *d_Ya,
+                           const double *d_Yb, double *d_out, double gA[6], double gB[6]);
+/* nsteps RKL2 super-steps of s stages (sts_integrator.f90:87-124) from d_T0
+ * into d_T, keeping T every `every` steps as checkpoints for the backward. */
+int sph_diff_forward(sph_ctx *ctx, const double K[6], int s, double dt, int nsteps, int every,
+                     const double *d_T0, double *d_T);
+/* Adjoint of the last sph_diff_forward: given dL/dT_final, dL/dT0 and dL/dK
+ * (recomputing each checkpoint segment). */
+int sph_diff_backward(sph_ctx *ctx, const double *d_gT, double *d_gT0, double gK[6]);
+
 int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
 const char *sph_error_string(int code);
 

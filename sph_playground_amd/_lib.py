@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libsph_hip.so")
 EXPORTS = (
     "sph_default_options", "sph_ctx_create", "sph_ctx_destroy", "sph_iterate", "sph_iterate_device",
     "sph_load", "sph_save", "sph_step", "sph_neighbours", "sph_leapfrog", "sph_sts", "sph_sts_coeffs",
-    "sph_get_stats", "sph_error_string",
+    "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
+    "sph_diff_forward", "sph_diff_backward",
 )
 
 SPH_OK = 0
@@ -73,6 +74,11 @@ def lib():
     L.sph_sts.argtypes = [P, D, ct.c_int, ct.c_double]
     L.sph_sts_coeffs.argtypes = [ct.c_int, ct.c_int, D]
     L.sph_get_stats.argtypes = [P, ct.POINTER(Stats)]
+    L.sph_diff_prepare.argtypes = [P, D]
+    L.sph_diff_apply.argtypes = [P, D, ct.c_void_p, ct.c_void_p]
+    L.sph_diff_apply_adjoint.argtypes = [P, D, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, D, D]
+    L.sph_diff_forward.argtypes = [P, D, ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
+    L.sph_diff_backward.argtypes = [P, ct.c_void_p, ct.c_void_p, D]
     L.sph_error_string.argtypes = [ct.c_int]
     L.sph_error_string.restype = ct.c_char_p
     for name in EXPORTS:

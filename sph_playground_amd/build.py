@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsph_hip.so")
-SOURCES = ["sph_kernels.hip", "sph_api.hip"]
+SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "sph_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPH_OFFLOAD_ARCH", "gfx950")
 

@@ -54,6 +54,17 @@ __global__ void k_dt_reduce(Fields, const uint8_t *, int, long long *);
 __global__ void k_sts_stage(const double *, const double *, const double *, const double *, const double *, int,
                             double, double, double, double, double *);
 __global__ void k_axpy_copy(double *, const double *, const double *, double, int);
+template <int DIM, int KT>
+__global__ void k_diff_apply(DiffOpArgs, const double *, double *);
+template <int DIM, int KT>
+__global__ void k_diff_adjoint(DiffOpArgs, const double *, const double *, const double *, double *, double *);
+__global__ void k_rev_count(const uint8_t *, const uint32_t *, const int *, int, int, int *);
+__global__ void k_rev_fill(const uint8_t *, const uint32_t *, const int *, int, int, const int *, int *, uint32_t *);
+__global__ void k_rev_sort(const int *, int, uint32_t *);
+__global__ void k_to_sorted(const double *, const int64_t *, int, double *);
+__global__ void k_from_sorted(const double *, const int64_t *, int, double *);
+__global__ void k_lincomb(int, double *, double, const double *, double, const double *, double, const double *,
+                          double, const double *, double, const double *);
 }  // namespace sph
 
 namespace {
@@ -126,6 +137,20 @@ struct sph_ctx {
   sph_stats stats{};
   bool have_step = false;
   hipEvent_t ev[8]{};
+  // differentiable diffusion (sph_diff_*)
+  bool diff_ready = false;
+  Phys diff_phys{};
+  int *roff = nullptr, *rtmp = nullptr;
+  uint32_t *rent = nullptr;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  double *dvec = nullptr;        // work vectors, diff_nvec x npad
+  int diff_nvec = 0;
+  double *gpart = nullptr;       // 12 partial sums per block
+  double *ckpt = nullptr;        // checkpoints of T, (nsteps / every + 1) x npad
+  int64_t ckpt_cap = 0;
+  int fw_s = 0, fw_steps = 0, fw_every = 0;
+  double fw_dt = 0.0, fw_K[6] = {0, 0, 0, 0, 0, 0};
 
   Fields fields() const {
     Fields F;
@@ -799,7 +824,8 @@ int sph_ctx_destroy(sph_ctx *c) {
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
                   c->sort_tmp, c->cstart, c->cend, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->band,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
-                  c->d_red, c->d_int, c->d_stripes, c->d_staging};
+                  c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
+                  c->dvec, c->gpart, c->ckpt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
@@ -1020,6 +1046,256 @@ const char *sph_error_string(int code) {
     case SPH_ERR_UNSUPPORTED: return "unsupported configuration";
     default: return "unknown error";
   }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- differentiable diffusion
+// Set-up, operator applications and the checkpointed RKL2 forward/backward of
+// the STS super-steps (sts_integrator.f90:87-124) with a global tensor K.
+static DiffOpArgs diff_args(sph_ctx *c, const double K[6]) {
+  DiffOpArgs a{};
+  a.pm = c->pm; a.q0 = (const double4 *)c->q[0]; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
+  a.roff = c->roff; a.rent = c->rent; a.b = c->box;
+  for (int k = 0; k < 6; k++) a.K[k] = K[k];
+  a.wcv = c->diff_phys.wcv; a.fwc = c->diff_phys.fwc; a.ddw = c->diff_phys.ddw;
+  a.n = (int)c->n; a.npad = (int)c->npad;
+  return a;
+}
+
+template <int DIM>
+static void launch_apply_d(sph_ctx *c, const DiffOpArgs &a, const double *Y, double *out) {
+  unsigned G = grid_for(c->n);
+  if (c->kt == KT_M4) hipLaunchKernelGGL((k_diff_apply<DIM, KT_M4>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
+  else if (c->kt == KT_M6) hipLaunchKernelGGL((k_diff_apply<DIM, KT_M6>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
+  else hipLaunchKernelGGL((k_diff_apply<DIM, KT_GAUSS>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
+}
+static void diff_apply(sph_ctx *c, const double K[6], const double *Y, double *out) {
+  DiffOpArgs a = diff_args(c, K);
+  if (c->diff_phys.dim == 1) launch_apply_d<1>(c, a, Y, out);
+  else if (c->diff_phys.dim == 2) launch_apply_d<2>(c, a, Y, out);
+  else launch_apply_d<3>(c, a, Y, out);
+}
+
+template <int DIM>
+static void launch_adj_d(sph_ctx *c, const DiffOpArgs &a, const double *lam, const double *Ya, const double *Yb,
+                         double *out) {
+  unsigned G = grid_for(c->n);
+  if (c->kt == KT_M4)
+    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_M4>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out, c->gpart);
+  else if (c->kt == KT_M6)
+    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_M6>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out, c->gpart);
+  else
+    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_GAUSS>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out,
+                       c->gpart);
+}
+// out = A^T lam; gA += lam^T dA/dK Ya, gB += lam^T dA/dK Yb (scaled by wa, wb)
+static int diff_adjoint(sph_ctx *c, const double K[6], const double *lam, const double *Ya, const double *Yb,
+                        double *out, double wa, double wb, double gK[6]) {
+  DiffOpArgs a = diff_args(c, K);
+  if (c->diff_phys.dim == 1) launch_adj_d<1>(c, a, lam, Ya, Yb, out);
+  else if (c->diff_phys.dim == 2) launch_adj_d<2>(c, a, lam, Ya, Yb, out);
+  else launch_adj_d<3>(c, a, lam, Ya, Yb, out);
+  unsigned G = grid_for(c->n);
+  std::vector<double> h((size_t)G * 12);
+  HIPCHK(hipMemcpyAsync(h.data(), c->gpart, sizeof(double) * 12 * G, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double sa[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0};
+  for (unsigned b = 0; b < G; b++)
+    for (int k = 0; k < 6; k++) { sa[k] += h[(size_t)b * 12 + k]; sb[k] += h[(size_t)b * 12 + 6 + k]; }
+  for (int k = 0; k < 6; k++) gK[k] += wa * sa[k] + wb * sb[k];
+  return SPH_OK;
+}
+
+static void lincomb(sph_ctx *c, double *y, double c0, const double *x0, double c1 = 0, const double *x1 = nullptr,
+                    double c2 = 0, const double *x2 = nullptr, double c3 = 0, const double *x3 = nullptr,
+                    double c4 = 0, const double *x4 = nullptr) {
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, (int)c->n, y, c0, x0, c1, x1, c2,
+                     x2, c3, x3, c4, x4);
+}
+
+static int ensure_dvec(sph_ctx *c, int nvec) {
+  if (nvec <= c->diff_nvec) return SPH_OK;
+  RCHK(dalloc(&c->dvec, (size_t)nvec * c->npad));
+  HIPCHK(hipMemset(c->dvec, 0, sizeof(double) * nvec * c->npad));
+  c->diff_nvec = nvec;
+  return SPH_OK;
+}
+static double *dv(sph_ctx *c, int k) { return c->dvec + (size_t)k * c->npad; }
+
+// One RKL2 super-step from Y[0] (sorted); stage states into Y[0..s]; work vector W.
+static void sts_forward_step(sph_ctx *c, const double K[6], int s, double dt, double *const *Y, double *DY0,
+                             double *DYm) {
+  diff_apply(c, K, Y[0], DY0);
+  lincomb(c, Y[1], 1.0, Y[0], sts_mu_hat(1, s) * dt, DY0);
+  for (int j = 2; j <= s; j++) {
+    diff_apply(c, K, Y[j - 1], DYm);
+    double mu = sts_mu(j), nu = sts_nu(j);
+    lincomb(c, Y[j], mu, Y[j - 1], nu, Y[j - 2], 1. - mu - nu, Y[0], sts_mu_hat(j, s) * dt, DYm,
+            sts_gm_hat(j, s) * dt, DY0);
+  }
+}
+
+// Adjoint of one super-step: lam[s] given (in L[s]); returns dL/dY0 in L[0].
+static int sts_backward_step(sph_ctx *c, const double K[6], int s, double dt, double *const *Y, double *const *L,
+                             double *U, double gK[6]) {
+  for (int j = s; j >= 2; j--) {
+    double mh = sts_mu_hat(j, s) * dt, gh = sts_gm_hat(j, s) * dt, mu = sts_mu(j), nu = sts_nu(j);
+    RCHK(diff_adjoint(c, K, L[j], Y[j - 1], Y[0], U, mh, gh, gK));
+    lincomb(c, L[j - 1], 1.0, L[j - 1], mu, L[j], mh, U);
+    lincomb(c, L[j - 2], 1.0, L[j - 2], nu, L[j]);
+    lincomb(c, L[0], 1.0, L[0], 1. - mu - nu, L[j], gh, U);
+  }
+  double m1 = sts_mu_hat(1, s) * dt;
+  RCHK(diff_adjoint(c, K, L[1], Y[0], Y[0], U, m1, 0.0, gK));
+  lincomb(c, L[0], 1.0, L[0], 1.0, L[1], m1, U);
+  return SPH_OK;
+}
+
+extern "C" {
+
+int sph_diff_prepare(sph_ctx *c, const double state[49]) {
+  if (!c || !state || c->n <= 0) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->opt.precision != 0 || c->n_fixed != 0) return SPH_ERR_UNSUPPORTED;
+  Phys P = call_phys(c, state);
+  if (!P.diff || P.fld || !(P.ddw == 400 || P.ddw == 401 || P.ddw == 402)) return SPH_ERR_UNSUPPORTED;
+  RCHK(run_step(c, state));  // neighbours, Newton h, EOS, loop 2: h, rho, q0 now frozen
+  c->diff_phys = P;
+  const int n = (int)c->n;
+  unsigned G = grid_for(n);
+  RCHK(dalloc(&c->roff, c->npad + 1));
+  RCHK(dalloc(&c->rtmp, c->npad + 1));
+  RCHK(dalloc(&c->rent, (size_t)std::max<int64_t>(c->stats.n_pairs, 1)));
+  HIPCHK(hipMemsetAsync(c->rtmp, 0, sizeof(int) * (n + 1), c->stream));
+  hipLaunchKernelGGL(k_rev_count, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
+                     c->rtmp);
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->rtmp, c->roff, n + 1, c->stream));
+  if (bytes > c->scan_tmp_bytes) {
+    RCHK(dev_alloc(&c->scan_tmp, bytes));
+    c->scan_tmp_bytes = bytes;
+  }
+  bytes = c->scan_tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, bytes, c->rtmp, c->roff, n + 1, c->stream));
+  HIPCHK(hipMemsetAsync(c->rtmp, 0, sizeof(int) * (n + 1), c->stream));
+  hipLaunchKernelGGL(k_rev_fill, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
+                     c->roff, c->rtmp, c->rent);
+  hipLaunchKernelGGL(k_rev_sort, dim3(G), dim3(kBlock), 0, c->stream, c->roff, n, c->rent);
+  if (c->gpart) (void)hipFree(c->gpart);
+  c->gpart = nullptr;
+  RCHK(dalloc(&c->gpart, (size_t)G * 12));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->diff_ready = true;
+  return SPH_OK;
+}
+
+int sph_diff_apply(sph_ctx *c, const double K[6], const double *d_T, double *d_out) {
+  if (!c || !c->diff_ready || !K || !d_T || !d_out) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  RCHK(ensure_dvec(c, 2));
+  const int n = (int)c->n;
+  unsigned G = grid_for(n);
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_T, c->orig[c->cur], n, dv(c, 0));
+  diff_apply(c, K, dv(c, 0), dv(c, 1));
+  hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, dv(c, 1), c->orig[c->cur], n, d_out);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+int sph_diff_apply_adjoint(sph_ctx *c, const double K[6], const double *d_lam, const double *d_Ya,
+                           const double *d_Yb, double *d_out, double gA[6], double gB[6]) {
+  if (!c || !c->diff_ready || !K || !d_lam || !d_Ya || !d_Yb || !d_out || !gA || !gB) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  RCHK(ensure_dvec(c, 4));
+  const int n = (int)c->n;
+  unsigned G = grid_for(n);
+  const int64_t *o = c->orig[c->cur];
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_lam, o, n, dv(c, 0));
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Ya, o, n, dv(c, 1));
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Yb, o, n, dv(c, 2));
+  double ga[6] = {0, 0, 0, 0, 0, 0}, gb[6] = {0, 0, 0, 0, 0, 0};
+  RCHK(diff_adjoint(c, K, dv(c, 0), dv(c, 1), dv(c, 2), dv(c, 3), 1.0, 0.0, ga));
+  RCHK(diff_adjoint(c, K, dv(c, 0), dv(c, 1), dv(c, 2), dv(c, 3), 0.0, 1.0, gb));
+  hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, dv(c, 3), o, n, d_out);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 6; k++) { gA[k] = ga[k]; gB[k] = gb[k]; }
+  return SPH_OK;
+}
+
+// Vectors: 0..s stage states, s+1 DY0, s+2 DYm, s+3 U, s+4 .. 2s+4 adjoints L[0..s];
+// segment recompute states after that: every x (s+1).
+int sph_diff_forward(sph_ctx *c, const double K[6], int s, double dt, int nsteps, int every, const double *d_T0,
+                     double *d_T) {
+  if (!c || !c->diff_ready || !K || !d_T0 || !d_T || s < 2 || nsteps < 1 || every < 1) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  const int n = (int)c->n;
+  unsigned G = grid_for(n);
+  int nck = nsteps / every + 1;
+  if ((int64_t)nck * c->npad > c->ckpt_cap) {
+    RCHK(dalloc(&c->ckpt, (size_t)nck * c->npad));
+    c->ckpt_cap = (int64_t)nck * c->npad;
+  }
+  RCHK(ensure_dvec(c, 2 * s + 5 + every * (s + 1)));
+  std::vector<double *> Y(s + 1);
+  for (int j = 0; j <= s; j++) Y[j] = dv(c, j);
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_T0, c->orig[c->cur], n, Y[0]);
+  for (int st = 0; st < nsteps; st++) {
+    if (st % every == 0)
+      HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(st / every) * c->npad, Y[0], sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, c->stream));
+    sts_forward_step(c, K, s, dt, Y.data(), dv(c, s + 1), dv(c, s + 2));
+    HIPCHK(hipMemcpyAsync(Y[0], Y[s], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, Y[0], c->orig[c->cur], n, d_T);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->fw_s = s; c->fw_steps = nsteps; c->fw_every = every; c->fw_dt = dt;
+  for (int k = 0; k < 6; k++) c->fw_K[k] = K[k];
+  return SPH_OK;
+}
+
+int sph_diff_backward(sph_ctx *c, const double *d_gT, double *d_gT0, double gK[6]) {
+  if (!c || !c->diff_ready || c->fw_steps < 1 || !d_gT || !d_gT0 || !gK) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  const int n = (int)c->n, s = c->fw_s, every = c->fw_every, nsteps = c->fw_steps;
+  const double dt = c->fw_dt;
+  const double *K = c->fw_K;
+  unsigned G = grid_for(n);
+  std::vector<double *> L(s + 1);
+  for (int j = 0; j <= s; j++) L[j] = dv(c, s + 4 + j);
+  double *U = dv(c, s + 3);
+  const int base = 2 * s + 5;
+  auto seg = [&](int k, int j) { return dv(c, base + k * (s + 1) + j); };
+  for (int k = 0; k < 6; k++) gK[k] = 0.0;
+  double *lam = dv(c, s + 1);  // running dL/dT (sorted)
+  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_gT, c->orig[c->cur], n, lam);
+  int nseg = (nsteps + every - 1) / every;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    int st0 = sg * every, st1 = std::min(nsteps, st0 + every);
+    // recompute the segment's stage states from its checkpoint
+    HIPCHK(hipMemcpyAsync(seg(0, 0), c->ckpt + (size_t)sg * c->npad, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                          c->stream));
+    for (int st = st0; st < st1; st++) {
+      int k = st - st0;
+      std::vector<double *> Y(s + 1);
+      for (int j = 0; j <= s; j++) Y[j] = seg(k, j);
+      sts_forward_step(c, K, s, dt, Y.data(), dv(c, s + 2), U);
+      if (st + 1 < st1)
+        HIPCHK(hipMemcpyAsync(seg(k + 1, 0), Y[s], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int st = st1 - 1; st >= st0; st--) {
+      int k = st - st0;
+      std::vector<double *> Y(s + 1);
+      for (int j = 0; j <= s; j++) Y[j] = seg(k, j);
+      HIPCHK(hipMemsetAsync(L[0], 0, sizeof(double) * (size_t)c->npad * s, c->stream));
+      HIPCHK(hipMemcpyAsync(L[s], lam, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+      RCHK(sts_backward_step(c, K, s, dt, Y.data(), L.data(), U, gK));
+      HIPCHK(hipMemcpyAsync(lam, L[0], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, lam, c->orig[c->cur], n, d_gT0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPH_OK;
 }
 
 }  // extern "C"

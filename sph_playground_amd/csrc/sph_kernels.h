@@ -99,4 +99,19 @@ struct Loop2Args {
   int n, npad;
 };
 
+// The differentiable diffusion operator (sph_adjoint.hip).
+struct DiffOpArgs {
+  const double4 *pm;   // x, y, z, m (sorted)
+  const double4 *q0;   // 1/h, den, om, - (sorted)
+  const uint8_t *real;
+  const uint32_t *nb;
+  const int *cnt;
+  const int *roff;     // reverse lists (CSR over sorted index)
+  const uint32_t *rent;
+  Box b;
+  double K[6];         // xx, yy, zz, xy, xz, yz
+  double wcv, fwc;
+  int ddw, n, npad;
+};
+
 }  // namespace sph

@@ -1,0 +1,155 @@
+"""The differentiable diffusion path: operator, transpose, tensor gradient and
+the checkpointed RKL2 adjoint behind torch.autograd.Function.
+
+Checks: (1) A(K) T equals loop 2's dT/dt (and the oracle's) for a uniform
+field; (2) <u, A v> = <A^T u, v>; (3) the tensor gradient equals <u, A(e_k) v>
+(A is linear in K); (4) the full gradient matches central finite differences
+(eps = 1e-4) and (5) an independent FP64 reference gradient: torch autograd on
+CPU through dense matrices A_k assembled column by column from the operator.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sph_playground_amd import consts as C
+from sph_playground_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n_side, b=(1.0, 1.0, 0.0), kperp=1e-3, ddw="fab"):
+    import torch  # noqa: F401
+    from sph_playground_amd import autograd as AG
+    from sph_playground_amd.engine import Engine
+    state, st = P.diffusion3d(n_side, ddw=ddw, b=b, h_scale=1.03)
+    eng = Engine(state, kernel="m6", tensor="full", kappa_perp=kperp)
+    eng.load(st)
+    AG.prepare(eng)
+    return state, st, eng, AG
+
+
+def _K_of_b(b, kpar, kperp):
+    b = np.asarray(b, dtype=np.float64)
+    b = b / np.linalg.norm(b)
+    K = kperp * np.eye(3) + (kpar - kperp) * np.outer(b, b)
+    return np.array([K[0, 0], K[1, 1], K[2, 2], K[0, 1], K[0, 2], K[1, 2]])
+
+
+def test_operator_equals_loop2_and_oracle(gpu):
+    import torch
+    state, st, eng, AG = _setup(12, b=(1.0, 0.4, 0.3), kperp=0.05)
+    after = eng.save()
+    T = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda()
+    K = _K_of_b((1.0, 0.4, 0.3), 1.0, 0.05)
+    got = AG.apply(eng, K, T).cpu().numpy()
+    ddt = after[:, C.es_ddt - 1]
+    assert np.abs(got - ddt).max() <= 1e-12 * np.abs(ddt).max()
+    ref = oracle.iterate(state, st, kernel=1, tensor_mode=1, kappa_perp=0.05)["store"][:, C.es_ddt - 1]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_transpose_and_tensor_gradient(gpu):
+    import torch
+    state, st, eng, AG = _setup(12)
+    n = st.shape[0]
+    rng = np.random.default_rng(3)
+    u = torch.from_numpy(rng.standard_normal(n)).cuda()
+    v = torch.from_numpy(rng.standard_normal(n)).cuda()
+    w = torch.from_numpy(rng.standard_normal(n)).cuda()
+    K = _K_of_b((1.0, 1.0, 0.0), 1.0, 1e-3)
+    Av = AG.apply(eng, K, v)
+    ATu, gA, gB = AG.apply_adjoint(eng, K, u, v, w)
+    lhs = float(torch.dot(u, Av))
+    rhs = float(torch.dot(ATu, v))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + float(torch.dot(u.abs(), Av.abs())))
+    for k in range(6):
+        ek = np.zeros(6)
+        ek[k] = 1.0
+        ref_a = float(torch.dot(u, AG.apply(eng, ek, v)))
+        ref_b = float(torch.dot(u, AG.apply(eng, ek, w)))
+        assert abs(gA[k] - ref_a) <= 1e-11 * (1 + abs(ref_a)) * 1e3
+        assert abs(gB[k] - ref_b) <= 1e-11 * (1 + abs(ref_b)) * 1e3
+
+
+def _loss_and_grads(AG, eng, T0, params, target, s, dt, nsteps, every, need_grad=True):
+    import torch
+    kpar, kperp, th, ph = [torch.tensor(p, dtype=torch.float64, requires_grad=need_grad) for p in params]
+    T0v = T0.clone().requires_grad_(need_grad)
+    K = AG.conduction_tensor(kpar, kperp, th, ph)
+    T = AG.sts(T0v, K, eng, s, dt, nsteps, every)
+    loss = 0.5 * torch.sum((T - target) ** 2)
+    if not need_grad:
+        return float(loss)
+    loss.backward()
+    return float(loss), [float(x.grad) for x in (kpar, kperp, th, ph)], T0v.grad.clone()
+
+
+def test_sts_gradient_matches_central_differences(gpu):
+    import torch
+    state, st, eng, AG = _setup(12)
+    n = st.shape[0]
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda()
+    target = torch.from_numpy(np.random.default_rng(5).standard_normal(n) * 0.1).cuda()
+    params = [1.0, 1e-3, math.pi / 2, math.pi / 4]
+    s, nsteps, every = 3, 4, 2
+    dt = 2e-5
+    _, g, gT0 = _loss_and_grads(AG, eng, T0, params, target, s, dt, nsteps, every)
+    eps = 1e-4
+    for k in range(4):
+        p1 = list(params)
+        p2 = list(params)
+        p1[k] += eps
+        p2[k] -= eps
+        fd = (_loss_and_grads(AG, eng, T0, p1, target, s, dt, nsteps, every, False)
+              - _loss_and_grads(AG, eng, T0, p2, target, s, dt, nsteps, every, False)) / (2 * eps)
+        assert abs(g[k] - fd) <= 1e-6 * max(1.0, abs(fd)), (k, g[k], fd)
+    d = torch.from_numpy(np.random.default_rng(6).standard_normal(n)).cuda()
+    fd = (_loss_and_grads(AG, eng, T0 + eps * d, params, target, s, dt, nsteps, every, False)
+          - _loss_and_grads(AG, eng, T0 - eps * d, params, target, s, dt, nsteps, every, False)) / (2 * eps)
+    assert abs(float(torch.dot(gT0, d)) - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_sts_gradient_matches_dense_fp64_reference(gpu):
+    import torch
+    from sph_playground_amd.engine import sts_coeffs
+    state, st, eng, AG = _setup(8)
+    n = st.shape[0]
+    # dense A_k, column by column from the operator (A is linear in K and T)
+    A = np.zeros((6, n, n))
+    for k in range(6):
+        ek = np.zeros(6)
+        ek[k] = 1.0
+        for c in range(n):
+            e = torch.zeros(n, dtype=torch.float64, device="cuda")
+            e[c] = 1.0
+            A[k][:, c] = AG.apply(eng, ek, e).cpu().numpy()
+    At = torch.from_numpy(A)
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy())
+    target = torch.from_numpy(np.random.default_rng(5).standard_normal(n) * 0.1)
+    params = [1.0, 1e-3, 1.1, 0.3]
+    s, nsteps, every, dt = 4, 5, 2, 2e-5
+    # reference: RKL2 recurrence (sts_integrator.f90:100-123) on CPU, torch autograd
+    kp = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in params]
+    T0r = T0.clone().requires_grad_(True)
+    K = AG.conduction_tensor(*kp)
+    Am = torch.einsum("k,kij->ij", K, At)
+    T = T0r
+    for _ in range(nsteps):
+        Y0 = T
+        DY0 = Am @ Y0
+        Ym2, Ym1 = Y0, Y0 + sts_coeffs(1, s)[2] * dt * DY0
+        for j in range(2, s + 1):
+            mu, nu, mh, gh = sts_coeffs(j, s)
+            Yj = mu * Ym1 + nu * Ym2 + (1 - mu - nu) * Y0 + mh * dt * (Am @ Ym1) + gh * dt * DY0
+            Ym2, Ym1 = Ym1, Yj
+        T = Ym1
+    loss_ref = 0.5 * torch.sum((T - target) ** 2)
+    loss_ref.backward()
+    loss, g, gT0 = _loss_and_grads(AG, eng, T0.cuda(), params, target.cuda(), s, dt, nsteps, every)
+    assert abs(loss - float(loss_ref)) <= 1e-12 * abs(float(loss_ref))
+    for k in range(4):
+        ref = float(kp[k].grad)
+        assert abs(g[k] - ref) <= 1e-9 * max(1.0, abs(ref)), (k, g[k], ref)
+    assert np.abs(gT0.cpu().numpy() - T0r.grad.numpy()).max() <= 1e-11 * np.abs(T0r.grad.numpy()).max()

@@ -136,7 +136,10 @@ def test_resident_steps_equal_host_iterate(gpu):
     eng2.step()
     eng2.step()
     b = eng2.save()
-    assert np.array_equal(a, b)
+    # same physics; the resident path keeps its own particle order between
+    # steps, so sums may run in another order (roundoff only)
+    errs = G.field_errors(a, b, a.shape[0])
+    assert max(errs.values()) <= FP64_TOL, errs
 
 
 def test_device_store_path(gpu):
