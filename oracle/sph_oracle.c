@@ -778,7 +778,7 @@ int orc_iterate(const double *state, int kernel, const orc_ext *ext, int64_t nto
   double emax[3] = {state[EC_XMIN + 1], state[EC_XMIN + 3], state[EC_XMIN + 5]};
   int process = (int)state[EC_PROCESS];
   int ics = (int)state[EC_ICS];
-  int wrap_axes = 0, ghost_axes = 0, do_eos = 1, iso = 0;
+  int wrap_axes = 0, ghost_axes = 0, do_eos = 1, iso = 0, gravity = 0;
   if (process == 601) { wrap_axes = ghost_axes = 7; }
   else if (process == 602) { }
   else if (process == 600) {
@@ -787,7 +787,8 @@ int orc_iterate(const double *state, int kernel, const orc_ext *ext, int64_t nto
       case 103: case 104: case 111: do_eos = 0; break;           /* pulse, ring, fld_gauss */
       case 105: wrap_axes = ghost_axes = 7; iso = 1; break;      /* soundwave */
       case 106: wrap_axes = ghost_axes = 6; break;               /* hydroshock */
-      case 101: case 110: wrap_axes = ghost_axes = 1; break;     /* mti, mtilowres */
+      case 101: wrap_axes = ghost_axes = 1; gravity = 1; break;  /* mti */
+      case 110: wrap_axes = ghost_axes = 1; gravity = 2; break;  /* mtilowres */
       case 108: wrap_axes = ghost_axes = 7; break;               /* OTvortex */
       default: return ORC_BAD_ARG;
     }
@@ -831,6 +832,11 @@ int orc_iterate(const double *state, int kernel, const orc_ext *ext, int64_t nto
   }
   if (rc == ORC_OK && do_eos) eos(store, nreal, F.gamma, iso);
   if (rc == ORC_OK) rc = loop2(&k, &F, store, nreal, ntot, &g, off, lst, &res->energy);
+  /* gravity of the mti set-ups, iterator.f90:143,153-154 */
+  if (rc == ORC_OK && gravity == 1)
+    for (int64_t i = 0; i < nreal; i++) ST(i, AY) = ST(i, AY) - 1.;
+  if (rc == ORC_OK && gravity == 2)
+    for (int64_t i = 0; i < nreal; i++) ST(i, AY) = ST(i, AY) - erf(ST(i, RY) * 8.);
   free(lst);
   free(g.pos);
   free(g.cref);

@@ -51,6 +51,7 @@ __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
 __global__ void k_predict(Fields, const uint8_t *, int, double, int, double *, int);
 __global__ void k_correct(Fields, const uint8_t *, int, double, int, const double *, int);
 __global__ void k_dt_reduce(Fields, const uint8_t *, int, long long *);
+__global__ void k_gravity(Fields, const uint8_t *, int, int);
 __global__ void k_sts_stage(const double *, const double *, const double *, const double *, const double *, int,
                             double, double, double, double, double *);
 __global__ void k_axpy_copy(double *, const double *, const double *, double, int);
@@ -101,7 +102,7 @@ struct sph_ctx {
   sph_options opt{};
   Phys frozen{};
   Box box{};
-  int process = 0, ics = 0, do_eos = 1, eos_iso = 0;
+  int process = 0, ics = 0, do_eos = 1, eos_iso = 0, gravity = 0;
   int maxneib = 150;
   // particles
   int64_t n = 0, n_real = 0, n_fixed = 0, cap = 0, npad = 0;
@@ -656,6 +657,9 @@ static int run_step(sph_ctx *c, const double *state) {
   RCHK(run_loop1(c, P));
   HIPCHK(hipEventRecord(c->ev[2], c->stream));
   int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
+  if (rc == SPH_OK && c->gravity)
+    hipLaunchKernelGGL(k_gravity, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
+                       c->gravity);
   HIPCHK(hipEventRecord(c->ev[3], c->stream));
   HIPCHK(hipEventSynchronize(c->ev[3]));
   float t01 = 0, t12 = 0, t23 = 0;
@@ -780,7 +784,9 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
       case 105: wrap = ghost = 7; c->eos_iso = 1; break;
       case 106: wrap = ghost = 6; break;
       case 108: wrap = ghost = 7; break;
-      default: delete c; return SPH_ERR_UNSUPPORTED;  // mti gravity terms etc.
+      case 101: wrap = ghost = 1; c->gravity = 1; break;  // mti
+      case 110: wrap = ghost = 1; c->gravity = 2; break;  // mtilowres
+      default: delete c; return SPH_ERR_UNSUPPORTED;
     }
   } else {
     delete c;
@@ -923,29 +929,30 @@ int sph_neighbours(sph_ctx *c, int64_t *off, int32_t *j, int8_t *code, int64_t c
   return rc;
 }
 
-// timestep.calc, timestep.f90:40-166 (no FLD branch)
+// timestep.calc, timestep.f90:40-166 (hydro, MHD, diffusion and FLD limits)
 static int calc_dt(sph_ctx *c, const Phys &P, double *dt) {
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
-  const bool is_min[6] = {true, false, true, true, false, false};
+  const bool is_min[8] = {true, false, true, true, false, false, true, false};
   long long *init = c->h_ll;
-  for (int v = 0; v < 6; v++)
+  for (int v = 0; v < 8; v++)
     for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = is_min[v] ? BIG : SMALL;
-  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 6 * kStripes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 8 * kStripes, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_dt_reduce, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
                      c->d_red);
   long long *r = c->h_ll + 512;
-  HIPCHK(hipMemcpyAsync(r, c->d_red, sizeof(long long) * 6 * kStripes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(r, c->d_red, sizeof(long long) * 8 * kStripes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  double v[6];
-  for (int k = 0; k < 6; k++) v[k] = ord2d(is_min[k] ? stripe_min(r, k) : stripe_max(r, k));
+  double v[8];
+  for (int k = 0; k < 8; k++) v[k] = ord2d(is_min[k] ? stripe_min(r, k) : stripe_max(r, k));
   double minh = v[0], maxc = v[1], minden = v[2], minc = v[3], maxkappa = v[4], maxb = v[5];
+  double minkappa = v[6], maxlambda = v[7];
   double cnhydro, cnd2;
   cn_of(c->kt, P.ddw, &cnhydro, &cnd2);
   double d = HUGE_VAL;
   if (P.hydro) d = std::min(d, cnhydro * minh / maxc);
   if (P.magneto) d = std::min(d, .1 * minh / maxc / maxb);
   if (P.diff) d = std::min(d, cnd2 * minden * minc * minh * minh / maxkappa);
-  if (P.fld) return SPH_ERR_UNSUPPORTED;
+  if (P.fld) d = std::min(d, cnd2 * minh * minh * minden * minkappa / kLightspeed / maxlambda);
   *dt = d;
   return SPH_OK;
 }

@@ -943,9 +943,13 @@ __global__ void k_find_orig(const int64_t *orig, int n, int64_t target, int *pos
 }
 
 // ----------------------------------------------------- step-driver updates
+// These elementwise kernels are HBM-bound; they turn FP contraction off so
+// every product and sum rounds as the reference's unfused Fortran does.
+//
 // Leapfrog predictor (main.f90:323-342) on real particles; saves the old
 // derivatives in `sv` (ax, ay, az, du, dh, ddt, dbx, dby, dbz) for the corrector.
 __global__ void k_predict(Fields F, const uint8_t *real, int n, double dt, int do_t, double *sv, int npad) {
+#pragma clang fp contract(off)
   int s = gtid();
   if (s >= n || !real[s]) return;
   const int src[9] = {F_AX, F_AY, F_AZ, F_DU, F_DH, F_DDT, F_DBX, F_DBY, F_DBZ};
@@ -964,6 +968,7 @@ __global__ void k_predict(Fields F, const uint8_t *real, int n, double dt, int d
 
 // Leapfrog corrector (main.f90:348-362).
 __global__ void k_correct(Fields F, const uint8_t *real, int n, double dt, int do_t, const double *sv, int npad) {
+#pragma clang fp contract(off)
   int s = gtid();
   if (s >= n || !real[s]) return;
   double o[9];
@@ -977,25 +982,32 @@ __global__ void k_correct(Fields F, const uint8_t *real, int n, double dt, int d
   if (do_t) F.f[F_T][s] = F.f[F_T][s] + 0.5 * dt * (F.f[F_DDT][s] - o[5]);
 }
 
-// Per-real-particle minima/maxima for timestep.calc (timestep.f90:62-81):
+// Per-real-particle minima/maxima for timestep.calc (timestep.f90:62-133):
 // red[v * kStripes + stripe] (ordered int64): 0 min h, 1 max c, 2 min den,
-// 3 min c, 4 max kappa, 5 max of the B components
+// 3 min c, 4 max kappa, 5 max of the B components, 6 min kappa,
+// 7 max flux limiter lambda(R), R = |grad T| / (kappa rho^2 T)
 __global__ __launch_bounds__(256) void k_dt_reduce(Fields F, const uint8_t *real, int n, long long *red) {
+#pragma clang fp contract(off)
   __shared__ long long lds[4];
   int s = gtid();
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
-  long long v[6] = {BIG, SMALL, BIG, BIG, SMALL, SMALL};
+  long long v[8] = {BIG, SMALL, BIG, BIG, SMALL, SMALL, BIG, SMALL};
   if (s < n && real[s]) {
     v[0] = d2ord(F.f[F_H][s]);
     v[1] = d2ord(F.f[F_C][s]);
     v[2] = d2ord(F.f[F_DEN][s]);
     v[3] = v[1];
-    v[4] = d2ord(F.f[F_KAPPA][s]);
+    double ka = F.f[F_KAPPA][s];
+    v[4] = d2ord(ka);
     double bm = fmax(fmax(F.f[F_BX][s], F.f[F_BY][s]), F.f[F_BZ][s]);
     v[5] = d2ord(bm);
+    v[6] = v[4];
+    double gx = F.f[F_DTDX][s], gy = F.f[F_DTDY][s], gz = F.f[F_DTDZ][s], da = F.f[F_DEN][s];
+    double fr = sqrt(gx * gx + gy * gy + gz * gz) / (ka * da * da * F.f[F_T][s]);
+    v[7] = d2ord((2. + fr) / (6. + 3 * fr + fr * fr));
   }
-  const bool is_min[6] = {true, false, true, true, false, false};
-  for (int k = 0; k < 6; k++) {
+  const bool is_min[8] = {true, false, true, true, false, false, true, false};
+  for (int k = 0; k < 8; k++) {
     long long t = is_min[k] ? block_reduce<R_MIN>(v[k], lds) : block_reduce<R_MAX>(v[k], lds);
     if (threadIdx.x == 0) {
       if (is_min[k]) atomicMin(&red[k * kStripes + stripe()], t);
@@ -1004,17 +1016,28 @@ __global__ __launch_bounds__(256) void k_dt_reduce(Fields F, const uint8_t *real
   }
 }
 
+// Legacy gravity of the mti set-ups after loop 2 (iterator.f90:143,153-154):
+// mode 1 a_y -= 1, mode 2 a_y -= erf(8 y), on real particles.
+__global__ void k_gravity(Fields F, const uint8_t *real, int n, int mode) {
+  int s = gtid();
+  if (s >= n || !real[s]) return;
+  if (mode == 1) F.f[F_AY][s] = F.f[F_AY][s] - 1.;
+  else F.f[F_AY][s] = F.f[F_AY][s] - erf(F.f[F_RY][s] * 8.);
+}
+
 // RKL2 stage combination on T (sts_integrator.f90:111-115) over every
 // particle slot, as the reference updates whole store rows.
 __global__ void k_sts_stage(const double *y0, const double *yjm1, const double *yjm2, const double *dy0,
                             const double *dyjm1, int n, double mu, double nu, double muh_dt, double gmh_dt,
                             double *yj) {
+#pragma clang fp contract(off)
   int s = gtid();
   if (s >= n) return;
   yj[s] = mu * yjm1[s] + nu * yjm2[s] + (1. - mu - nu) * y0[s] + muh_dt * dyjm1[s] + gmh_dt * dy0[s];
 }
 
 __global__ void k_axpy_copy(double *dst, const double *x, const double *y, double a, int n) {
+#pragma clang fp contract(off)
   int s = gtid();
   if (s >= n) return;
   dst[s] = x[s] + a * y[s];

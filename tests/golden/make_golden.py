@@ -68,13 +68,22 @@ def cases():
     c.append(("fld3d_fab_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, fld=True), {}))
     c.append(("fld3d_2nwds_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, fld=True), {C.ec_ddw: C.esd_2nw_ds}))
     c.append(("hyrad3d_2000", 1, 0, 0, 0.0, lambda: _slab(seed=5, equations="hyrad"), {}))
+    # legacy mti set-up: periodic in x only, gravity a_y -= 1 after loop 2 (iterator.f90:134-143)
+    c.append(("hydro2d_mti_16", 1, 0, 0, 0.0, lambda: P.hydro2d(16, au=1.0),
+              {C.ec_process: C.epc_backcompatibility, C.ec_ics: C.ett_mti}))
+    # one leapfrog step with the flux-limited-diffusion time step (timestep.f90:83-136)
+    c.append(("fld3d_fab_3000_leapfrog", 1, 1, 0, 1.0,
+              lambda: P.random_slab3d(3000, seed=7, hfac=0.85, velocity_seed=9, fld=True), {}))
     return c
 
 
 def main():
     if not refrun.available(1):
         sys.exit("reference harness not built: make -C oracle ref")
+    only_missing = "--only-missing" in sys.argv
     for name, kernel, mode, sts_s, dtcap, build, over in cases():
+        if only_missing and os.path.exists(os.path.join(OUT, name + ".npz")):
+            continue
         state, st = build()
         for k, v in over.items():
             state[k - 1] = v

@@ -39,13 +39,17 @@ def test_iterate_matches_reference_golden(gpu, name):
     eng.close()
 
 
-def test_leapfrog_matches_reference_golden(gpu):
-    g = G.load("hydro2d_m6_32_leapfrog")
+@pytest.mark.parametrize("name", G.names(mode=1))
+def test_leapfrog_matches_reference_golden(gpu, name):
+    g = G.load(name)
+    print("case", name, flush=True)
     nr = int(g["state"][C.ec_realpn - 1])
     eng = _engine(g["state"], 1)
     eng.load(g["store_in"])
     dt = eng.leapfrog(g["meta"]["dtcap"])
-    assert abs(dt - g["meta"]["dt_used"]) <= 1e-15 * abs(g["meta"]["dt_used"])
+    # dt is a min/max over loop outputs (h, c, rho, kappa, grad T) that agree
+    # with the reference to roundoff (neighbour sums run in another order)
+    assert abs(dt - g["meta"]["dt_used"]) <= 1e-14 * abs(g["meta"]["dt_used"])
     out = eng.save()
     errs = G.field_errors(g["store_out"], out, nr)
     assert max(errs.values()) <= FP64_TOL, errs

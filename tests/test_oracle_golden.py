@@ -34,7 +34,8 @@ def test_golden_set_is_complete():
     for tag in ("fab", "n2w", "fw", "2nwds", "2nwsd"):
         assert any(tag in n for n in names), tag
     for n in ("hydro2d_m4_16", "hydro2d_gauss_16", "hydro1d_m6_64", "mhd3d_hydroshock_2000",
-              "fld3d_fab_2000", "hyrad3d_2000", "diff2d_fab_sts3_16", "hydro2d_m6_32_leapfrog"):
+              "fld3d_fab_2000", "hyrad3d_2000", "diff2d_fab_sts3_16", "hydro2d_m6_32_leapfrog",
+              "hydro2d_mti_16", "fld3d_fab_3000_leapfrog"):
         assert n in names
 
 
