@@ -130,8 +130,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the step runs on one GPU; runs on more than one GPU are not supported "
+              "(pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

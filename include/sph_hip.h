@@ -37,6 +37,7 @@ extern "C" {
 #define SPH_ERR_HIP 8             /* HIP runtime failure */
 #define SPH_ERR_NOMEM 9           /* device allocation failed */
 #define SPH_ERR_UNSUPPORTED 10    /* outside the implemented scope (e.g. cylindrical, kernel.f90:137) */
+#define SPH_ERR_RAD_NEGATIVE 11   /* rad_exchange.f90:60-63 (energy <= 0 after the exchange) */
 
 typedef struct sph_ctx sph_ctx;
 
@@ -109,6 +110,12 @@ int sph_leapfrog(sph_ctx *ctx, const double state[49], double dtcap, double *dt_
 /* One RKL2 super-time-step of T (sts_integrator.f90:87-124) with s stages on
  * the resident state; s iterates. */
 int sph_sts(sph_ctx *ctx, const double state[49], int s, double dt);
+
+/* Implicit gas/radiation energy exchange over dt on the resident state, every
+ * loaded particle (real and fixed): update_radenergy(n, store, dt),
+ * rad_exchange.f90:9-118, as main.f90:367-369 calls it after the corrector
+ * (and after sts_integrate when STS is on). */
+int sph_rad_exchange(sph_ctx *ctx, const double state[49], double dt);
 
 /* FP64 RKL2 coefficients (sts_integrator.f90:10-51) for stage j of s:
  * out[0..3] = mu(j), nu(j), mu_hat(j,s), gm_hat(j,s); j >= 1. */

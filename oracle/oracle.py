@@ -53,6 +53,8 @@ def lib():
         for name in ("orc_sts_mu_hat", "orc_sts_gm_hat"):
             getattr(L, name).argtypes = [ct.c_int, ct.c_int]
             getattr(L, name).restype = ct.c_double
+        L.orc_rad_exchange.argtypes = [ct.POINTER(ct.c_double), ct.c_int64, ct.POINTER(ct.c_double), ct.c_double]
+        L.orc_rad_exchange.restype = ct.c_int
         _lib = L
     return _lib
 
@@ -105,3 +107,14 @@ def knorm(kernel, dim):
 
 def sts(name, *args):
     return getattr(lib(), "orc_sts_" + name)(*args)
+
+
+def rad_exchange(state, store, dt, check=True):
+    """update_radenergy on a copy of ``store`` (all rows).  Returns (store, rc)."""
+    st = np.ascontiguousarray(np.array(store, dtype=np.float64, copy=True))
+    s = np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+    rc = lib().orc_rad_exchange(s.ctypes.data_as(ct.POINTER(ct.c_double)), st.shape[0],
+                                st.ctypes.data_as(ct.POINTER(ct.c_double)), dt)
+    if check and rc != 0:
+        raise OracleError(rc)
+    return st, rc

@@ -25,7 +25,7 @@ FFLAGS  := -fdefault-real-8 -fdefault-double-8 -O2 -cpp -fopenmp -ffp-contract=o
 
 MODS := utils/errprinter utils/arrayresize utils/timing utils/list utils/kdtree2 \
         const kernel/base_$(KERNEL) state kernel/kernel BC utils/neighboursearch \
-        eos circuit1 circuit2 iterator timestep sts_integrator
+        eos circuit1 circuit2 iterator timestep sts_integrator rad_exchange
 
 OBJS := $(addprefix $(OUT)/,$(addsuffix .o,$(notdir $(MODS))))
 
@@ -35,7 +35,7 @@ $(OUT):
 	mkdir -p $(OUT)
 
 # Modules must compile in dependency order (each writes its .mod into $(OUT)).
-$(OUT)/stamp: | $(OUT)
+$(OUT)/stamp: oracle/ref.mk | $(OUT)
 	set -e; for m in $(MODS); do \
 	  $(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) -c $(REFSRC)/$$m.f90 -o $(OUT)/$$(basename $$m).o; \
 	done

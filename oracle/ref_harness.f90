@@ -10,6 +10,8 @@
 !   mode 2: sts_integrate(n, s, store, dt)                 sts_integrator.f90:87
 !   mode 3: iterate twice, timing only the second call (steady state: the
 !           first call also runs initIterate's extra neighbour search)
+!   mode 4: mode 1, then update_radenergy(n, store, dt) as main.f90:367-369
+!   mode 5: update_radenergy(n, store, dtcap) alone        rad_exchange.f90:9
 !   mode + 100: same, without writing the neighbour lists
 !
 ! Input  (stream, little endian): int32 mode, n_real, n_fixed, sts_s;
@@ -26,6 +28,7 @@ program ref_harness
   use iterator,       only: iterate
   use timestep,       only: calc
   use sts_integrator, only: sts_integrate, sts_init => init
+  use rad_exchange,   only: update_radenergy
   use neighboursearch, only: xgetneighnumber, xgetneighindex
   use omp_lib
   implicit none
@@ -67,7 +70,7 @@ program ref_harness
   select case (mode)
   case (0, 3)
     call iterate(ntot, store, dedt)
-  case (1)
+  case (1, 4)
     ! main.f90:219 initial accelerations, then one pass of the loop body
     call iterate(ntot, store, dedt)
     call calc(store, dthydro, dtdiff, dtfld, dt)
@@ -92,6 +95,10 @@ program ref_harness
     store(es_h,1:ntot) = store(es_h,1:ntot) + 0.5*dt*(store(es_dh,1:ntot) - tdh(:))
     store(es_bx:es_bz,1:ntot) = store(es_bx:es_bz,1:ntot) + 0.5*dt*(store(es_dbx:es_dbz,1:ntot) - tdb(:,:))
     store(es_t,1:ntot) = store(es_t,1:ntot) + 0.5*dt*(store(es_ddt,1:ntot) - tddt(:))
+    if (mode == 4) call update_radenergy(ntot, store, dt)
+  case (5)
+    dt = dtcap
+    call update_radenergy(ntot, store, dt)
   case (2)
     dt = dtcap
     call sts_init(size(store, 2))
@@ -106,12 +113,14 @@ program ref_harness
   write(11) ntot
   write(11) store(:, 1:ntot)
   write(11) dt
-  if (dumpnb == 0) then
-    ! only the pair count P = sum_i |N_i|
+  if (dumpnb == 0 .or. mode == 5) then
+    ! only the pair count P = sum_i |N_i| (none: mode 5 searches nothing)
     npairs = 0
-    do i = 1, nreal
-      npairs = npairs + xgetneighnumber(i)
-    end do
+    if (mode /= 5) then
+      do i = 1, nreal
+        npairs = npairs + xgetneighnumber(i)
+      end do
+    end if
     write(11) -1
     write(11) npairs
     close(11)

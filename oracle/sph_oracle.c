@@ -23,7 +23,8 @@ enum { EC_DIM = 0, EC_EQS = 1, EC_DDW = 2, EC_ICS = 3, EC_ADDEN = 4, EC_ARTTS = 
        EC_DISO = 7, EC_DCOND = 8, EC_MUZERO = 9, EC_HFAC = 12, EC_XMIN = 16,
        EC_BORDSIZE = 22, EC_PADDING = 23, EC_REALPN = 24, EC_FIXEDPN = 25,
        EC_GAMMA = 26, EC_PROCESS = 30, EC_AU = 31, EC_EQONHYDRO = 33,
-       EC_EQONDIFF = 34, EC_EQONFLD = 35, EC_EQONRADEXCH = 36, EC_EQONSTS = 37 };
+       EC_EQONDIFF = 34, EC_EQONFLD = 35, EC_EQONRADEXCH = 36, EC_EQONSTS = 37,
+       EC_MOLMASS = 46 };
 
 #define ESD_FAB 400
 #define ESD_N2W 401
@@ -33,6 +34,8 @@ enum { EC_DIM = 0, EC_EQS = 1, EC_DDW = 2, EC_ICS = 3, EC_ADDEN = 4, EC_ARTTS = 
 
 static const double PI_ = 3.141592653589793;     /* 4*atan(1), const.f90:4 */
 static const double LIGHTSPEED = 2.997924e10;    /* const.f90:6 */
+static const double RG = 8.314e7;                /* const.f90:7 */
+static const double SIGMAB = 5.67051e-5;         /* const.f90:8 */
 
 #define ST(i, f) store[(int64_t)(i) * 32 + (f)]
 
@@ -852,3 +855,36 @@ double orc_sts_nu(int j) { return -(j - 1.) / j * sts_b(j) / sts_b(j - 2); }
 static double sts_w(int s) { return 4. / (s * s + s - 2.); }
 double orc_sts_mu_hat(int j, int s) { return j == 1 ? sts_b(1) * sts_w(s) : orc_sts_mu(j) * sts_w(s); }
 double orc_sts_gm_hat(int j, int s) { return -sts_a(j - 1) * orc_sts_mu_hat(j, s); }
+
+/* --------------------------------------------------------- rad exchange */
+/* update_radenergy(n, store, dt), rad_exchange.f90:9-66, with
+ * solve_internal_energy_implicit (:100-118): per particle (real and fixed,
+ * i = 1..n), Newton on
+ *   f(u) = u/dt - u0/dt - du - ack (rho (etot - u)/a - (u cv1)^4)
+ * from u = u0 while |u - u_old| > 1e-16 and at most 10 iterations; then
+ * u <- u_new, t <- etot - u_new.  Returns ORC_RAD_NEGATIVE where the
+ * reference stops with "radiation/thermal energy is negative after exchange". */
+int orc_rad_exchange(const double *state, int64_t n, double *store, double dt) {
+  const double gamma = state[EC_GAMMA], gmw = state[EC_MOLMASS];
+  const double a = 4. * SIGMAB / LIGHTSPEED;
+  const double cv1 = (gamma - 1.) * gmw / RG;
+  int bad = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const double kappa = ST(i, KAPPA), ack = 4. * SIGMAB * kappa;
+    const double rho = ST(i, DEN), u0 = ST(i, U), dudt = ST(i, DU), xi = ST(i, T);
+    const double etot = u0 + xi;
+    double unew = u0, uold = 2 * u0;
+    int iter = 0;
+    while (fabs(unew - uold) > 1e-16 && iter < 10) {
+      uold = unew;
+      iter++;
+      double fu = unew / dt - u0 / dt - dudt - ack * (rho * (etot - unew) / a - ipow(unew * cv1, 4));
+      double dfu = 1. / dt + ack * (rho / a + 4. * (ipow(unew, 3) * ipow(cv1, 4)));
+      unew = unew - fu / dfu;
+    }
+    ST(i, U) = unew;
+    ST(i, T) = etot - unew;
+    if (ST(i, T) <= 0 || ST(i, U) <= 0) bad = 1;
+  }
+  return bad ? ORC_RAD_NEGATIVE : ORC_OK;
+}

@@ -57,6 +57,7 @@ typedef struct {
 #define ORC_FLD_ANISO 4
 #define ORC_BAD_DDW 5
 #define ORC_BAD_ARG 7
+#define ORC_RAD_NEGATIVE 11
 
 /* iterate(n, store, dedt): iterator.f90:44-179.  store is (n_total x 32)
  * row-major (row i == Fortran column store(:, i+1)).  kernel 0=M4 1=M6 2=gauss. */
@@ -75,6 +76,9 @@ double orc_sts_mu(int j);
 double orc_sts_nu(int j);
 double orc_sts_mu_hat(int j, int s);
 double orc_sts_gm_hat(int j, int s);
+
+/* update_radenergy(n, store, dt), rad_exchange.f90:9-118, on all n rows. */
+int orc_rad_exchange(const double *state, int64_t n, double *store, double dt);
 
 #ifdef __cplusplus
 }

@@ -13,13 +13,13 @@ EXPORTS = (
     "sph_default_options", "sph_ctx_create", "sph_ctx_destroy", "sph_iterate", "sph_iterate_device",
     "sph_load", "sph_save", "sph_step", "sph_neighbours", "sph_leapfrog", "sph_sts", "sph_sts_coeffs",
     "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
-    "sph_diff_forward", "sph_diff_backward",
+    "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange",
 )
 
 SPH_OK = 0
 ERRORS = {
     1: "NEIGH_TRUNCATED", 2: "NEG_H", 3: "Q_NAN_OR_NEG", 4: "FLD_ANISO", 5: "BAD_DDW",
-    6: "PRECONDITION", 7: "BAD_ARG", 8: "HIP", 9: "NOMEM", 10: "UNSUPPORTED",
+    6: "PRECONDITION", 7: "BAD_ARG", 8: "HIP", 9: "NOMEM", 10: "UNSUPPORTED", 11: "RAD_NEGATIVE",
 }
 
 
@@ -72,6 +72,7 @@ def lib():
                                  ct.c_int64, ct.POINTER(ct.c_int64)]
     L.sph_leapfrog.argtypes = [P, D, ct.c_double, D]
     L.sph_sts.argtypes = [P, D, ct.c_int, ct.c_double]
+    L.sph_rad_exchange.argtypes = [P, D, ct.c_double]
     L.sph_sts_coeffs.argtypes = [ct.c_int, ct.c_int, D]
     L.sph_get_stats.argtypes = [P, ct.POINTER(Stats)]
     L.sph_diff_prepare.argtypes = [P, D]

@@ -28,11 +28,12 @@ __global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, l
 __global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
 __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
-__global__ void k_cell_range(const uint32_t *, int, Grid, int *, int *);
-__global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, int, double4 *, double *, uint16_t *,
+__global__ void k_cell_range(const uint32_t *, int, Grid, int2 *);
+__global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
                        uint8_t *, double4 *, double *);
 template <int DIM, int RS>
-__global__ void k_neigh(NeighArgs);
+__global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
+                        int *__restrict__);
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1(Loop1Args<double4>);
 template <int DIM, int KT, int GRAD, typename PT>
@@ -48,8 +49,9 @@ __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, c
                                const int64_t *, int32_t *, int8_t *);
 __global__ void k_counts_by_orig(const int *, const int64_t *, const uint8_t *, int, int64_t *);
 __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
-__global__ void k_predict(Fields, const uint8_t *, int, double, int, double *, int);
-__global__ void k_correct(Fields, const uint8_t *, int, double, int, const double *, int);
+__global__ void k_predict(Fields, int, double, int, double *, int);
+__global__ void k_correct(Fields, int, double, int, const double *, int);
+__global__ void k_rad_exchange(Fields, int, double, double, double, double, int *);
 __global__ void k_dt_reduce(Fields, const uint8_t *, int, long long *);
 __global__ void k_gravity(Fields, const uint8_t *, int, int);
 __global__ void k_sts_stage(const double *, const double *, const double *, const double *, const double *, int,
@@ -77,7 +79,7 @@ constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fie
 enum { S_DIM = 0, S_EQS = 1, S_DDW = 2, S_ICS = 3, S_ADDEN = 4, S_ARTTS = 5, S_COORD = 6, S_DISO = 7,
        S_DCOND = 8, S_MUZERO = 9, S_HFAC = 12, S_XMIN = 16, S_BORD = 22, S_PAD = 23, S_REALPN = 24,
        S_FIXEDPN = 25, S_GAMMA = 26, S_PROCESS = 30, S_AU = 31, S_EQONHYDRO = 33, S_EQONDIFF = 34,
-       S_EQONFLD = 35 };
+       S_EQONFLD = 35, S_MOLMASS = 46 };
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
@@ -113,11 +115,11 @@ struct sph_ctx {
   uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  int *cstart = nullptr, *cend = nullptr;
+  int2 *crange = nullptr;
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
   double *h0 = nullptr, *eps = nullptr;
-  uint16_t *band = nullptr;
+  float4 *pf = nullptr;   // FP32 search records (relative coordinates + band bits)
   uint8_t *real = nullptr;
   uint32_t *nb = nullptr;
   int nb_cap = 0;
@@ -249,7 +251,8 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
   RCHK(dalloc(&c->pm, npad));
   RCHK(dalloc(&c->h0, npad));
   RCHK(dalloc(&c->eps, npad));
-  RCHK(dalloc(&c->band, npad));
+  RCHK(dalloc(&c->pf, npad + 4));  // + 4 records read past the last cell by k_neigh
+  HIPCHK(hipMemset(c->pf, 0, sizeof(float4) * (npad + 4)));
   RCHK(dalloc(&c->real, npad));
   RCHK(dalloc(&c->cnt, npad));
   size_t v4 = c->opt.precision == 1 ? sizeof(float4) : sizeof(double4);
@@ -280,8 +283,7 @@ static int ensure_nb(sph_ctx *c, int cap) {
 static int ensure_cells(sph_ctx *c, int64_t ncell) {
   if (ncell <= c->ncell_cap) return SPH_OK;
   int64_t cap = std::max<int64_t>(ncell, 1024);
-  RCHK(dalloc(&c->cstart, cap));
-  RCHK(dalloc(&c->cend, cap));
+  RCHK(dalloc(&c->crange, cap));
   c->ncell_cap = cap;
   return SPH_OK;
 }
@@ -324,6 +326,12 @@ static long long stripe_min(const long long *h, int v) {
   return t;
 }
 
+// Stencil half-width of the neighbour search in cells (cells rmax/RS wide).
+#ifndef SPH_NEIGH_RS
+#define SPH_NEIGH_RS 1
+#endif
+static const int kNeighRS = SPH_NEIGH_RS;
+
 static int build_neighbours(sph_ctx *c) {
   const int n = (int)c->n;
   const int dim = c->frozen.dim;
@@ -351,11 +359,13 @@ static int build_neighbours(sph_ctx *c) {
   // 2. cell grid: cells at least rmax/RS wide, searched over +-RS cells;
   //    imaged axes tile [emin, emax] and need 2 RS + 1 cells for every
   //    (cell, image) pair to be distinct, else RS drops to 1.  Per-axis cell
-  //    counts stay within the 30-bit Morton key.
+  //    counts stay within the 30-bit Morton key.  RS = kNeighRS by default:
+  //    one-radius cells measured faster than half-radius cells on MI355X (the
+  //    search is bound by cell-range loads and gathers, not by tests).
   const int axis_cap = dim == 3 ? 1024 : (dim == 2 ? 32768 : (1 << 30));
   Grid g{};
   double cs[3];
-  int RS = 2;
+  int RS = kNeighRS;
   for (int attempt = 0; attempt < 2; attempt++) {
     g = Grid{};
     g.mdim = dim;
@@ -397,7 +407,10 @@ static int build_neighbours(sph_ctx *c) {
       g.nc[a] = (int)std::floor((hi[a] - lo[a]) / cs[a]) + 1;
     }
   }
-  for (int a = 0; a < 3; a++) g.inv_cs[a] = cs[a] > 0 ? 1.0 / cs[a] : 0.0;
+  for (int a = 0; a < 3; a++) {
+    g.cs[a] = cs[a];
+    g.inv_cs[a] = cs[a] > 0 ? 1.0 / cs[a] : 0.0;
+  }
   c->grid = g;
   int64_t ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
   RCHK(ensure_cells(c, ncell));
@@ -436,30 +449,40 @@ static int build_neighbours(sph_ctx *c) {
     F = c->fields();
     c->stats.sorted = 1;
   }
-  HIPCHK(hipMemsetAsync(c->cstart, 0, sizeof(int) * ncell, c->stream));
-  HIPCHK(hipMemsetAsync(c->cend, 0, sizeof(int) * ncell, c->stream));
-  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->cstart, c->cend);
-  hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, dim,
-                     c->pm, c->h0, c->band, c->real, c->st[0], c->eps);
+  HIPCHK(hipMemsetAsync(c->crange, 0, sizeof(int2) * ncell, c->stream));
+  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->crange);
+  hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->eps);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
     HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 2, c->stream));
     HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     NeighArgs a;
-    a.pm = c->pm; a.hs = c->h0; a.band = c->band; a.real = c->real; a.cstart = c->cstart; a.cend = c->cend;
+    a.hs = c->h0; a.real = c->real; a.crange = c->crange;
     a.g = g; a.b = c->box; a.krad = c->frozen.krad; a.n = n; a.npad = (int)c->npad; a.cap = c->nb_cap;
-    a.maxneib = c->maxneib; a.nb = c->nb; a.cnt = c->cnt; a.err = c->d_int; a.stripes = c->d_stripes;
+    // FP32 search records: coordinates relative to the grid origin lie within
+    // the grid span S (imaged axes: the box), images within 2 S; each of the
+    // roundings involved is at most 2^-24 S, six of them at most, so
+    // 2^-21 S bounds the error of an FP32 separation component.
+    double span = 0.0;
+    for (int d = 0; d < dim; d++) span = std::max(span, g.nc[d] * cs[d]);
+    a.delta = std::ldexp(span, -21);
+    for (int d = 0; d < 3; d++)  // only imaged axes shift (elsewhere the shift is 0, keep 0 * lenf finite)
+      a.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
+    a.maxneib = c->maxneib; a.err = c->d_int; a.stripes = c->d_stripes;
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
+#define NEIGH(D, R) hipLaunchKernelGGL((k_neigh<D, R>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
     if (RS == 2) {
-      if (dim == 1) hipLaunchKernelGGL((k_neigh<1, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
-      else if (dim == 2) hipLaunchKernelGGL((k_neigh<2, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
-      else hipLaunchKernelGGL((k_neigh<3, 2>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      if (dim == 1) NEIGH(1, 2);
+      else if (dim == 2) NEIGH(2, 2);
+      else NEIGH(3, 2);
     } else {
-      if (dim == 1) hipLaunchKernelGGL((k_neigh<1, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
-      else if (dim == 2) hipLaunchKernelGGL((k_neigh<2, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
-      else hipLaunchKernelGGL((k_neigh<3, 1>), dim3(G), dim3(kBlock), 0, c->stream, a);
+      if (dim == 1) NEIGH(1, 1);
+      else if (dim == 2) NEIGH(2, 1);
+      else NEIGH(3, 1);
     }
+#undef NEIGH
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
                           c->stream));
@@ -828,7 +851,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->cstart, c->cend, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->band,
+                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
                   c->dvec, c->gpart, c->ckpt};
@@ -967,15 +990,29 @@ int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_us
   dt = std::min(dt, dtcap);
   if (dt_used) *dt_used = dt;
   c->naux = 9;
-  hipLaunchKernelGGL(k_predict, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
-                     dt, 1, c->aux(0), (int)c->npad);
+  hipLaunchKernelGGL(k_predict, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, 1,
+                     c->aux(0), (int)c->npad);
   int rc = run_step(c, state);
   if (rc == SPH_OK)
-    hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real,
-                       (int)c->n, dt, 1, c->aux(0), (int)c->npad);
+    hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, 1,
+                       c->aux(0), (int)c->npad);
   c->naux = 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   return rc;
+}
+
+// update_radenergy(n, store, dt), rad_exchange.f90:9-66, on every loaded
+// particle (rows 1..n = real + fixed, as main.f90:367-369 passes n).
+int sph_rad_exchange(sph_ctx *c, const double state[49], double dt) {
+  if (!c || !state || c->n <= 0 || !(dt > 0.0)) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  const double a = 4. * kSigmaB / kLightspeed;                 // rad_exchange.f90:25
+  const double cv1 = (state[S_GAMMA] - 1.) * state[S_MOLMASS] / kRg;  // :26
+  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_rad_exchange, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, a,
+                     cv1, kSigmaB, c->d_int);
+  RCHK(read_ints(c, 0, 1));
+  return (c->h_int[0] & EB_RAD) ? SPH_ERR_RAD_NEGATIVE : SPH_OK;
 }
 
 // sts_integrator.f90:10-51
@@ -1051,6 +1088,7 @@ const char *sph_error_string(int code) {
     case SPH_ERR_HIP: return "HIP runtime error";
     case SPH_ERR_NOMEM: return "device allocation failed";
     case SPH_ERR_UNSUPPORTED: return "unsupported configuration";
+    case SPH_ERR_RAD_NEGATIVE: return "radiation or thermal energy is negative after exchange";
     default: return "unknown error";
   }
 }

@@ -15,9 +15,11 @@ namespace sph {
 
 enum { KT_M4 = 0, KT_M6 = 1, KT_GAUSS = 2 };
 
-// Reference constants (const.f90:4,6; base_*.f90 knorm/krad/fwc/maxneibnum).
+// Reference constants (const.f90:4-8; base_*.f90 knorm/krad/fwc/maxneibnum).
 constexpr double kPi = 3.141592653589793;
 constexpr double kLightspeed = 2.997924e10;
+constexpr double kRg = 8.314e7;
+constexpr double kSigmaB = 5.67051e-5;
 
 __host__ __device__ constexpr double krad_of(int kt) { return kt == KT_M4 ? 2.0 : (kt == KT_M6 ? 3.0 : 10.0); }
 

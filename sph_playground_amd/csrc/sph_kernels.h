@@ -22,6 +22,7 @@ enum : int {
   EB_Q = 4,          // q NaN or negative
   EB_OVERFLOW = 8,   // neighbour storage too small (host grows it and reruns)
   EB_OUTSIDE = 16,   // particle outside the box on an imaged axis
+  EB_RAD = 32,       // energy <= 0 after the radiative exchange
 };
 
 struct Fields {
@@ -43,18 +44,18 @@ constexpr int kStripes = 64;
 // hydro/AV/MHD/FLD terms, the density+force step of the diffusion workloads.
 enum { MODE_GENERIC = 0, MODE_DIFF = 1 };  // counters are striped over 64 slots to avoid same-address atomics
 
+// Positions, search records, lists and counts travel as separate
+// __restrict__ kernel arguments.
 struct NeighArgs {
-  const double4 *pm;       // x, y, z, m (sorted)
   const double *hs;        // h used for the search radius (pre-loop-1 h)
-  const uint16_t *band;    // ghost-band codes
   const uint8_t *real;     // 1 = real particle
-  const int *cstart, *cend;
+  const int2 *crange;      // [first, last) sorted index of every cell
   Grid g;
   Box b;
+  float lenf[3];           // box lengths in FP32 (image shifts of the search records)
+  double delta;            // bound on the FP32 coordinate error of a search record
   double krad;
   int n, npad, cap, maxneib;
-  uint32_t *nb;            // chunked, see nb_slot
-  int *cnt;                // [n]
   int *err;                // error bits
   long long *stripes;      // [kStripes] pair counts, [kStripes..2k) max counts
 };

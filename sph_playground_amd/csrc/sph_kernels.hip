@@ -199,7 +199,7 @@ __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const in
 }
 
 // Particle range of every occupied cell (cells indexed row-major).
-__global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int *cstart, int *cend) {
+__global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int2 *crange) {
   int s = gtid();
   if (s >= n) return;
   uint32_t k = keys[s];
@@ -208,15 +208,16 @@ __global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int *cstart, i
   int c[3];
   morton_cell(k, g.mdim, c);
   int rm = (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
-  if (first) cstart[rm] = s;
-  if (last) cend[rm] = s + 1;
+  if (first) crange[rm].x = s;
+  if (last) crange[rm].y = s + 1;
 }
 
-// Positions/masses, search h, ghost-band codes (BC.f90:196-309: strict
+// Positions/masses, search h, the FP32 search record (coordinates relative
+// to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
-__global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, int dim, double4 *pm,
-                       double *h0, uint16_t *band, uint8_t *real, double4 *st, double *eps) {
+__global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *eps) {
   int s = gtid();
   if (s >= n) return;
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
@@ -245,23 +246,27 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
       code |= (st_ << (2 * a)) | (lo << (6 + 2 * a));
     }
   }
-  band[s] = (uint16_t)code;
+  pf[s] = make_float4((float)(x - g.lo[0]), (float)(y - g.lo[1]), (float)(z - g.lo[2]), __uint_as_float(code));
 }
 
-// Does the reference hold a ghost of j shifted by (sx,sy,sz)?  The first
-// shifted axis uses j's own (strict) band, later axes the chained test.
-__device__ __forceinline__ bool image_exists(uint32_t band, int sx, int sy, int sz) {
-  int sh[3] = {sx, sy, sz};
+// Does the reference hold a ghost of j shifted by (sx, sy, sz)?  The first
+// shifted axis uses j's own (strict) band, later axes the chained test
+// (BC.f90:342-355).  For fixed shifts that is (band & m) == v, with m, v set
+// here; no shift: m = v = 0.
+__device__ __forceinline__ void image_mask(int sx, int sy, int sz, uint32_t &m, uint32_t &v) {
+  const int sh[3] = {sx, sy, sz};
   bool first = true;
+  m = 0;
+  v = 0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     if (sh[a] == 0) continue;
-    uint32_t want = sh[a] > 0 ? 1u : 2u;
-    uint32_t got = first ? (band >> (2 * a)) & 3u : (band >> (6 + 2 * a)) & 3u;
-    if (got != want) return false;
+    const uint32_t want = sh[a] > 0 ? 1u : 2u;
+    const int pos = first ? 2 * a : 6 + 2 * a;
+    m |= 3u << pos;
+    v |= want << pos;
     first = false;
   }
-  return true;
 }
 
 // Squared separation exactly as kdtree2.f90:1673-1677 rounds it:
@@ -282,22 +287,42 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 // (cells of about r_max / RS; cells wrap with an image shift along imaged
 // axes) with the kd-tree's FP64 inclusion rule sd <= (krad*h_i)^2; self
 // (unshifted) is dropped.  Cells whose box lies farther than the radius
-// (with a safety margin) are skipped; candidates are loaded four at a time.
+// (with a safety margin) are skipped.
+//
+// Candidates arrive as one 16-byte record each (pf: FP32 coordinates
+// relative to the grid origin, and the band bits), four at a time.  The FP32
+// squared distance decides every candidate farther than the error band
+// around the radius: |sd32 - |d|^2| <= 2 sqrt3 r delta + 3 delta^2 + 2^-20 sd
+// with delta = a.delta (the FP32 coordinate error bound, 2^-21 x the grid
+// span).  Candidates inside the band, and any lane whose radius is not a
+// finite number, take the exact FP64 rule on pm, so the sets are those of
+// the FP64 test.
 template <int DIM, int RS>
-__global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
+__global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
+                                               const float4 *__restrict__ pf, uint32_t *__restrict__ nb,
+                                               int *__restrict__ cnt) {
   __shared__ long long lds[4];
   int s = gtid();
   int count = 0;
   bool live = s < a.n && a.real[s];
   if (live) {
-    const double4 pi = a.pm[s];
+    const double4 pi = pm[s];
+    const float4 pif = pf[s];
     const double rr = a.krad * a.hs[s];
     const double r2 = rr * rr;
     const double r2m = r2 * (1.0 + 1e-6);  // pruning bound, never tighter than the exact test
+    // FP32 decision thresholds; a radius that is not a finite number sends
+    // every candidate to the exact test
+    const double mabs = 2.0 * 1.7320508075688772 * 1.01 * rr * a.delta + 3.0 * a.delta * a.delta;
+    const bool finite_r = rr == rr && rr < HUGE_VAL;
+    const float hi_thr = finite_r ? (float)(r2 * (1.0 + 1e-5) + mabs) : HUGE_VALF;
+    const float lo_thr = finite_r ? (float)(r2 * (1.0 - 1e-5) - mabs) : -HUGE_VALF;
+    // list slot of entry `count` (nb_slot), advanced incrementally
+    size_t widx = (size_t)s * 8;
+    const size_t wstep = (size_t)a.npad * 8 - 7;
     int ci[3];
     cell_of(a.g, pi.x, pi.y, pi.z, ci);
-    const double cs[3] = {a.g.inv_cs[0] > 0 ? 1.0 / a.g.inv_cs[0] : 0.0, a.g.inv_cs[1] > 0 ? 1.0 / a.g.inv_cs[1] : 0.0,
-                          a.g.inv_cs[2] > 0 ? 1.0 / a.g.inv_cs[2] : 0.0};
+    const double *cs = a.g.cs;
     const double px[3] = {pi.x, pi.y, pi.z};
     const int rz = DIM == 3 ? RS : 0, ry = DIM >= 2 ? RS : 0;
     for (int dz = -rz; dz <= rz; dz++) {
@@ -339,40 +364,52 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a) {
           }
           const int c = (cz * a.g.nc[1] + cy) * a.g.nc[0] + cx;
           const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
-          const int t0 = a.cstart[c], t1 = a.cend[c];
-          for (int t = t0; t < t1; t += 4) {
-            double4 pj[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) pj[u] = a.pm[min(t + u, t1 - 1)];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int tt = t + u;
-              if (tt >= t1) break;
-              double sd;
-              if (code == kNoImage) {
-                if (tt == s) continue;
-                sd = kd_sqdist(pj[u].x, pj[u].y, pj[u].z, pi.x, pi.y, pi.z);
-              } else {
-                if (!image_exists(a.band[tt], sx, sy, sz)) continue;
-                double xj = image_coord(pj[u].x, sx, a.b.emin[0], a.b.emax[0]);
-                double yj = image_coord(pj[u].y, sy, a.b.emin[1], a.b.emax[1]);
-                double zj = image_coord(pj[u].z, sz, a.b.emin[2], a.b.emax[2]);
-                sd = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z);
-              }
-              if (sd <= r2) {
-                if (count < a.cap) a.nb[nb_slot(count, s, a.npad)] = nb_pack((uint32_t)tt, code);
-                count++;
-              }
+          // the image shift in the FP32 frame, and the lane position in it
+          const float ox = pif.x - (float)sx * a.lenf[0], oy = pif.y - (float)sy * a.lenf[1],
+                      oz = pif.z - (float)sz * a.lenf[2];
+          const int2 tr = a.crange[c];
+          const int t0 = tr.x, t1 = tr.y;
+          // one candidate: FP32 decision, the exact FP64 rule inside the band
+          auto test = [&](int tt, const float4 &q, bool ok) {
+            const float fx = q.x - ox, fy = q.y - oy, fz = q.z - oz;
+            const float sd32 = fx * fx + fy * fy + fz * fz;
+            bool hit = ok && sd32 <= hi_thr;
+            if (hit && !(sd32 < lo_thr)) {
+              const double4 pj = pm[tt];
+              const double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
+              const double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
+              const double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
+              hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
             }
+            if (hit) {
+              if (count < a.cap) nb[widx] = nb_pack((uint32_t)tt, code);
+              count++;
+              widx += (count & 7) ? 1 : wstep;
+            }
+          };
+          // image existence as (band & bm) == bv; self (unshifted) excluded
+          uint32_t bm, bv;
+          image_mask(sx, sy, sz, bm, bv);
+          const int self_t = code == kNoImage ? s : -1;
+          // records t..t+3 (pf holds 4 records of padding past n; the ones
+          // past t1 belong to other cells and are masked)
+          for (int t = t0; t < t1; t += 4) {
+            const float4 *p = pf + t;
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) q[u] = p[u];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
           }
         }
       }
     }
-    a.cnt[s] = count;
+    cnt[s] = count;
     if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
     if (count > a.cap) atomicOr(a.err, EB_OVERFLOW);
   } else if (s < a.n) {
-    a.cnt[s] = 0;
+    cnt[s] = 0;
   }
   long long tot = block_reduce<R_SUM>(count, lds);
   if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long *)&a.stripes[stripe()], (unsigned long long)tot);
@@ -946,12 +983,13 @@ __global__ void k_find_orig(const int64_t *orig, int n, int64_t target, int *pos
 // These elementwise kernels are HBM-bound; they turn FP contraction off so
 // every product and sum rounds as the reference's unfused Fortran does.
 //
-// Leapfrog predictor (main.f90:323-342) on real particles; saves the old
-// derivatives in `sv` (ax, ay, az, du, dh, ddt, dbx, dby, dbz) for the corrector.
-__global__ void k_predict(Fields F, const uint8_t *real, int n, double dt, int do_t, double *sv, int npad) {
+// Leapfrog predictor (main.f90:323-342) on rows 1..n of the reference, i.e.
+// real and fixed particles (main.f90:181-184); saves the old derivatives in
+// `sv` (ax, ay, az, du, dh, ddt, dbx, dby, dbz) for the corrector.
+__global__ void k_predict(Fields F, int n, double dt, int do_t, double *sv, int npad) {
 #pragma clang fp contract(off)
   int s = gtid();
-  if (s >= n || !real[s]) return;
+  if (s >= n) return;
   const int src[9] = {F_AX, F_AY, F_AZ, F_DU, F_DH, F_DDT, F_DBX, F_DBY, F_DBZ};
   double o[9];
   for (int k = 0; k < 9; k++) { o[k] = F.f[src[k]][s]; sv[(size_t)k * npad + s] = o[k]; }
@@ -966,11 +1004,11 @@ __global__ void k_predict(Fields F, const uint8_t *real, int n, double dt, int d
   if (do_t) F.f[F_T][s] = F.f[F_T][s] + dt * o[5];
 }
 
-// Leapfrog corrector (main.f90:348-362).
-__global__ void k_correct(Fields F, const uint8_t *real, int n, double dt, int do_t, const double *sv, int npad) {
+// Leapfrog corrector (main.f90:348-362), same rows as the predictor.
+__global__ void k_correct(Fields F, int n, double dt, int do_t, const double *sv, int npad) {
 #pragma clang fp contract(off)
   int s = gtid();
-  if (s >= n || !real[s]) return;
+  if (s >= n) return;
   double o[9];
   for (int k = 0; k < 9; k++) o[k] = sv[(size_t)k * npad + s];
   for (int a = 0; a < 3; a++) {
@@ -1025,6 +1063,34 @@ __global__ void k_gravity(Fields F, const uint8_t *real, int n, int mode) {
   else F.f[F_AY][s] = F.f[F_AY][s] - erf(F.f[F_RY][s] * 8.);
 }
 
+// update_radenergy, rad_exchange.f90:9-66 with solve_internal_energy_implicit
+// (:100-118): per particle, Newton on
+//   f(u) = u/dt - u0/dt - du - ack (rho (etot - u)/a - (u cv1)^4),
+// ack = 4 sigma kappa, from u = u0 while |u - u_old| > 1e-16, at most 10
+// iterations; then u = u_new, T (radiation energy per mass) = etot - u_new.
+// Powers are products in the reference's order; no FP contraction.
+__global__ void k_rad_exchange(Fields F, int n, double dt, double a, double cv1, double sigmab, int *err) {
+#pragma clang fp contract(off)
+  int s = gtid();
+  if (s >= n) return;
+  const double ack = 4. * sigmab * F.f[F_KAPPA][s];
+  const double rho = F.f[F_DEN][s], u0 = F.f[F_U][s], dudt = F.f[F_DU][s];
+  const double etot = u0 + F.f[F_T][s];
+  const double cv2 = cv1 * cv1, cv4 = cv2 * cv2;
+  double unew = u0, uold = 2 * u0;
+  for (int iter = 0; fabs(unew - uold) > 1e-16 && iter < 10; iter++) {
+    uold = unew;
+    const double x = unew * cv1, x2 = x * x;
+    const double fu = unew / dt - u0 / dt - dudt - ack * (rho * (etot - unew) / a - x2 * x2);
+    const double dfu = 1. / dt + ack * (rho / a + 4. * (unew * (unew * unew) * cv4));
+    unew = unew - fu / dfu;
+  }
+  const double t = etot - unew;
+  F.f[F_U][s] = unew;
+  F.f[F_T][s] = t;
+  if (!(t > 0.0) || !(unew > 0.0)) atomicOr(err, EB_RAD);
+}
+
 // RKL2 stage combination on T (sts_integrator.f90:111-115) over every
 // particle slot, as the reference updates whole store rows.
 __global__ void k_sts_stage(const double *y0, const double *yjm1, const double *yjm2, const double *dy0,
@@ -1066,12 +1132,15 @@ SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
 SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
 
-template __global__ void k_neigh<1, 1>(NeighArgs);
-template __global__ void k_neigh<2, 1>(NeighArgs);
-template __global__ void k_neigh<3, 1>(NeighArgs);
-template __global__ void k_neigh<1, 2>(NeighArgs);
-template __global__ void k_neigh<2, 2>(NeighArgs);
-template __global__ void k_neigh<3, 2>(NeighArgs);
+#define SPH_INST_NEIGH(DIM, RS)                                                                       \
+  template __global__ void k_neigh<DIM, RS>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
+                                            uint32_t *__restrict__, int *__restrict__);
+SPH_INST_NEIGH(1, 1)
+SPH_INST_NEIGH(2, 1)
+SPH_INST_NEIGH(3, 1)
+SPH_INST_NEIGH(1, 2)
+SPH_INST_NEIGH(2, 2)
+SPH_INST_NEIGH(3, 2)
 template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
                                           double, double4 *, double4 *, double4 *, double4 *, double4 *,
                                           long long *);

@@ -17,6 +17,7 @@ struct Box {
 struct Grid {
   double lo[3];
   double inv_cs[3];
+  double cs[3];      // cell widths (the box of cell k spans lo + [k, k+1) cs)
   int nc[3];
   int periodic[3];  // 1 if the axis is imaged (cells wrap with a shift)
   int mdim;         // axes interleaved into the Morton sort key (= dim)

@@ -99,6 +99,12 @@ class Engine:
         s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
         check(lib().sph_sts(self._h, _dptr(s), int(s_stages), float(dt)), "sph_sts")
 
+    def rad_exchange(self, dt, state=None):
+        """Implicit gas/radiation energy exchange over dt on the resident state
+        (update_radenergy, rad_exchange.f90:9-118)."""
+        s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        check(lib().sph_rad_exchange(self._h, _dptr(s), float(dt)), "sph_rad_exchange")
+
     def neighbours(self):
         """Neighbour lists of the last step: (off, j, code) in original numbering."""
         nr = self.n_real

@@ -75,6 +75,12 @@ module sph_hip_capi
       integer(c_int), value :: s
       real(c_double), value :: dt
     end function
+    integer(c_int) function sph_rad_exchange(ctx, state, dt) bind(C, name="sph_rad_exchange")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: state(49)
+      real(c_double), value :: dt
+    end function
     integer(c_int) function sph_sts_coeffs(j, s, out) bind(C, name="sph_sts_coeffs")
       import :: c_int, c_double
       integer(c_int), value :: j, s

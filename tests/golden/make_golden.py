@@ -74,6 +74,11 @@ def cases():
     # one leapfrog step with the flux-limited-diffusion time step (timestep.f90:83-136)
     c.append(("fld3d_fab_3000_leapfrog", 1, 1, 0, 1.0,
               lambda: P.random_slab3d(3000, seed=7, hfac=0.85, velocity_seed=9, fld=True), {}))
+    # radiative exchange alone (rad_exchange.f90:9-118), then after a hyrad leapfrog step
+    c.append(("radexch3d_3000", 1, 5, 0, 1e-3,
+              lambda: P.random_slab3d(3000, seed=7, hfac=0.85, equations="hyrad"), {}))
+    c.append(("hyrad3d_3000_leapfrog_radexch", 1, 4, 0, 1.0,
+              lambda: P.random_slab3d(3000, seed=7, hfac=0.85, velocity_seed=9, equations="hyrad"), {}))
     return c
 
 
@@ -90,8 +95,10 @@ def main():
         r = refrun.run(state, st, kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, threads=1)
         nr = int(state[C.ec_realpn - 1])
         off = np.zeros(nr + 1, dtype=np.int64)
-        off[1:] = np.cumsum([len(a) for a in r["neigh"]])
-        nb = np.concatenate(r["neigh"]) if off[-1] else np.zeros((0, 2), np.int32)
+        neigh = r["neigh"] if r["neigh"] is not None else []   # mode 5 searches nothing
+        if neigh:
+            off[1:] = np.cumsum([len(a) for a in neigh])
+        nb = np.concatenate(neigh) if off[-1] else np.zeros((0, 2), np.int32)
         meta = dict(kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, dt_used=r["dt"],
                     reference_build="amdflang -fdefault-real-8 -fdefault-double-8 -O2 -fopenmp "
                                     "-ffp-contract=off, OMP_NUM_THREADS=1")

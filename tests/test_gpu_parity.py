@@ -193,3 +193,30 @@ def test_error_codes(gpu):
     with pytest.raises(SphError) as e:
         _engine(state, 1).iterate(st.copy())
     assert e.value.code == 4  # anisotropic FLD, circuit2.f90:189
+
+
+def test_rad_exchange_matches_reference_golden(gpu):
+    # update_radenergy alone (rad_exchange.f90:9-118), every loaded particle
+    g = G.load("radexch3d_3000")
+    eng = _engine(g["state"], 1)
+    eng.load(g["store_in"])
+    eng.rad_exchange(g["meta"]["dtcap"])
+    out = eng.save()
+    errs = G.field_errors(g["store_out"], out, out.shape[0])
+    assert max(errs.values()) <= FP64_TOL, errs
+
+
+def test_hyrad_leapfrog_then_rad_exchange_matches_reference_golden(gpu):
+    # main.f90:323-369 for hyrad: predictor, iterate, corrector, update_radenergy
+    g = G.load("hyrad3d_3000_leapfrog_radexch")
+    nr = int(g["state"][C.ec_realpn - 1])
+    eng = _engine(g["state"], 1)
+    eng.load(g["store_in"])
+    dt = eng.leapfrog(g["meta"]["dtcap"])
+    assert abs(dt - g["meta"]["dt_used"]) <= 1e-14 * abs(g["meta"]["dt_used"])
+    eng.rad_exchange(dt)
+    out = eng.save()
+    errs = G.field_errors(g["store_out"], out, nr)
+    assert max(errs.values()) <= FP64_TOL, errs
+    off, j, code = eng.neighbours()
+    assert not G.compare_neighbours(g, off, j, code)

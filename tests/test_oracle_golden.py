@@ -35,7 +35,8 @@ def test_golden_set_is_complete():
         assert any(tag in n for n in names), tag
     for n in ("hydro2d_m4_16", "hydro2d_gauss_16", "hydro1d_m6_64", "mhd3d_hydroshock_2000",
               "fld3d_fab_2000", "hyrad3d_2000", "diff2d_fab_sts3_16", "hydro2d_m6_32_leapfrog",
-              "hydro2d_mti_16", "fld3d_fab_3000_leapfrog"):
+              "hydro2d_mti_16", "fld3d_fab_3000_leapfrog", "radexch3d_3000",
+              "hyrad3d_3000_leapfrog_radexch"):
         assert n in names
 
 
@@ -63,3 +64,41 @@ def test_oracle_error_codes_mirror_reference_aborts():
     # random 3D set with hfac 1.2 overflows the M6 cap of 150 (neighboursearch.f90:127-132)
     state, st = P.random_slab3d(2000, seed=3, hfac=1.2)
     assert oracle.iterate(state, st, kernel=1, check=False)["rc"] == 1
+
+
+def test_oracle_rad_exchange_matches_reference():
+    """update_radenergy (rad_exchange.f90:9-118) alone: every row of the
+    reference's output, 1e-12 relative; u + T is conserved exactly and the
+    Newton residual of the implicit equation vanishes."""
+    g = G.load("radexch3d_3000")
+    st, rc = oracle.rad_exchange(g["state"], g["store_in"], g["meta"]["dtcap"])
+    assert rc == 0
+    n = st.shape[0]
+    errs = G.field_errors(g["store_out"], st, n)
+    assert max(errs.values()) <= 1e-12, errs
+    u0 = g["store_in"][:, C.es_u - 1]
+    t0 = g["store_in"][:, C.es_t - 1]
+    u = st[:, C.es_u - 1]
+    assert np.array_equal(u + st[:, C.es_t - 1], u0 + t0) or \
+        np.abs(u + st[:, C.es_t - 1] - (u0 + t0)).max() <= 1e-15 * np.abs(u0 + t0).max()
+    # residual of f(u) = (u - u0)/dt - du - ack (rho (etot - u)/a - (u cv1)^4)
+    dt = g["meta"]["dtcap"]
+    sigmab, c, rg = 5.67051e-5, 2.997924e10, 8.314e7
+    a = 4 * sigmab / c
+    cv1 = (g["state"][C.ec_gamma - 1] - 1) * g["state"][C.ec_molecularmass - 1] / rg
+    ack = 4 * sigmab * g["store_in"][:, C.es_kappa - 1]
+    rho = g["store_in"][:, C.es_den - 1]
+    du = g["store_in"][:, C.es_du - 1]
+    f = (u - u0) / dt - du - ack * (rho * (u0 + t0 - u) / a - (u * cv1) ** 4)
+    scale = np.abs(u / dt) + np.abs(u0 / dt) + np.abs(du) + np.abs(ack * rho * (u0 + t0) / a)
+    assert np.max(np.abs(f) / scale) <= 1e-13
+
+
+def test_oracle_rad_exchange_flags_negative_energy():
+    """A planted negative radiation energy (on the CPU, in the oracle's input)
+    makes the exchange report the reference's stop (rad_exchange.f90:60-63)."""
+    g = G.load("radexch3d_3000")
+    st = g["store_in"].copy()
+    st[0, C.es_t - 1] = -10.0 * (abs(st[0, C.es_u - 1]) + 1.0)
+    _, rc = oracle.rad_exchange(g["state"], st, g["meta"]["dtcap"], check=False)
+    assert rc == 11
