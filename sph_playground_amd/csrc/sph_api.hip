@@ -30,7 +30,7 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_range(const uint32_t *, int, Grid, int2 *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *);
+                       uint8_t *, double4 *, double *, double *);
 template <int DIM, int RS>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -40,7 +40,7 @@ template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1_simple(Loop1Args<double4>, Fields);
 template <typename V4>
 __global__ void k_post1(Fields, const double4 *, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
-                        V4 *, V4 *, V4 *, long long *);
+                        V4 *, V4 *, V4 *, double4 *, double *, long long *);
 template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ void k_loop2(Loop2Args<V4>);
 __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
@@ -118,6 +118,9 @@ struct sph_ctx {
   int2 *crange = nullptr;
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
+  double *tr[2] = {nullptr, nullptr};  // T den of the loop-1 sweep state st[k]
+  double4 *pd = nullptr;                // loop 2, diffusion mode: x, y, z, m/den
+  double *trd = nullptr;                // loop 2, diffusion mode: T den
   double *h0 = nullptr, *eps = nullptr;
   float4 *pf = nullptr;   // FP32 search records (relative coordinates + band bits)
   uint8_t *real = nullptr;
@@ -243,7 +246,10 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
     RCHK(dalloc(&c->fbuf[b], nf * npad));
     RCHK(dalloc(&c->orig[b], npad));
     RCHK(dalloc(&c->st[b], npad));
+    RCHK(dalloc(&c->tr[b], npad));
   }
+  RCHK(dalloc(&c->pd, npad));
+  RCHK(dalloc(&c->trd, npad));
   RCHK(dalloc(&c->keys, npad));
   RCHK(dalloc(&c->keys2, npad));
   RCHK(dalloc(&c->vals, npad));
@@ -452,7 +458,7 @@ static int build_neighbours(sph_ctx *c) {
   HIPCHK(hipMemsetAsync(c->crange, 0, sizeof(int2) * ncell, c->stream));
   hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->crange);
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->eps);
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->eps);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -557,6 +563,8 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
   for (int it = 1; c->n_real > 0 && it <= maxsw; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
+    a.trc = c->tr[(it - 1) & 1];
+    a.trn = c->tr[it & 1];
     HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
@@ -630,7 +638,8 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   V4 *q4 = want_q4 ? (V4 *)c->q[4] : nullptr;
   HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
-                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, c->d_stripes);
+                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
+                     generic ? nullptr : c->trd, c->d_stripes);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
   HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * kStripes, hipMemcpyDeviceToHost, c->stream));
@@ -640,6 +649,8 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   Loop2Args<V4> a{};
   a.pm = c->pm; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
+  a.pd = generic ? nullptr : c->pd;
+  a.trd = generic ? nullptr : c->trd;
   a.v_nonzero = (fl & 1) ? 1 : 0;
   a.b_uniform = (fl & 2) ? 0 : 1;
   a.out = F; a.energy_part = c->energy_part; a.err = c->d_int; a.b = c->box; a.p = P; a.n = n;
@@ -851,7 +862,8 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->h0, c->eps, c->pf,
+                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->pd, c->trd,
+                  c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
                   c->dvec, c->gpart, c->ckpt};

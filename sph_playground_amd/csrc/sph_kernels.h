@@ -69,6 +69,8 @@ struct Loop1Args {
   const int *cnt;
   const V4 *cur;           // h, den, om, T of sweep k
   V4 *next;                // sweep k+1
+  const double *trc;       // T den of sweep k (the only j-side value grad T needs)
+  double *trn;             // T den of sweep k+1
   double *eps;
   double *dtdx, *dtdy, *dtdz;
   long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs
@@ -90,6 +92,8 @@ struct Loop2Args {
   const V4 *q2;  // c, u, kappa, -
   const V4 *q3;  // dtdx, dtdy, dtdz, -
   const V4 *q4;  // bx, by, bz, -
+  const double4 *pd;  // MODE_DIFF: x, y, z, m / den
+  const double *trd;  // MODE_DIFF: T den
   int v_nonzero;  // 0: every velocity is exactly 0 (v_ab = 0 for all pairs)
   int b_uniform;  // 1: every particle carries the same B bit for bit (kappa_b = kappa_a)
   Fields out;

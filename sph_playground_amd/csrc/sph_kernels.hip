@@ -18,6 +18,8 @@
 // reduced per block and added to one of 64 stripes, never all to one address.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sph_kernels.h"
 
 namespace sph {
@@ -217,7 +219,7 @@ __global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int2 *crange) 
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
-                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *eps) {
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double *eps) {
   int s = gtid();
   if (s >= n) return;
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
@@ -227,6 +229,7 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
   bool r = orig[s] < n_real;
   real[s] = r ? 1 : 0;
   st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
+  tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
   eps[s] = r ? 1.0 : 0.0;
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
@@ -458,7 +461,10 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   bool act = inb && a.real[s] && a.eps[s] > a.tol;
   bool still = false;
   int nn = 0;
-  if (inb && !act) a.next[s] = c;
+  if (inb && !act) {
+    a.next[s] = c;
+    if constexpr (GRAD == 1) a.trn[s] = a.trc[s];
+  }
   if (act) {
     using K = KernelFn<KT, PT>;
     const double ha = c.x, rhoa = c.y, oma = c.z, ta = c.w;
@@ -469,13 +475,16 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     PT g0 = 0, g1 = 0, g2 = 0, h0x = 0, h0y = 0, h0z = 0;
     bool qbad = !(ha > 0.0);
     nn = a.cnt[s];
-    struct J { double4 pb, sb; uint32_t code; };
+    // j side: x, y, z, m, and what grad T needs of the previous sweep (GRAD 1:
+    // T_b rho_b alone; GRAD 2: h_b, rho_b, om_b, T_b)
+    struct J { double4 pb, sb; double tr; uint32_t code; };
     auto load = [&](uint32_t ent) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
       d.pb = a.pm[j];
-      if constexpr (GRAD != 0) d.sb = a.cur[j];  // hb, rhob, omb, tb of the previous sweep
+      if constexpr (GRAD == 1) d.tr = a.trc[j];
+      if constexpr (GRAD == 2) d.sb = a.cur[j];
       return d;
     };
     for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
@@ -494,7 +503,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
         const double4 sb = d.sb;
         PT inv_dr = PT(1) / dr;
         if constexpr (GRAD == 1) {
-          PT w = mb * ((PT)sb.w * (PT)sb.y - (PT)ta * (PT)rhoa) * df * inv_dr;
+          PT w = mb * ((PT)d.tr - (PT)ta * (PT)rhoa) * df * inv_dr;
           g0 += w * r0; g1 += w * r1; g2 += w * r2;
         } else {
           PT inv_hb = (PT)(1.0 / sb.x);
@@ -527,6 +536,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     a.eps[s] = ep;
     still = ep > a.tol;
     a.next[s] = make_double4(hn, den, osm, ta);
+    if constexpr (GRAD == 1) a.trn[s] = ta * den;
     double gx = 0, gy = 0, gz = 0;
     if constexpr (GRAD == 1) {
       double cN = wcv * ipow<DIM + 1>(ih) / rhoa / oma;
@@ -624,7 +634,8 @@ __global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fiel
 template <typename V4>
 __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n,
                                                int advanced, int do_eos, int iso, double c1, double gamma, V4 *q0,
-                                               V4 *q1, V4 *q2, V4 *q3, V4 *q4, long long *flags) {
+                                               V4 *q1, V4 *q2, V4 *q3, V4 *q4, double4 *pd, double *trd,
+                                               long long *flags) {
   __shared__ long long lds[4];
   using T = decltype(V4().x);
   int s = gtid();
@@ -654,6 +665,10 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
     if (q4) q4[s] = V4{(T)bx, (T)by, (T)bz, (T)0};
+    if (pd) {
+      pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
+      trd[s] = F.f[F_T][s] * den;
+    }
   }
   long long t = block_reduce<R_OR>(fl, lds);
   if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
@@ -759,15 +774,24 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
     PT dvx = 0, dvy = 0, dvz = 0, du = 0, ddt = 0, dbx = 0, dby = 0, dbz = 0, drho = 0;
     bool qbad = !(inv_ha > PT(0));
     const int nn = a.cnt[s];
-    // batched: positions/mass and {1/h, den, om, T}; the optional packs
-    // (v, c/u/kappa, grad T, B) are gathered in the body when enabled
-    struct J { double4 pb; V4 q0; uint32_t j, code; };
+    // batched: generic mode gathers positions/mass and {1/h, den, om, T};
+    // MODE_DIFF gathers only what the Hessian term reads, {x, y, z, m/den}
+    // and T den (the same FP64 values the term forms from m, den, T).  The
+    // optional packs (v, c/u/kappa, grad T, B) are gathered in the body.
+    struct JG { double4 pb; V4 q0; uint32_t j, code; };
+    struct JD { double4 pb; double tr; uint32_t j, code; };
+    using J = std::conditional_t<MODE == MODE_DIFF, JD, JG>;
     auto load = [&](uint32_t ent) {
       J d;
       d.j = nb_index(ent);
       d.code = nb_code(ent);
-      d.pb = a.pm[d.j];
-      d.q0 = a.q0[d.j];
+      if constexpr (MODE == MODE_DIFF) {
+        d.pb = a.pd[d.j];
+        d.tr = a.trd[d.j];
+      } else {
+        d.pb = a.pm[d.j];
+        d.q0 = a.q0[d.j];
+      }
       return d;
     };
     constexpr int NBATCH = MODE == MODE_DIFF ? 4 : 1;
@@ -777,9 +801,17 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
       double xb = pb.x, yb = pb.y, zb = pb.z;
       if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
       const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
-      const PT mb = (PT)pb.w;
-      const V4 B0 = d.q0;
-      const PT inv_hb = (PT)B0.x, rhob = (PT)B0.y, omb = (PT)B0.z, tb = (PT)B0.w;
+      // j-side scalars; in MODE_DIFF mb_rhob = m_b / rho_b and trb = T_b rho_b
+      PT mb = 0, inv_hb = 0, rhob = 0, omb = 0, tb = 0, mb_rhob = 0, trb = 0;
+      if constexpr (MODE == MODE_DIFF) {
+        mb_rhob = (PT)pb.w;
+        trb = (PT)d.tr;
+        if (need_v && P.adden) mb = (PT)a.pm[d.j].w;  // d rho/dt of moving particles only
+      } else {
+        const V4 B0 = d.q0;
+        mb = (PT)pb.w;
+        inv_hb = (PT)B0.x; rhob = (PT)B0.y; omb = (PT)B0.z; tb = (PT)B0.w;
+      }
       const PT r2 = rx * rx + ry * ry + rz * rz;
       const PT inv_r2 = PT(1) / r2;
       const PT dr = sqrt(r2);
@@ -876,7 +908,8 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
                                      : -wcv * dfa * ihd1 * inv_dr;  // 0.5*Gab
             contr = half * (PT(DIM + 2) * S - Tr);
           }
-          ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
+          if constexpr (MODE == MODE_DIFF) ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+          else ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
         } else if (MODE == MODE_GENERIC) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
@@ -1142,9 +1175,10 @@ SPH_INST_NEIGH(1, 2)
 SPH_INST_NEIGH(2, 2)
 SPH_INST_NEIGH(3, 2)
 template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
-                                          double, double4 *, double4 *, double4 *, double4 *, double4 *,
-                                          long long *);
+                                          double, double4 *, double4 *, double4 *, double4 *, double4 *, double4 *,
+                                          double *, long long *);
 template __global__ void k_post1<float4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
-                                         double, float4 *, float4 *, float4 *, float4 *, float4 *, long long *);
+                                         double, float4 *, float4 *, float4 *, float4 *, float4 *, double4 *,
+                                         double *, long long *);
 
 }  // namespace sph
