@@ -11,6 +11,13 @@
 // - lam_b rho_b sum_c w_bc, a gather over the reverse neighbour list R_b, and
 // dL/dK_k = sum_a lam_a sum_b m_b/(rho_b rho_a) (T_b rho_b - T_a rho_a) G_ab,k
 // with G_ab,k = d(K:H_ab)/dK_k, K_k = (xx, yy, zz, xy, xz, yz).
+//
+// For one K every pair weight is fixed, so the weights are assembled once per
+// K (k_diff_weights in list order, k_diff_rweights in reverse-list order) and
+// each STS stage is a streaming sparse product with one gather per pair
+// (k_diff_spmv, k_diff_adj_spmv).  The adjoint stages accumulate one
+// coefficient per pair, S_t += lam_a (Zt_b - Zt_a), which k_diff_gcontract
+// contracts with G once per backward pass.
 #include <hip/hip_runtime.h>
 
 #include "sph_kernels.h"
@@ -69,80 +76,238 @@ __device__ __forceinline__ double kdot(const double K[6], const double G[6]) {
   return K[0] * G[0] + K[1] * G[1] + K[2] * G[2] + K[3] * G[3] + K[4] * G[4] + K[5] * G[5];
 }
 
-// out = A(K) Y over every real particle (sorted order).
+// Z = Y rho (the j-side value the operator reads).
+__global__ void k_zrho(int n, const double *Y, const double4 *q0, double *Z) {
+  int s = gtid_a();
+  if (s < n) Z[s] = Y[s] * q0[s].y;
+}
+
+// pxh = (x, y, z, 1 / h) and the 1 / rho of the frozen geometry.
+__global__ void k_diff_pack(int n, const double4 *pm, const double4 *q0, double4 *pxh, double *irho) {
+  int s = gtid_a();
+  if (s < n) {
+    const double4 p = pm[s];
+    const double4 q = q0[s];
+    pxh[s] = make_double4(p.x, p.y, p.z, q.x);
+    irho[s] = 1.0 / q.y;
+  }
+}
+
+// Pair weights of A(K) in list order: W_ab = (m_b / rho_b) K:H_ab at the
+// pair's list slot, and dW_a = sum_b W_ab (A's diagonal times rho_a^2 / rho_a).
 template <int DIM, int KT>
-__global__ __launch_bounds__(256) void k_diff_apply(DiffOpArgs a, const double *Y, double *out) {
+__global__ __launch_bounds__(256) void k_diff_weights(DiffOpArgs a, double *W, double *dW) {
   int s = gtid_a();
   if (s >= a.n || !a.real[s]) return;
   const double4 pa = a.pm[s];
-  const double4 qa = a.q0[s];
-  const double ya = Y[s] * qa.y;
-  double acc = 0.0;
-  const int nn = a.cnt[s];
-  for (int k = 0; k < nn; k++) {
-    const uint32_t e = a.nb[nb_slot(k, s, a.npad)];
-    const uint32_t j = nb_index(e), code = nb_code(e);
-    const double4 pb = a.pm[j];
-    double xb = pb.x, yb = pb.y, zb = pb.z;
-    if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+  const double inv_ha = a.q0[s].x;
+  double sum = 0.0;
+  size_t widx = (size_t)s * 8;
+  const size_t wstep = (size_t)a.npad * 8 - 7;
+  int k = 0;
+  struct J { double4 pb; double rb; uint32_t code; };
+  auto load = [&](uint32_t e) {
+    J d;
+    const uint32_t j = nb_index(e);
+    d.code = nb_code(e);
+    d.pb = a.pm[j];
+    d.rb = a.q0[j].y;
+    return d;
+  };
+  for_neighbours<4>(a.nb, s, a.npad, a.cnt[s], load, [&](const J &d) {
+    double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
+    if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
     double G[6];
-    hess_basis<DIM, KT>(a, pa.x - xb, pa.y - yb, pa.z - zb, qa.x, G);
-    const double rhob = a.q0[j].y;
-    acc += pb.w / rhob * (Y[j] * rhob - ya) * kdot(a.K, G);
-  }
-  out[s] = acc / qa.y;
+    hess_basis<DIM, KT>(a, pa.x - xb, pa.y - yb, pa.z - zb, inv_ha, G);
+    const double w = d.pb.w / d.rb * kdot(a.K, G);
+    W[widx] = w;
+    sum += w;
+    k++;
+    widx += (k & 7) ? 1 : wstep;
+  });
+  dW[s] = sum;
 }
 
-// out = A(K)^T lam, and gA_k = lam^T (dA/dK_k) Ya, gB_k = lam^T (dA/dK_k) Yb
-// (12 FP64 partial sums per block into gpart[block * 12 + ...]).
+// Reverse weights R_t = m_s K:H(r_as, h_a) / rho_a for the entries t = (a -> s)
+// of s's reverse list, stored like the lists (nb_slot layout).
 template <int DIM, int KT>
-__global__ __launch_bounds__(256) void k_diff_adjoint(DiffOpArgs a, const double *lam, const double *Ya,
-                                                      const double *Yb, double *out, double *gpart) {
+__global__ __launch_bounds__(256) void k_diff_rweights(DiffOpArgs a, const double *irho, double *R) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const int nr = a.rcnt[s];
+  for (int k0 = 0; k0 < nr; k0 += 8) {
+    const size_t base = nb_slot(k0, s, a.npad);
+    const uint4 *pe = reinterpret_cast<const uint4 *>(a.rent + base);
+    const uint4 e0 = pe[0], e1 = pe[1];
+    const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+    const int m = min(8, nr - k0);
+#pragma unroll
+    for (int h = 0; h < 8; h += 4) {
+      if (h >= m) break;
+      double4 pi[4];
+      double ri[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t i = h + u < m ? nb_index(e8[h + u]) : (uint32_t)s;
+        pi[u] = a.pxh[i];
+        ri[u] = irho[i];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (h + u >= m) break;
+        const uint32_t cd = nb_code(e8[h + u]);
+        double xb = ps.x, yb = ps.y, zb = ps.z;
+        if (cd != kNoImage) image_shift(xb, yb, zb, cd, a.b);
+        double G[6];
+        hess_basis<DIM, KT>(a, pi[u].x - xb, pi[u].y - yb, pi[u].z - zb, pi[u].w, G);
+        R[base + h + u] = ps.w * kdot(a.K, G) * ri[u];
+      }
+    }
+  }
+}
+
+// out = A(K) Y from Z = Y rho: out_a = (1 / rho_a) sum_b W_ab (Z_b - Z_a);
+// weights stream in list order, Z_b is the one gather per pair.
+__global__ __launch_bounds__(256) void k_diff_spmv(DiffOpArgs a, const double *W, const double *Z, double *out) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double za = Z[s];
+  const int nn = a.cnt[s];
+  double acc = 0.0;
+  for (int k0 = 0; k0 < nn; k0 += 8) {
+    const size_t base = nb_slot(k0, s, a.npad);
+    const uint4 *pe = reinterpret_cast<const uint4 *>(a.nb + base);
+    const double4 *pw = reinterpret_cast<const double4 *>(W + base);
+    const uint4 e0 = pe[0], e1 = pe[1];
+    const double4 w0 = pw[0], w1 = pw[1];
+    const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+    const double w8[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    const int m = min(8, nn - k0);
+    double z8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) z8[u] = Z[u < m ? nb_index(e8[u]) : (uint32_t)s];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (u < m) acc += w8[u] * (z8[u] - za);
+  }
+  out[s] = acc / a.q0[s].y;
+}
+
+// One adjoint stage over the reverse lists, for every real s:
+//   out_s = sum_t lam_a R_t - lam_s dW_s              (A^T lam, when out != null)
+//   S_t  += sum_q lam^q_a (Zt^q_s - Zt^q_a)           (gradient coefficients, UPD)
+// with t = (a -> s); lam comes from LZ.p[0], the stages q < NS of the S update
+// from LZ.p[q].  rent, R and S stream in the list layout; the LZ records of
+// the owners are the gathers per entry.  Folding several stages into one S
+// update divides the S traffic of the backward pass.
+template <int NS, int UPD>
+__global__ __launch_bounds__(256) void k_diff_adj_spmv(DiffOpArgs a, const double *R, const double *dW, LZSet LZ,
+                                                       double *S, double *out) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  double zs[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) zs[q] = UPD ? LZ.p[q][s].y : 0.0;
+  const double ls = LZ.p[0][s].x;
+  const bool tr = out != nullptr;
+  double acc = 0.0;
+  const int nr = a.rcnt[s];
+  for (int k0 = 0; k0 < nr; k0 += 8) {
+    const size_t base = nb_slot(k0, s, a.npad);
+    const uint4 *pe = reinterpret_cast<const uint4 *>(a.rent + base);
+    const uint4 e0 = pe[0], e1 = pe[1];
+    const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+    const int m = min(8, nr - k0);
+    uint32_t i8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) i8[u] = u < m ? nb_index(e8[u]) : (uint32_t)s;
+    if (tr) {
+      const double4 *pr = reinterpret_cast<const double4 *>(R + base);
+      const double4 r0 = pr[0], r1 = pr[1];
+      const double r8[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+      double l8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) l8[u] = LZ.p[0][i8[u]].x;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < m) acc += l8[u] * r8[u];
+    }
+    if constexpr (UPD) {
+      double4 *ps = reinterpret_cast<double4 *>(S + base);
+      double4 s0 = ps[0], s1 = ps[1];
+      double s8[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        double2 g8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) g8[u] = LZ.p[q][i8[u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (u < m) s8[u] += g8[u].x * (zs[q] - g8[u].y);
+      }
+      ps[0] = make_double4(s8[0], s8[1], s8[2], s8[3]);
+      ps[1] = make_double4(s8[4], s8[5], s8[6], s8[7]);
+    }
+  }
+  if (tr) out[s] = acc - ls * dW[s];
+}
+
+// LZ = (lam, (wa Ya + wb Yb) rho).
+__global__ void k_adj_lz(int n, const double *lam, const double *Ya, const double *Yb, double wa, double wb,
+                         const double4 *q0, double2 *LZ) {
+  int s = gtid_a();
+  if (s >= n) return;
+  const double yb = Yb ? wb * Yb[s] : 0.0;
+  LZ[s] = make_double2(lam[s], (wa * Ya[s] + yb) * q0[s].y);
+}
+
+// Tensor gradient from the accumulated coefficients over the reverse lists:
+//   g_k = sum_s sum_t m_s / (rho_s rho_a) G_k(r_as, h_a) S_t
+// as 6 partial sums per block into gpart[block * 12 + k].
+template <int DIM, int KT>
+__global__ __launch_bounds__(256) void k_diff_gcontract(DiffOpArgs a, const double *irho, const double *S,
+                                                        double *gpart) {
   __shared__ double red[4][12];
   int s = gtid_a();
-  double g[12];
-  for (int k = 0; k < 12; k++) g[k] = 0.0;
+  double g[6] = {0, 0, 0, 0, 0, 0};
   if (s < a.n && a.real[s]) {
-    const double4 pa = a.pm[s];
-    const double4 qa = a.q0[s];
-    const double rhoa = qa.y, la = lam[s];
-    const double yaA = Ya[s] * rhoa, yaB = Yb[s] * rhoa;
-    double diag = 0.0;
-    // own list: diagonal of A^T and the tensor gradient
-    const int nn = a.cnt[s];
-    for (int k = 0; k < nn; k++) {
-      const uint32_t e = a.nb[nb_slot(k, s, a.npad)];
-      const uint32_t j = nb_index(e), code = nb_code(e);
-      const double4 pb = a.pm[j];
-      double xb = pb.x, yb = pb.y, zb = pb.z;
-      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
-      double G[6];
-      hess_basis<DIM, KT>(a, pa.x - xb, pa.y - yb, pa.z - zb, qa.x, G);
-      const double rhob = a.q0[j].y;
-      const double mr = pb.w / (rhob * rhoa);
-      diag += mr * kdot(a.K, G);
-      const double dA = mr * (Ya[j] * rhob - yaA), dB = mr * (Yb[j] * rhob - yaB);
-      for (int c = 0; c < 6; c++) { g[c] += dA * G[c]; g[6 + c] += dB * G[c]; }
+    const double4 ps = a.pm[s];
+    const double ms = ps.w * irho[s];
+    const int nr = a.rcnt[s];
+    for (int k0 = 0; k0 < nr; k0 += 8) {
+      const size_t base = nb_slot(k0, s, a.npad);
+      const uint4 *pe = reinterpret_cast<const uint4 *>(a.rent + base);
+      const uint4 e0 = pe[0], e1 = pe[1];
+      const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+      const int m = min(8, nr - k0);
+#pragma unroll
+      for (int h = 0; h < 8; h += 4) {
+        if (h >= m) break;
+        double4 pi[4];
+        double ri[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t i = h + u < m ? nb_index(e8[h + u]) : (uint32_t)s;
+          pi[u] = a.pxh[i];
+          ri[u] = irho[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (h + u >= m) break;
+          const uint32_t cd = nb_code(e8[h + u]);
+          double xb = ps.x, yb = ps.y, zb = ps.z;
+          if (cd != kNoImage) image_shift(xb, yb, zb, cd, a.b);
+          double G[6];
+          hess_basis<DIM, KT>(a, pi[u].x - xb, pi[u].y - yb, pi[u].z - zb, pi[u].w, G);
+          const double c = ms * ri[u] * S[base + h + u];
+          for (int k = 0; k < 6; k++) g[k] += c * G[k];
+        }
+      }
     }
-    for (int c = 0; c < 12; c++) g[c] *= la;
-    // reverse list: lam_a w_ab rho_b for every a with s in N_a
-    double acc = 0.0;
-    const double4 pb = pa;
-    for (int t = a.roff[s]; t < a.roff[s + 1]; t++) {
-      const uint32_t e = a.rent[t];
-      const uint32_t i = nb_index(e), code = nb_code(e);
-      const double4 pi = a.pm[i];
-      const double4 qi = a.q0[i];
-      double xb = pb.x, yb = pb.y, zb = pb.z;
-      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
-      double G[6];
-      hess_basis<DIM, KT>(a, pi.x - xb, pi.y - yb, pi.z - zb, qi.x, G);
-      acc += lam[i] * pb.w / (rhoa * qi.y) * kdot(a.K, G);
-    }
-    out[s] = rhoa * acc - la * rhoa * diag;
   }
   for (int c = 0; c < 12; c++) {
-    double v = g[c];
+    double v = c < 6 ? g[c] : 0.0;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
   }
@@ -154,36 +319,53 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(DiffOpArgs a, const double
   }
 }
 
-// Reverse neighbour lists: count, fill (atomics), then sort each segment so
-// the order (and so every sum over it) does not depend on scheduling.
+// gk[c] += wa * sum_blocks gpart[b*12 + c] + wb * sum_blocks gpart[b*12 + 6 + c]
+// in a fixed order (one block of 256 threads).
+__global__ __launch_bounds__(256) void k_gk_accum(const double *gpart, int nblk, double wa, double wb, double *gk) {
+  __shared__ double sh[12][256];
+  double v[12];
+  for (int c = 0; c < 12; c++) v[c] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int c = 0; c < 12; c++) v[c] += gpart[(size_t)b * 12 + c];
+  for (int c = 0; c < 12; c++) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 12; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) gk[threadIdx.x] += wa * sh[threadIdx.x][0] + wb * sh[6 + threadIdx.x][0];
+}
+
+// Reverse neighbour lists: every list pair (a, k) -> (key j, value: a and the
+// image code of j in a's list), enumerated owner-major at foff[a] + k; a
+// stable radix sort on j then gives each reverse segment in (a, k) order,
+// independent of scheduling.
 __global__ void k_rev_count(const uint8_t *real, const uint32_t *nb, const int *cnt, int n, int npad, int *rcnt) {
   int s = gtid_a();
   if (s >= n || !real[s]) return;
   for (int k = 0; k < cnt[s]; k++) atomicAdd(&rcnt[nb_index(nb[nb_slot(k, s, npad)])], 1);
 }
 
-__global__ void k_rev_fill(const uint8_t *real, const uint32_t *nb, const int *cnt, int n, int npad, const int *roff,
-                           int *rfill, uint32_t *rent) {
+__global__ void k_pair_enum(const uint8_t *real, const uint32_t *nb, const int *cnt, int n, int npad,
+                            const int *foff, uint32_t *keys, uint32_t *vals) {
   int s = gtid_a();
   if (s >= n || !real[s]) return;
+  const int base = foff[s];
   for (int k = 0; k < cnt[s]; k++) {
-    uint32_t e = nb[nb_slot(k, s, npad)];
-    uint32_t j = nb_index(e);
-    int p = atomicAdd(&rfill[j], 1);
-    rent[roff[j] + p] = nb_pack((uint32_t)s, nb_code(e));
+    const uint32_t e = nb[nb_slot(k, s, npad)];
+    keys[base + k] = nb_index(e);
+    vals[base + k] = nb_pack((uint32_t)s, nb_code(e));
   }
 }
 
-__global__ void k_rev_sort(const int *roff, int n, uint32_t *rent) {
+// Sorted reverse entries (CSR by roff) into the list layout of rent.
+__global__ void k_rev_chunk(const int *roff, const uint32_t *flat, int n, int npad, int *rcnt, uint32_t *rent) {
   int s = gtid_a();
   if (s >= n) return;
-  int b = roff[s], e = roff[s + 1];
-  for (int i = b + 1; i < e; i++) {
-    uint32_t v = rent[i];
-    int k = i - 1;
-    while (k >= b && rent[k] > v) { rent[k + 1] = rent[k]; k--; }
-    rent[k + 1] = v;
-  }
+  const int b = roff[s], e = roff[s + 1];
+  rcnt[s] = e - b;
+  for (int k = 0; k < e - b; k++) rent[nb_slot(k, s, npad)] = flat[b + k];
 }
 
 // Vectors in original particle order <-> sorted order.
@@ -194,6 +376,19 @@ __global__ void k_to_sorted(const double *src, const int64_t *orig, int n, doubl
 __global__ void k_from_sorted(const double *src, const int64_t *orig, int n, double *dst) {
   int s = gtid_a();
   if (s < n) dst[orig[s]] = src[s];
+}
+
+// Adjoint of one RKL2 stage's combination (sts_integrator.f90:111-115):
+// L1 += mu Lj + mh U, L2 += nu Lj, L0 += (1 - mu - nu) Lj + gh U (L1 or L2 may
+// be L0; each is updated in turn, as three separate sums would).
+__global__ void k_stage_adj(int n, const double *Lj, const double *U, double mu, double nu, double mh, double gh,
+                            double *L1, double *L2, double *L0) {
+  int s = gtid_a();
+  if (s >= n) return;
+  const double lj = Lj[s], u = U[s];
+  L1[s] = 1.0 * L1[s] + mu * lj + mh * u;
+  L2[s] = 1.0 * L2[s] + nu * lj;
+  L0[s] = 1.0 * L0[s] + (1. - mu - nu) * lj + gh * u;
 }
 
 // y = sum_k c_k x_k over up to 5 vectors (null x_k skipped).
@@ -210,11 +405,18 @@ __global__ void k_lincomb(int n, double *y, double c0, const double *x0, double 
 }
 
 #define SPH_INST_ADJ(DIM, KT)                                                                  \
-  template __global__ void k_diff_apply<DIM, KT>(DiffOpArgs, const double *, double *);       \
-  template __global__ void k_diff_adjoint<DIM, KT>(DiffOpArgs, const double *, const double *, \
-                                                   const double *, double *, double *);
+  template __global__ void k_diff_weights<DIM, KT>(DiffOpArgs, double *, double *);                 \
+  template __global__ void k_diff_rweights<DIM, KT>(DiffOpArgs, const double *, double *);          \
+  template __global__ void k_diff_gcontract<DIM, KT>(DiffOpArgs, const double *, const double *, double *);
 SPH_INST_ADJ(1, 0) SPH_INST_ADJ(1, 1) SPH_INST_ADJ(1, 2)
 SPH_INST_ADJ(2, 0) SPH_INST_ADJ(2, 1) SPH_INST_ADJ(2, 2)
 SPH_INST_ADJ(3, 0) SPH_INST_ADJ(3, 1) SPH_INST_ADJ(3, 2)
+#define SPH_INST_ADJS(NS, UPD) \
+  template __global__ void k_diff_adj_spmv<NS, UPD>(DiffOpArgs, const double *, const double *, LZSet, double *, double *);
+SPH_INST_ADJS(1, 0)
+SPH_INST_ADJS(1, 1)
+SPH_INST_ADJS(2, 1)
+SPH_INST_ADJS(3, 1)
+SPH_INST_ADJS(4, 1)
 
 }  // namespace sph

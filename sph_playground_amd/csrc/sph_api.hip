@@ -58,12 +58,25 @@ __global__ void k_sts_stage(const double *, const double *, const double *, cons
                             double, double, double, double, double *);
 __global__ void k_axpy_copy(double *, const double *, const double *, double, int);
 template <int DIM, int KT>
-__global__ void k_diff_apply(DiffOpArgs, const double *, double *);
+__global__ void k_diff_weights(DiffOpArgs, double *, double *);
 template <int DIM, int KT>
-__global__ void k_diff_adjoint(DiffOpArgs, const double *, const double *, const double *, double *, double *);
+__global__ void k_diff_rweights(DiffOpArgs, const double *, double *);
+template <int DIM, int KT>
+__global__ void k_diff_gcontract(DiffOpArgs, const double *, const double *, double *);
+__global__ void k_diff_spmv(DiffOpArgs, const double *, const double *, double *);
+template <int NS, int UPD>
+__global__ void k_diff_adj_spmv(DiffOpArgs, const double *, const double *, LZSet, double *, double *);
+__global__ void k_stage_adj(int, const double *, const double *, double, double, double, double, double *, double *,
+                            double *);
+__global__ void k_adj_lz(int, const double *, const double *, const double *, double, double, const double4 *,
+                         double2 *);
+__global__ void k_gk_accum(const double *, int, double, double, double *);
+__global__ void k_zrho(int, const double *, const double4 *, double *);
+__global__ void k_diff_pack(int, const double4 *, const double4 *, double4 *, double *);
 __global__ void k_rev_count(const uint8_t *, const uint32_t *, const int *, int, int, int *);
-__global__ void k_rev_fill(const uint8_t *, const uint32_t *, const int *, int, int, const int *, int *, uint32_t *);
-__global__ void k_rev_sort(const int *, int, uint32_t *);
+__global__ void k_rev_chunk(const int *, const uint32_t *, int, int, int *, uint32_t *);
+__global__ void k_pair_enum(const uint8_t *, const uint32_t *, const int *, int, int, const int *, uint32_t *,
+                            uint32_t *);
 __global__ void k_to_sorted(const double *, const int64_t *, int, double *);
 __global__ void k_from_sorted(const double *, const int64_t *, int, double *);
 __global__ void k_lincomb(int, double *, double, const double *, double, const double *, double, const double *,
@@ -147,7 +160,20 @@ struct sph_ctx {
   bool diff_ready = false;
   Phys diff_phys{};
   int *roff = nullptr, *rtmp = nullptr;
-  uint32_t *rent = nullptr;
+  uint32_t *rent = nullptr;      // reverse entries (owner, image), list layout, rcnt per particle
+  double4 *dpxh = nullptr;       // x, y, z, 1 / h of the frozen geometry
+  double *dirho = nullptr;       // 1 / rho of the frozen geometry
+  double *dW = nullptr, *ddW = nullptr;  // forward pair weights (list layout) and their row sums
+  double *dR = nullptr, *dS = nullptr;   // reverse weights and gradient coefficients (reverse order)
+  double *dZ = nullptr;
+  double2 *dLZ[4] = {nullptr, nullptr, nullptr, nullptr};  // (lam, Zt) of adjoint stages, a ring
+  int lz_pend = 0;               // stages in the ring whose S update is deferred (buffers 0..lz_pend-1)
+  bool fw_tail = false;          // the forward kept its last checkpoint segment's stage states
+  int *rcnt = nullptr;           // reverse-list lengths
+  int rcap = 0;                  // reverse-list capacity (list layout, multiple of 8)
+  bool w_valid = false;          // dW/dR hold the weights of w_K
+  double w_K[6] = {0, 0, 0, 0, 0, 0};
+  double *gkdev = nullptr;       // 12 tensor-gradient accumulators
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   double *dvec = nullptr;        // work vectors, diff_nvec x npad
@@ -683,6 +709,9 @@ static Phys call_phys(const sph_ctx *c, const double *state) {
 
 static int run_step(sph_ctx *c, const double *state) {
   if (c->n <= 0) return SPH_ERR_BAD_ARG;
+  // a new step changes the geometry the differentiable operator was prepared on
+  c->diff_ready = false;
+  c->w_valid = false;
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -866,7 +895,8 @@ int sph_ctx_destroy(sph_ctx *c) {
                   c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
-                  c->dvec, c->gpart, c->ckpt};
+                  c->dvec, c->gpart, c->ckpt, c->rcnt, c->dpxh, c->dirho, c->dW, c->ddW, c->dR, c->dS, c->dZ, c->dLZ[0], c->dLZ[1], c->dLZ[2], c->dLZ[3],
+                  c->gkdev};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
@@ -1113,7 +1143,7 @@ const char *sph_error_string(int code) {
 static DiffOpArgs diff_args(sph_ctx *c, const double K[6]) {
   DiffOpArgs a{};
   a.pm = c->pm; a.q0 = (const double4 *)c->q[0]; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
-  a.roff = c->roff; a.rent = c->rent; a.b = c->box;
+  a.rcnt = c->rcnt; a.rent = c->rent; a.pxh = c->dpxh; a.b = c->box;
   for (int k = 0; k < 6; k++) a.K[k] = K[k];
   a.wcv = c->diff_phys.wcv; a.fwc = c->diff_phys.fwc; a.ddw = c->diff_phys.ddw;
   a.n = (int)c->n; a.npad = (int)c->npad;
@@ -1121,46 +1151,97 @@ static DiffOpArgs diff_args(sph_ctx *c, const double K[6]) {
 }
 
 template <int DIM>
-static void launch_apply_d(sph_ctx *c, const DiffOpArgs &a, const double *Y, double *out) {
+static void launch_weights_d(sph_ctx *c, const DiffOpArgs &a) {
   unsigned G = grid_for(c->n);
-  if (c->kt == KT_M4) hipLaunchKernelGGL((k_diff_apply<DIM, KT_M4>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
-  else if (c->kt == KT_M6) hipLaunchKernelGGL((k_diff_apply<DIM, KT_M6>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
-  else hipLaunchKernelGGL((k_diff_apply<DIM, KT_GAUSS>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, out);
+#define WTS(KT)                                                                                                  \
+  do {                                                                                                           \
+    hipLaunchKernelGGL((k_diff_weights<DIM, KT>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dW, c->ddW);       \
+    hipLaunchKernelGGL((k_diff_rweights<DIM, KT>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dirho, c->dR);    \
+  } while (0)
+  if (c->kt == KT_M4) WTS(KT_M4);
+  else if (c->kt == KT_M6) WTS(KT_M6);
+  else WTS(KT_GAUSS);
+#undef WTS
 }
-static void diff_apply(sph_ctx *c, const double K[6], const double *Y, double *out) {
+// Pair weights of A(K), assembled once per K (list order and reverse order).
+static void ensure_weights(sph_ctx *c, const double K[6]) {
+  if (c->w_valid && std::equal(K, K + 6, c->w_K)) return;
   DiffOpArgs a = diff_args(c, K);
-  if (c->diff_phys.dim == 1) launch_apply_d<1>(c, a, Y, out);
-  else if (c->diff_phys.dim == 2) launch_apply_d<2>(c, a, Y, out);
-  else launch_apply_d<3>(c, a, Y, out);
+  if (c->diff_phys.dim == 1) launch_weights_d<1>(c, a);
+  else if (c->diff_phys.dim == 2) launch_weights_d<2>(c, a);
+  else launch_weights_d<3>(c, a);
+  std::copy(K, K + 6, c->w_K);
+  c->w_valid = true;
 }
-
+// out = A(K) Y (sorted vectors).
+static void diff_apply(sph_ctx *c, const double K[6], const double *Y, double *out) {
+  ensure_weights(c, K);
+  DiffOpArgs a = diff_args(c, K);
+  const unsigned G = grid_for(c->n);
+  hipLaunchKernelGGL(k_zrho, dim3(G), dim3(kBlock), 0, c->stream, (int)c->n, Y, a.q0, c->dZ);
+  hipLaunchKernelGGL(k_diff_spmv, dim3(G), dim3(kBlock), 0, c->stream, a, c->dW, c->dZ, out);
+}
+// One adjoint stage: out = A^T lam, and the stage's gradient coefficients
+// S += lam_a (Zt_b - Zt_a), Zt = (wa Ya + wb Yb) rho (Yb may be null).  S
+// updates are deferred and done four stages at a time (one pass over S per
+// four stages); diff_flush settles the deferred ones.
+static void diff_adj_stage(sph_ctx *c, const double K[6], const double *lam, const double *Ya, const double *Yb,
+                           double wa, double wb, double *out) {
+  ensure_weights(c, K);
+  DiffOpArgs a = diff_args(c, K);
+  const unsigned G = grid_for(c->n);
+  // this stage's record goes first in the set; the deferred ones follow
+  LZSet set{};
+  set.p[0] = c->dLZ[c->lz_pend];
+  for (int q = 0; q < c->lz_pend; q++) set.p[q + 1] = c->dLZ[q];
+  hipLaunchKernelGGL(k_adj_lz, dim3(G), dim3(kBlock), 0, c->stream, (int)c->n, lam, Ya, Yb, wa, wb, a.q0,
+                     c->dLZ[c->lz_pend]);
+  if (c->lz_pend == 3) {
+    hipLaunchKernelGGL((k_diff_adj_spmv<4, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dR, c->ddW, set, c->dS,
+                       out);
+    c->lz_pend = 0;
+  } else {
+    hipLaunchKernelGGL((k_diff_adj_spmv<1, 0>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dR, c->ddW, set, c->dS,
+                       out);
+    c->lz_pend++;
+  }
+}
+static void diff_flush(sph_ctx *c, const double K[6]) {
+  if (c->lz_pend == 0) return;
+  DiffOpArgs a = diff_args(c, K);
+  const unsigned G = grid_for(c->n);
+  LZSet set{};
+  for (int q = 0; q < c->lz_pend; q++) set.p[q] = c->dLZ[q];
+  double *none = nullptr;
+  if (c->lz_pend == 1)
+    hipLaunchKernelGGL((k_diff_adj_spmv<1, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dR, c->ddW, set, c->dS, none);
+  else if (c->lz_pend == 2)
+    hipLaunchKernelGGL((k_diff_adj_spmv<2, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dR, c->ddW, set, c->dS, none);
+  else
+    hipLaunchKernelGGL((k_diff_adj_spmv<3, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dR, c->ddW, set, c->dS, none);
+  c->lz_pend = 0;
+}
 template <int DIM>
-static void launch_adj_d(sph_ctx *c, const DiffOpArgs &a, const double *lam, const double *Ya, const double *Yb,
-                         double *out) {
+static void launch_gcontract_d(sph_ctx *c, const DiffOpArgs &a) {
   unsigned G = grid_for(c->n);
   if (c->kt == KT_M4)
-    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_M4>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out, c->gpart);
+    hipLaunchKernelGGL((k_diff_gcontract<DIM, KT_M4>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dirho, c->dS, c->gpart);
   else if (c->kt == KT_M6)
-    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_M6>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out, c->gpart);
+    hipLaunchKernelGGL((k_diff_gcontract<DIM, KT_M6>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dirho, c->dS, c->gpart);
   else
-    hipLaunchKernelGGL((k_diff_adjoint<DIM, KT_GAUSS>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, Yb, out,
+    hipLaunchKernelGGL((k_diff_gcontract<DIM, KT_GAUSS>), dim3(G), dim3(kBlock), 0, c->stream, a, c->dirho, c->dS,
                        c->gpart);
 }
-// out = A^T lam; gA += lam^T dA/dK Ya, gB += lam^T dA/dK Yb (scaled by wa, wb)
-static int diff_adjoint(sph_ctx *c, const double K[6], const double *lam, const double *Ya, const double *Yb,
-                        double *out, double wa, double wb, double gK[6]) {
+// gk[0..5] += sum over pairs of the accumulated coefficients times G (fixed
+// reduction order), then the coefficients are cleared.
+static int diff_gcontract(sph_ctx *c, const double K[6], double *gk) {
+  diff_flush(c, K);
   DiffOpArgs a = diff_args(c, K);
-  if (c->diff_phys.dim == 1) launch_adj_d<1>(c, a, lam, Ya, Yb, out);
-  else if (c->diff_phys.dim == 2) launch_adj_d<2>(c, a, lam, Ya, Yb, out);
-  else launch_adj_d<3>(c, a, lam, Ya, Yb, out);
-  unsigned G = grid_for(c->n);
-  std::vector<double> h((size_t)G * 12);
-  HIPCHK(hipMemcpyAsync(h.data(), c->gpart, sizeof(double) * 12 * G, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  double sa[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0};
-  for (unsigned b = 0; b < G; b++)
-    for (int k = 0; k < 6; k++) { sa[k] += h[(size_t)b * 12 + k]; sb[k] += h[(size_t)b * 12 + 6 + k]; }
-  for (int k = 0; k < 6; k++) gK[k] += wa * sa[k] + wb * sb[k];
+  if (c->diff_phys.dim == 1) launch_gcontract_d<1>(c, a);
+  else if (c->diff_phys.dim == 2) launch_gcontract_d<2>(c, a);
+  else launch_gcontract_d<3>(c, a);
+  hipLaunchKernelGGL(k_gk_accum, dim3(1), dim3(256), 0, c->stream, c->gpart, (int)grid_for(c->n), 1.0, 0.0, gk);
+  HIPCHK(hipMemsetAsync(c->dS, 0, sizeof(double) * (size_t)c->npad * c->rcap, c->stream));
   return SPH_OK;
 }
 
@@ -1193,18 +1274,18 @@ static void sts_forward_step(sph_ctx *c, const double K[6], int s, double dt, do
   }
 }
 
-// Adjoint of one super-step: lam[s] given (in L[s]); returns dL/dY0 in L[0].
+// Adjoint of one super-step: lam[s] given (in L[s]); returns dL/dY0 in L[0];
+// the tensor-gradient coefficients accumulate per pair (dS).
 static int sts_backward_step(sph_ctx *c, const double K[6], int s, double dt, double *const *Y, double *const *L,
-                             double *U, double gK[6]) {
+                             double *U) {
   for (int j = s; j >= 2; j--) {
     double mh = sts_mu_hat(j, s) * dt, gh = sts_gm_hat(j, s) * dt, mu = sts_mu(j), nu = sts_nu(j);
-    RCHK(diff_adjoint(c, K, L[j], Y[j - 1], Y[0], U, mh, gh, gK));
-    lincomb(c, L[j - 1], 1.0, L[j - 1], mu, L[j], mh, U);
-    lincomb(c, L[j - 2], 1.0, L[j - 2], nu, L[j]);
-    lincomb(c, L[0], 1.0, L[0], 1. - mu - nu, L[j], gh, U);
+    diff_adj_stage(c, K, L[j], Y[j - 1], Y[0], mh, gh, U);
+    hipLaunchKernelGGL(k_stage_adj, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, (int)c->n, L[j], U, mu, nu,
+                       mh, gh, L[j - 1], L[j - 2], L[0]);
   }
   double m1 = sts_mu_hat(1, s) * dt;
-  RCHK(diff_adjoint(c, K, L[1], Y[0], Y[0], U, m1, 0.0, gK));
+  diff_adj_stage(c, K, L[1], Y[0], nullptr, m1, 0.0, U);
   lincomb(c, L[0], 1.0, L[0], 1.0, L[1], m1, U);
   return SPH_OK;
 }
@@ -1221,28 +1302,87 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
   c->diff_phys = P;
   const int n = (int)c->n;
   unsigned G = grid_for(n);
+  const int64_t npairs = c->stats.n_pairs;
+  if (npairs >= ((int64_t)1 << 31)) return SPH_ERR_UNSUPPORTED;  // 32-bit pair counts
   RCHK(dalloc(&c->roff, c->npad + 1));
   RCHK(dalloc(&c->rtmp, c->npad + 1));
-  RCHK(dalloc(&c->rent, (size_t)std::max<int64_t>(c->stats.n_pairs, 1)));
-  HIPCHK(hipMemsetAsync(c->rtmp, 0, sizeof(int) * (n + 1), c->stream));
-  hipLaunchKernelGGL(k_rev_count, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
-                     c->rtmp);
-  size_t bytes = 0;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->rtmp, c->roff, n + 1, c->stream));
-  if (bytes > c->scan_tmp_bytes) {
-    RCHK(dev_alloc(&c->scan_tmp, bytes));
-    c->scan_tmp_bytes = bytes;
-  }
-  bytes = c->scan_tmp_bytes;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, bytes, c->rtmp, c->roff, n + 1, c->stream));
-  HIPCHK(hipMemsetAsync(c->rtmp, 0, sizeof(int) * (n + 1), c->stream));
-  hipLaunchKernelGGL(k_rev_fill, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
-                     c->roff, c->rtmp, c->rent);
-  hipLaunchKernelGGL(k_rev_sort, dim3(G), dim3(kBlock), 0, c->stream, c->roff, n, c->rent);
+  RCHK(dalloc(&c->rcnt, c->npad));
+  RCHK(dalloc(&c->dpxh, c->npad));
+  RCHK(dalloc(&c->dirho, c->npad));
+  RCHK(dalloc(&c->dW, (size_t)c->npad * c->nb_cap));
+  RCHK(dalloc(&c->ddW, c->npad));
+  RCHK(dalloc(&c->dZ, c->npad));
+  for (int q = 0; q < 4; q++) RCHK(dalloc(&c->dLZ[q], c->npad));
+  c->lz_pend = 0;
+  RCHK(dalloc(&c->gkdev, 12));
+  c->w_valid = false;
+  hipLaunchKernelGGL(k_diff_pack, dim3(G), dim3(kBlock), 0, c->stream, n, c->pm, (const double4 *)c->q[0], c->dpxh,
+                     c->dirho);
+  // reverse lists: stable radix sort of the owner-major pairs on the neighbour
+  // index, then into the list layout (capacity = longest reverse list)
+  int *foff = nullptr;
+  uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *flat = nullptr;
+  auto release = [&]() {
+    void *t[] = {foff, keys, keys2, vals, flat};
+    for (void *q : t)
+      if (q) (void)hipFree(q);
+  };
+  auto tmp_bytes = [&](size_t need) -> int {
+    if (need > c->scan_tmp_bytes) {
+      RCHK(dev_alloc(&c->scan_tmp, need));
+      c->scan_tmp_bytes = need;
+    }
+    return SPH_OK;
+  };
+  int rc = SPH_OK;
+  do {
+    const size_t np = (size_t)std::max<int64_t>(npairs, 1);
+    if ((rc = dalloc(&foff, c->npad + 1)) || (rc = dalloc(&keys, np)) || (rc = dalloc(&keys2, np)) ||
+        (rc = dalloc(&vals, np)) || (rc = dalloc(&flat, np)))
+      break;
+    // reverse segment offsets: how often each particle is someone's neighbour
+    HIPCHK(hipMemsetAsync(c->rtmp, 0, sizeof(int) * (n + 1), c->stream));
+    hipLaunchKernelGGL(k_rev_count, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
+                       c->rtmp);
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, c->rtmp, c->roff, n + 1, c->stream));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->cnt, foff, n, c->stream));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, keys, keys2, vals, flat, (int)npairs, 0, 32, c->stream));
+    if ((rc = tmp_bytes(std::max(b1, std::max(b2, b3))))) break;
+    size_t bytes = c->scan_tmp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, bytes, c->rtmp, c->roff, n + 1, c->stream));
+    // the longest reverse list sets the capacity of the list-layout arrays
+    std::vector<int> hoff(n + 1);
+    HIPCHK(hipMemcpyAsync(hoff.data(), c->roff, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rmax = 0;
+    for (int k = 0; k < n; k++) rmax = std::max(rmax, hoff[k + 1] - hoff[k]);
+    c->rcap = std::max(8, (rmax + 7) / 8 * 8);
+    const size_t rslots = (size_t)c->npad * c->rcap;
+    if ((rc = dalloc(&c->rent, rslots)) || (rc = dalloc(&c->dR, rslots)) || (rc = dalloc(&c->dS, rslots))) break;
+    HIPCHK(hipMemsetAsync(c->dS, 0, sizeof(double) * rslots, c->stream));
+    // owner-major enumeration of the pairs, stable sort on the neighbour index
+    bytes = c->scan_tmp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, bytes, c->cnt, foff, n, c->stream));
+    hipLaunchKernelGGL(k_pair_enum, dim3(G), dim3(kBlock), 0, c->stream, c->real, c->nb, c->cnt, n, (int)c->npad,
+                       foff, keys, vals);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) bits++;
+    bytes = c->scan_tmp_bytes;
+    if (npairs > 0)
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->scan_tmp, bytes, keys, keys2, vals, flat, (int)npairs, 0, bits,
+                                                c->stream));
+    hipLaunchKernelGGL(k_rev_chunk, dim3(G), dim3(kBlock), 0, c->stream, c->roff, flat, n, (int)c->npad, c->rcnt,
+                       c->rent);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } while (false);
+  release();
+  RCHK(rc);
   if (c->gpart) (void)hipFree(c->gpart);
   c->gpart = nullptr;
   RCHK(dalloc(&c->gpart, (size_t)G * 12));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->fw_tail = false;
   c->diff_ready = true;
   return SPH_OK;
 }
@@ -1271,12 +1411,16 @@ int sph_diff_apply_adjoint(sph_ctx *c, const double K[6], const double *d_lam, c
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_lam, o, n, dv(c, 0));
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Ya, o, n, dv(c, 1));
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Yb, o, n, dv(c, 2));
-  double ga[6] = {0, 0, 0, 0, 0, 0}, gb[6] = {0, 0, 0, 0, 0, 0};
-  RCHK(diff_adjoint(c, K, dv(c, 0), dv(c, 1), dv(c, 2), dv(c, 3), 1.0, 0.0, ga));
-  RCHK(diff_adjoint(c, K, dv(c, 0), dv(c, 1), dv(c, 2), dv(c, 3), 0.0, 1.0, gb));
+  HIPCHK(hipMemsetAsync(c->gkdev, 0, sizeof(double) * 12, c->stream));
+  diff_adj_stage(c, K, dv(c, 0), dv(c, 1), nullptr, 1.0, 0.0, dv(c, 3));
+  RCHK(diff_gcontract(c, K, c->gkdev));
+  diff_adj_stage(c, K, dv(c, 0), dv(c, 2), nullptr, 1.0, 0.0, dv(c, 3));
+  RCHK(diff_gcontract(c, K, c->gkdev + 6));
   hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, dv(c, 3), o, n, d_out);
+  double g12[12];
+  HIPCHK(hipMemcpyAsync(g12, c->gkdev, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 6; k++) { gA[k] = ga[k]; gB[k] = gb[k]; }
+  for (int k = 0; k < 6; k++) { gA[k] = g12[k]; gB[k] = g12[6 + k]; }
   return SPH_OK;
 }
 
@@ -1294,20 +1438,34 @@ int sph_diff_forward(sph_ctx *c, const double K[6], int s, double dt, int nsteps
     c->ckpt_cap = (int64_t)nck * c->npad;
   }
   RCHK(ensure_dvec(c, 2 * s + 5 + every * (s + 1)));
-  std::vector<double *> Y(s + 1);
+  std::vector<double *> Y(s + 1), Yk(s + 1);
   for (int j = 0; j <= s; j++) Y[j] = dv(c, j);
+  // the last checkpoint segment runs in the backward pass's segment buffers,
+  // so the backward pass need not recompute it
+  const int tail0 = ((nsteps + every - 1) / every - 1) * every, base = 2 * s + 5;
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_T0, c->orig[c->cur], n, Y[0]);
   for (int st = 0; st < nsteps; st++) {
     if (st % every == 0)
       HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(st / every) * c->npad, Y[0], sizeof(double) * n,
                             hipMemcpyDeviceToDevice, c->stream));
-    sts_forward_step(c, K, s, dt, Y.data(), dv(c, s + 1), dv(c, s + 2));
-    HIPCHK(hipMemcpyAsync(Y[0], Y[s], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    if (st < tail0) {
+      sts_forward_step(c, K, s, dt, Y.data(), dv(c, s + 1), dv(c, s + 2));
+      HIPCHK(hipMemcpyAsync(Y[0], Y[s], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+      continue;
+    }
+    const int k = st - tail0;
+    for (int j = 0; j <= s; j++) Yk[j] = dv(c, base + k * (s + 1) + j);
+    HIPCHK(hipMemcpyAsync(Yk[0], k == 0 ? Y[0] : dv(c, base + (k - 1) * (s + 1) + s), sizeof(double) * n,
+                          hipMemcpyDeviceToDevice, c->stream));
+    sts_forward_step(c, K, s, dt, Yk.data(), dv(c, s + 1), dv(c, s + 2));
+    if (st == nsteps - 1)
+      HIPCHK(hipMemcpyAsync(Y[0], Yk[s], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   }
   hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, Y[0], c->orig[c->cur], n, d_T);
   HIPCHK(hipStreamSynchronize(c->stream));
   c->fw_s = s; c->fw_steps = nsteps; c->fw_every = every; c->fw_dt = dt;
   for (int k = 0; k < 6; k++) c->fw_K[k] = K[k];
+  c->fw_tail = true;
   return SPH_OK;
 }
 
@@ -1323,16 +1481,20 @@ int sph_diff_backward(sph_ctx *c, const double *d_gT, double *d_gT0, double gK[6
   double *U = dv(c, s + 3);
   const int base = 2 * s + 5;
   auto seg = [&](int k, int j) { return dv(c, base + k * (s + 1) + j); };
-  for (int k = 0; k < 6; k++) gK[k] = 0.0;
+  HIPCHK(hipMemsetAsync(c->dS, 0, sizeof(double) * (size_t)c->npad * c->rcap, c->stream));
+  HIPCHK(hipMemsetAsync(c->gkdev, 0, sizeof(double) * 12, c->stream));
   double *lam = dv(c, s + 1);  // running dL/dT (sorted)
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_gT, c->orig[c->cur], n, lam);
   int nseg = (nsteps + every - 1) / every;
   for (int sg = nseg - 1; sg >= 0; sg--) {
     int st0 = sg * every, st1 = std::min(nsteps, st0 + every);
-    // recompute the segment's stage states from its checkpoint
-    HIPCHK(hipMemcpyAsync(seg(0, 0), c->ckpt + (size_t)sg * c->npad, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                          c->stream));
-    for (int st = st0; st < st1; st++) {
+    // recompute the segment's stage states from its checkpoint (the last
+    // segment's are still there from the forward pass)
+    const bool have = sg == nseg - 1 && c->fw_tail;
+    if (!have)
+      HIPCHK(hipMemcpyAsync(seg(0, 0), c->ckpt + (size_t)sg * c->npad, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                            c->stream));
+    for (int st = st0; st < st1 && !have; st++) {
       int k = st - st0;
       std::vector<double *> Y(s + 1);
       for (int j = 0; j <= s; j++) Y[j] = seg(k, j);
@@ -1346,11 +1508,14 @@ int sph_diff_backward(sph_ctx *c, const double *d_gT, double *d_gT0, double gK[6
       for (int j = 0; j <= s; j++) Y[j] = seg(k, j);
       HIPCHK(hipMemsetAsync(L[0], 0, sizeof(double) * (size_t)c->npad * s, c->stream));
       HIPCHK(hipMemcpyAsync(L[s], lam, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-      RCHK(sts_backward_step(c, K, s, dt, Y.data(), L.data(), U, gK));
+      RCHK(sts_backward_step(c, K, s, dt, Y.data(), L.data(), U));
       HIPCHK(hipMemcpyAsync(lam, L[0], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     }
   }
   hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, lam, c->orig[c->cur], n, d_gT0);
+  RCHK(diff_gcontract(c, K, c->gkdev));
+  c->fw_tail = false;  // segment buffers now hold the first segment
+  HIPCHK(hipMemcpyAsync(gK, c->gkdev, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return SPH_OK;
 }

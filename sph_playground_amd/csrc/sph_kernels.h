@@ -60,6 +60,32 @@ struct NeighArgs {
   long long *stripes;      // [kStripes] pair counts, [kStripes..2k) max counts
 };
 
+// Neighbour iteration with batched gathers: entries come 8 per coalesced
+// chunk; for every NB of them `load(entry)` issues all gathers first (slots
+// past the list end reload particle s, always a valid address), then
+// `body(data)` runs on the valid ones.  NB gathers are then in flight at once.
+template <int NB, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbours(const uint32_t *nb, int s, int npad, int nn, Load load, Body body) {
+  const uint32_t self = nb_pack((uint32_t)s, kNoImage);
+  for (int k0 = 0; k0 < nn; k0 += 8) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(nb + nb_slot(k0, s, npad));
+    const uint4 lo = p[0], hi = p[1];
+    const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    const int m = min(8, nn - k0);
+#pragma unroll
+    for (int h = 0; h < 8; h += NB) {
+      if (h < m) {
+        decltype(load(0u)) jd[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) jd[u] = load(h + u < m ? e8[h + u] : self);
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+          if (h + u < m) body(jd[u]);
+      }
+    }
+  }
+}
+
 template <typename V4>
 struct Loop1Args {
   const double4 *pm;
@@ -104,6 +130,11 @@ struct Loop2Args {
   int n, npad;
 };
 
+// Up to four adjoint stages' (lam, Zt) records (sph_adjoint.hip).
+struct LZSet {
+  const double2 *p[4];
+};
+
 // The differentiable diffusion operator (sph_adjoint.hip).
 struct DiffOpArgs {
   const double4 *pm;   // x, y, z, m (sorted)
@@ -111,8 +142,9 @@ struct DiffOpArgs {
   const uint8_t *real;
   const uint32_t *nb;
   const int *cnt;
-  const int *roff;     // reverse lists (CSR over sorted index)
-  const uint32_t *rent;
+  const int *rcnt;     // reverse-list lengths
+  const uint32_t *rent;  // reverse entries, nb_slot layout: owner a and the image of b in a's list
+  const double4 *pxh;  // x, y, z, 1 / h (sorted)
   Box b;
   double K[6];         // xx, yy, zz, xy, xz, yz
   double wcv, fwc;

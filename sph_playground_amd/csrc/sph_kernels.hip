@@ -420,33 +420,6 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
   if (threadIdx.x == 0 && mx) atomicMax(&a.stripes[kStripes + stripe()], mx);
 }
 
-// Neighbour iteration with batched gathers: entries come 8 per coalesced
-// chunk; for every NB of them `load(entry)` issues all gathers first (slots
-// past the list end reload particle s, always a valid address), then
-// `body(data)` runs on the valid ones.  NB gathers are then in flight at once.
-template <int NB, typename Load, typename Body>
-__device__ __forceinline__ void for_neighbours(const uint32_t *nb, int s, int npad, int nn, Load load, Body body) {
-  const uint32_t self = nb_pack((uint32_t)s, kNoImage);
-  for (int k0 = 0; k0 < nn; k0 += 8) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(nb + nb_slot(k0, s, npad));
-    const uint4 lo = p[0], hi = p[1];
-    const uint32_t e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-    const int m = min(8, nn - k0);
-#pragma unroll
-    for (int h = 0; h < 8; h += NB) {
-      if (h < m) {
-        decltype(load(0u)) jd[NB];
-#pragma unroll
-        for (int u = 0; u < NB; u++) jd[u] = load(h + u < m ? e8[h + u] : self);
-#pragma unroll
-        for (int u = 0; u < NB; u++)
-          if (h + u < m) body(jd[u]);
-      }
-    }
-  }
-}
-
-
 // ---------------------------------------------------------------- loop 1
 // One Jacobi sweep of c1advanced (circuit1.f90:114-211) for the particles
 // still active (eps > tol); frozen ones carry their state to the next buffer.
