@@ -32,6 +32,10 @@ FLOP_PER_PAIR_LOOP1_GRAD = 50     # loop 1 sweep with grad T
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector and dense matrix)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the 128^3 workload's kernels, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the counters read
+# for a plain 1 GiB copy (x2 and x1): profiles/hbm_traffic_diffusion3d_128cube.md
+MEASURED_TRAFFIC_128 = {"neigh": 3.508e9, "loop1_mixed": 1.446e9, "loop2_mixed": 1.691e9}
 
 
 def log(msg):
@@ -162,13 +166,17 @@ def main():
     k64 = {k: avg(per64, k) for k in ("ms_k_neigh", "ms_k_loop1", "ms_k_loop2", "ms_total")}
     kmx = {k: avg(permx, k) for k in ("ms_k_neigh", "ms_k_loop1", "ms_k_loop2", "ms_total")}
     sweeps = avg(permx, "sweeps")
+
+    def traffic(key):  # measured HBM bytes per launch (only for the default 128^3 workload)
+        return MEASURED_TRAFFIC_128[key] if args.size == 128 else None
+
     # dominant kernel of the headline (mixed) path
     dom = max(("ms_k_neigh", "ms_k_loop1", "ms_k_loop2"), key=lambda k: kmx[k])
     if dom == "ms_k_loop2":
         flops = pairs * FLOP_PER_PAIR_LOOP2_FAB
         ach = flops / (kmx[dom] * 1e-3) / 1e12
         roof = dict(kernel="k_loop2<3,M6,float> (loop 2, fab, full tensor)", bound="valu", achieved=ach,
-                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=None,
+                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=traffic("loop2_mixed"),
                     flop_per_pair=FLOP_PER_PAIR_LOOP2_FAB, pairs_per_launch=pairs, ms_per_launch=kmx[dom])
     elif dom == "ms_k_loop1":
         p1 = avg(permx, "loop1_pairs") / max(sweeps, 1)
@@ -176,14 +184,14 @@ def main():
         ms = kmx[dom] / max(sweeps, 1)
         ach = flops / (ms * 1e-3) / 1e12
         roof = dict(kernel="k_loop1<3,M6,grad,float> (one Newton sweep)", bound="valu", achieved=ach,
-                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=None,
+                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=traffic("loop1_mixed"),
                     flop_per_pair=FLOP_PER_PAIR_LOOP1_GRAD, pairs_per_launch=p1, ms_per_launch=ms)
     else:
         # neighbour build: algorithmic bytes = read x,y,z,m,h (40 B) + write the list (4 B/pair) + count
         byts = n * 44 + pairs * 4
         ach = byts / (kmx[dom] * 1e-3) / 1e9
-        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm", achieved=ach,
-                    peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=None,
+        roof = dict(kernel="k_neigh<3,1> (cell-list neighbour build)", bound="hbm", achieved=ach,
+                    peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=traffic("neigh"),
                     bytes_per_launch=byts, ms_per_launch=kmx[dom])
 
     adj = None
