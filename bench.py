@@ -100,6 +100,93 @@ def adjoint_cost(size=64, s=5, nsteps=20, every=5, reps=3):
             "forward_ms": f * 1e3, "backward_ms": b * 1e3, "adjoint_over_forward": b / f}
 
 
+def batched_cost(nprob=256, s=10, nsteps=5, reps=3):
+    """The batched 2D anisotropic-diffusion study: nprob problems with N
+    log-uniform in [32^2, 512^2] on periodic square lattices, field angle
+    U[0, pi), kappa_perp / kappa_par in {1, 0.1, 0.01, 0.001}, sine T, M6, fab;
+    nsteps RKL2 super-steps of s stages over all problems (one launch per
+    stage, a replayed hipGraph per super-step), against the same super-steps
+    run problem by problem (sph_diff_forward)."""
+    import ctypes as ct
+    import numpy as np
+    import torch
+    from sph_playground_amd import autograd as AG
+    from sph_playground_amd import consts as C
+    from sph_playground_amd import problems as P
+    from sph_playground_amd._lib import check, lib
+    from sph_playground_amd.batch import Batch
+    from sph_playground_amd.engine import Engine
+    specs = P.batch2d_specs(nprob)
+    engines, Ks, T0s, dts = [], [], [], []
+    t0 = time.perf_counter()
+    for k, sp in enumerate(specs):
+        state, st, K = P.batch2d_problem(sp)
+        eng = Engine(state, kernel="m6", tensor="full", kappa_perp=sp["ratio"])
+        eng.load(st)
+        AG.prepare(eng)
+        engines.append(eng)
+        Ks.append(K)
+        T0s.append(st[:, C.es_t - 1].copy())
+        dx = 1.0 / sp["n_side"]
+        dts.append(0.05 * dx * dx * (s * s + s - 2) / 4.0)
+        if (k + 1) % 64 == 0:
+            log("batch: %d problems prepared" % (k + 1))
+    t_prep = time.perf_counter() - t0
+    batch = Batch(engines, np.stack(Ks))
+    info = batch.info()
+    T0 = torch.from_numpy(np.concatenate(T0s)).cuda()
+    out = torch.empty_like(T0)
+    res = {}
+    for mode, graph in (("graph", True), ("direct", False)):
+        batch.sts(T0, s, dts, nsteps, graph=graph, out=out)  # warm-up (captures the graph)
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            batch.sts(T0, s, dts, nsteps, graph=graph, out=out)
+            ts.append(time.perf_counter() - a)
+        res[mode] = float(np.median(ts))
+    # the same super-steps problem by problem
+    T0p = [torch.from_numpy(x).cuda() for x in T0s]
+    outp = [torch.empty_like(x) for x in T0p]
+    Kh = [np.ascontiguousarray(K) for K in Ks]
+
+    def per_problem():
+        for p in range(nprob):
+            check(lib().sph_diff_forward(engines[p]._h, Kh[p].ctypes.data_as(ct.POINTER(ct.c_double)), s, dts[p],
+                                         nsteps, nsteps, ct.c_void_p(T0p[p].data_ptr()),
+                                         ct.c_void_p(outp[p].data_ptr())), "sph_diff_forward")
+    per_problem()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        per_problem()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - a)
+    t_single = float(np.median(ts))
+    same = all(torch.equal(x, y) for x, y in zip(batch.split(out), outp))
+    P_ = info["n_pairs"]
+    N_ = info["n_total"]
+    inter = P_ * s * nsteps
+    stage_ms = res["graph"] * 1e3 / (s * nsteps)
+    byts = 12 * P_ + 80 * N_
+    batch.close()
+    for e in engines:
+        e.close()
+    return {"workload": "%d 2D anisotropic-diffusion problems, N log-uniform in [32^2, 512^2] (total %d), "
+                        "theta U[0,pi), kappa_perp/kappa_par in {1,0.1,0.01,0.001}, M6, fab, FP64, "
+                        "%d super-steps x %d stages" % (nprob, N_, nsteps, s),
+            "n_total": N_, "pairs_per_stage": P_, "pair_interactions_per_s": inter / res["graph"],
+            "ms_graph": res["graph"] * 1e3, "ms_direct": res["direct"] * 1e3,
+            "ms_problem_by_problem": t_single * 1e3, "speedup_vs_problem_by_problem": t_single / res["graph"],
+            "bitwise_equal_to_problem_by_problem": same,
+            "stage_ms": stage_ms, "stage_algorithmic_bytes": byts,
+            "stage_hbm_gbs": byts / (stage_ms * 1e-3) / 1e9,
+            "autotune_block": info["block"], "autotune_ms_per_4_stages": info["tune"],
+            "prepare_s": t_prep}
+
+
 def cpu_baseline(size):
     """Reference Fortran hot path (oracle/_ref, built from the reference
     sources) on a bounded sample of the same workload, all host cores."""
@@ -132,6 +219,7 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the step runs on one GPU; runs on more than one GPU are not supported "
@@ -199,6 +287,11 @@ def main():
         log("adjoint cost")
         adj = adjoint_cost()
 
+    bat = None
+    if not args.no_batch:
+        log("batched problems")
+        bat = batched_cost()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -228,6 +321,7 @@ def main():
                   "ddt_max_err_rel_to_max": ddt_err},
         "mixed_vs_fp64": vmx / v64,
         "adjoint": adj,
+        "batched": bat,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -146,6 +146,38 @@ int sph_diff_forward(sph_ctx *ctx, const double K[6], int s, double dt, int nste
  * (recomputing each checkpoint segment). */
 int sph_diff_backward(sph_ctx *ctx, const double *d_gT, double *d_gT0, double gK[6]);
 
+/* ---- Batched problems (no reference counterpart: the reference runs one
+ * ./execute per problem, e.g. 000_anisopaper_v1_gls_kxx.py:44-118).  Many
+ * problems, each prepared with sph_diff_prepare on its own context, merged
+ * into one operator; every RKL2 stage of every problem is one kernel launch,
+ * a super-step is one replayed hipGraph.  The batch copies what it needs: the
+ * contexts may be destroyed afterwards. */
+typedef struct sph_batch sph_batch;
+typedef struct {
+  int n_problems;
+  int64_t n_total;    /* particles over all problems */
+  int64_t n_pairs;    /* pair interactions of one operator application */
+  int list_cap;       /* merged neighbour-list capacity */
+  int block;          /* stage-kernel block size picked by the autotuner */
+  int tune_block[4];  /* candidates ... */
+  float tune_ms[4];   /* ... and their time for a 4-stage super-step */
+} sph_batch_info;
+/* Merge ctxs[0..nprob) (same device, sph_diff_prepare done) with tensors
+ * K[6 p .. 6 p + 5] (xx, yy, zz, xy, xz, yz); autotunes the stage launch. */
+int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch **out);
+int sph_batch_destroy(sph_batch *batch);
+/* nsteps RKL2 super-steps of s stages (sts_integrator.f90:87-124), problem p
+ * with time step dt[p] (host array).  d_T0 / d_T: device vectors of n_total
+ * doubles, problem p's particles (original order) at offset off[p]
+ * (sph_batch_offsets).  use_graph: replay one captured super-step per step
+ * (1) or launch the stages directly (0); both give the same bits, which are
+ * each problem's sph_diff_forward bits. */
+int sph_batch_sts(sph_batch *batch, int s, const double *dt, int nsteps, const double *d_T0, double *d_T,
+                  int use_graph);
+int sph_batch_get_info(sph_batch *batch, sph_batch_info *info);
+/* off[0..n_problems]: row offsets of the problems in the batch vectors. */
+int sph_batch_offsets(sph_batch *batch, int64_t *off);
+
 int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
 const char *sph_error_string(int code);
 

@@ -13,7 +13,8 @@ EXPORTS = (
     "sph_default_options", "sph_ctx_create", "sph_ctx_destroy", "sph_iterate", "sph_iterate_device",
     "sph_load", "sph_save", "sph_step", "sph_neighbours", "sph_leapfrog", "sph_sts", "sph_sts_coeffs",
     "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
-    "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange",
+    "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange", "sph_batch_create", "sph_batch_destroy",
+    "sph_batch_sts", "sph_batch_get_info", "sph_batch_offsets",
 )
 
 SPH_OK = 0
@@ -35,6 +36,12 @@ class Stats(ct.Structure):
                 ("energy", ct.c_double), ("ms_neigh", ct.c_double), ("ms_loop1", ct.c_double),
                 ("ms_loop2", ct.c_double), ("ms_total", ct.c_double), ("ms_k_neigh", ct.c_double),
                 ("ms_k_loop1", ct.c_double), ("ms_k_loop2", ct.c_double)]
+
+
+class BatchInfo(ct.Structure):
+    _fields_ = [("n_problems", ct.c_int), ("n_total", ct.c_int64), ("n_pairs", ct.c_int64),
+                ("list_cap", ct.c_int), ("block", ct.c_int), ("tune_block", ct.c_int * 4),
+                ("tune_ms", ct.c_float * 4)]
 
 
 class SphError(RuntimeError):
@@ -76,6 +83,11 @@ def lib():
     L.sph_sts_coeffs.argtypes = [ct.c_int, ct.c_int, D]
     L.sph_get_stats.argtypes = [P, ct.POINTER(Stats)]
     L.sph_diff_prepare.argtypes = [P, D]
+    L.sph_batch_create.argtypes = [ct.POINTER(P), ct.c_int, D, ct.POINTER(P)]
+    L.sph_batch_destroy.argtypes = [P]
+    L.sph_batch_sts.argtypes = [P, ct.c_int, D, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]
+    L.sph_batch_get_info.argtypes = [P, ct.POINTER(BatchInfo)]
+    L.sph_batch_offsets.argtypes = [P, ct.POINTER(ct.c_int64)]
     L.sph_diff_apply.argtypes = [P, D, ct.c_void_p, ct.c_void_p]
     L.sph_diff_apply_adjoint.argtypes = [P, D, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, D, D]
     L.sph_diff_forward.argtypes = [P, D, ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_diff_spmv(DiffOpArgs a, const double *W
     for (int u = 0; u < 8; u++) z8[u] = Z[u < m ? nb_index(e8[u]) : (uint32_t)s];
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (u < m) acc += w8[u] * (z8[u] - za);
+      if (u < m) acc = fma(w8[u], z8[u] - za, acc);
   }
   out[s] = acc / a.q0[s].y;
 }
@@ -368,6 +368,96 @@ __global__ void k_rev_chunk(const int *roff, const uint32_t *flat, int n, int np
   for (int k = 0; k < e - b; k++) rent[nb_slot(k, s, npad)] = flat[b + k];
 }
 
+// ------------------------------------------------------- batched problems
+// Many prepared problems merged into one index space: problem p's row s is
+// off_p + s, its list entries point to off_p + j and its pair weights are
+// copied from its own list layout into the merged one (npad_g rows).  Every
+// RKL2 stage of every problem is then one launch of k_batch_stage.
+__global__ void k_batch_merge(int n, int npad, int64_t off, int pid, const int *cnt, const uint32_t *nb,
+                              const double *W, const double4 *q0, const int64_t *orig, int npad_g, int *mcnt,
+                              uint32_t *mnb, double *mW, double *mrho, int64_t *morig, int *mpid) {
+  int s = gtid_a();
+  if (s >= n) return;
+  const int64_t g = off + s;
+  const int c = cnt[s];
+  mcnt[g] = c;
+  mrho[g] = q0[s].y;
+  morig[g] = off + orig[s];
+  mpid[g] = pid;
+  for (int k = 0; k < c; k++) {
+    const size_t src = nb_slot(k, s, npad), dst = nb_slot(k, (int)g, npad_g);
+    mnb[dst] = (uint32_t)(off + nb_index(nb[src]));
+    mW[dst] = W[src];
+  }
+}
+
+// A product that is never fused into a following add (Z = Y rho is rounded
+// before the difference, as k_zrho stores it).
+__device__ __forceinline__ double mul_rn(double x, double y) {
+#pragma clang fp contract(off)
+  return x * y;
+}
+
+// Batch input: Y_0 in merged order and its Z_0 = Y_0 rho.
+__global__ void k_batch_in(const double *T0, const int64_t *orig, const double *rho, int n, double *Y0,
+                           double *Z0) {
+  int s = gtid_a();
+  if (s >= n) return;
+  const double y = T0[orig[s]];
+  Y0[s] = y;
+  Z0[s] = mul_rn(y, rho[s]);
+}
+
+// One RKL2 stage j over all problems (sts_integrator.f90:105-118), fused:
+//   D   = (A Y_{j-1})_s = (1 / rho_s) sum_b W_sb (Z_b - Z_s),  Z = Y_{j-1} rho
+//   j=1: DY0 = D;  Y_1 = Y_0 + mhat dt DY0
+//   j>1: Y_j = mu Y_{j-1} + nu Y_{j-2} + (1 - mu - nu) Y_0 + mhat dt D + ghat dt DY0
+// and Z_j = Y_j rho for the next stage's gathers.  The per-row summation
+// is k_diff_spmv's and the combination k_lincomb's (a rounded first product,
+// then explicit fused multiply-adds in the same order), so every problem's
+// result is bitwise the unbatched super-step's.  Y_j may be Y_0's buffer:
+// row s reads only its own Y_0 entry before writing it.
+template <bool FIRST>
+__global__ __launch_bounds__(512) void k_batch_stage(BatchStage a) {
+  int s = gtid_a();
+  if (s >= a.n) return;
+  const double ya = a.Yjm1[s], rs = a.rho[s];
+  const double za = mul_rn(ya, rs);
+  const int nn = a.cnt[s];
+  double acc = 0.0;
+  for (int k0 = 0; k0 < nn; k0 += 8) {
+    const size_t base = nb_slot(k0, s, a.npad);
+    const uint4 *pe = reinterpret_cast<const uint4 *>(a.nb + base);
+    const double4 *pw = reinterpret_cast<const double4 *>(a.W + base);
+    const uint4 e0 = pe[0], e1 = pe[1];
+    const double4 w0 = pw[0], w1 = pw[1];
+    const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+    const double w8[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    const int m = min(8, nn - k0);
+    double z8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) z8[u] = a.Zjm1[u < m ? e8[u] : (uint32_t)s];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (u < m) acc = fma(w8[u], z8[u] - za, acc);
+  }
+  const double D = acc / rs;
+  const double h = a.dt[a.pid[s]];
+  double v;
+  if (FIRST) {
+    a.DY0[s] = D;
+    v = fma(a.mhat * h, D, ya);
+  } else {
+    v = a.mu * ya;
+    v = fma(a.nu, a.Yjm2[s], v);
+    v = fma(a.c0, a.Y0[s], v);
+    v = fma(a.mhat * h, D, v);
+    v = fma(a.ghat * h, a.DY0[s], v);
+  }
+  a.Yj[s] = v;
+  a.Zj[s] = mul_rn(v, rs);
+}
+
 // Vectors in original particle order <-> sorted order.
 __global__ void k_to_sorted(const double *src, const int64_t *orig, int n, double *dst) {
   int s = gtid_a();
@@ -396,13 +486,17 @@ __global__ void k_lincomb(int n, double *y, double c0, const double *x0, double 
                           const double *x2, double c3, const double *x3, double c4, const double *x4) {
   int s = gtid_a();
   if (s >= n) return;
+  // one rounded product, then one fused multiply-add per further term
   double v = c0 * x0[s];
-  if (x1) v += c1 * x1[s];
-  if (x2) v += c2 * x2[s];
-  if (x3) v += c3 * x3[s];
-  if (x4) v += c4 * x4[s];
+  if (x1) v = fma(c1, x1[s], v);
+  if (x2) v = fma(c2, x2[s], v);
+  if (x3) v = fma(c3, x3[s], v);
+  if (x4) v = fma(c4, x4[s], v);
   y[s] = v;
 }
+
+template __global__ void k_batch_stage<true>(BatchStage);
+template __global__ void k_batch_stage<false>(BatchStage);
 
 #define SPH_INST_ADJ(DIM, KT)                                                                  \
   template __global__ void k_diff_weights<DIM, KT>(DiffOpArgs, double *, double *);                 \

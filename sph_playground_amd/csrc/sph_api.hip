@@ -81,6 +81,11 @@ __global__ void k_to_sorted(const double *, const int64_t *, int, double *);
 __global__ void k_from_sorted(const double *, const int64_t *, int, double *);
 __global__ void k_lincomb(int, double *, double, const double *, double, const double *, double, const double *,
                           double, const double *, double, const double *);
+__global__ void k_batch_merge(int, int, int64_t, int, const int *, const uint32_t *, const double *, const double4 *,
+                              const int64_t *, int, int *, uint32_t *, double *, double *, int64_t *, int *);
+template <bool FIRST>
+__global__ void k_batch_stage(BatchStage);
+__global__ void k_batch_in(const double *, const int64_t *, const double *, int, double *, double *);
 }  // namespace sph
 
 namespace {
@@ -139,6 +144,7 @@ struct sph_ctx {
   uint8_t *real = nullptr;
   uint32_t *nb = nullptr;
   int nb_cap = 0;
+  int last_maxcnt = 0;  // longest neighbour list of the last search
   int *cnt = nullptr;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *energy_part = nullptr;
@@ -521,6 +527,7 @@ static int build_neighbours(sph_ctx *c) {
     RCHK(read_ints(c, 0, 1));
     c->stats.n_pairs = stripe_sum(c->h_ll, 0);
     int maxcnt = (int)stripe_max(c->h_ll, 1);
+    c->last_maxcnt = maxcnt;
     c->stats.ms_k_neigh = elapsed(c, c->ev[5], c->ev[6]);
     int bits = c->h_int[0];
     if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
@@ -1517,6 +1524,214 @@ int sph_diff_backward(sph_ctx *c, const double *d_gT, double *d_gT0, double gK[6
   c->fw_tail = false;  // segment buffers now hold the first segment
   HIPCHK(hipMemcpyAsync(gK, c->gkdev, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- batched problems
+// Independent prepared diffusion problems merged into one operator: one
+// launch per RKL2 stage covers every particle of every problem, a super-step
+// is captured once as a hipGraph and replayed, and the stage kernel's block
+// size is picked by timing the candidates on the merged operator.
+struct sph_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nprob = 0;
+  int64_t n = 0, npad = 0, n_pairs = 0;
+  int cap = 0;
+  std::vector<int64_t> off;            // nprob + 1 offsets of the problems' rows
+  int *cnt = nullptr, *pid = nullptr;  // list lengths, problem of each row
+  uint32_t *nb = nullptr;              // merged neighbour indices (list layout)
+  double *W = nullptr, *rho = nullptr; // merged pair weights, densities
+  double *dt = nullptr;                // per-problem time steps
+  int64_t *orig = nullptr;             // merged row -> position in the concatenated vectors
+  double *vec = nullptr;               // Y0, three stage states, DY0, three Z (npad each)
+  int block = 256;
+  float tune_ms[4] = {0, 0, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_s = 0, graph_block = 0;
+};
+
+namespace {
+constexpr int kBatchBlocks[4] = {64, 128, 256, 512};
+constexpr int kBatchVecs = 8;
+
+double *bvec(sph_batch *b, int k) { return b->vec + (size_t)k * b->npad; }
+
+// The launches of one RKL2 super-step of s stages (sts_integrator.f90:87-124)
+// from Y_0 (buffer 0, its Z in buffer 5) back into buffers 0 and 5; stage
+// states rotate over buffers 1-3, their Z over 6-7.
+void batch_super_step(sph_batch *b, int s, int block) {
+  double *Y0 = bvec(b, 0), *DY0 = bvec(b, 4), *Z0 = bvec(b, 5);
+  double *R[3] = {bvec(b, 1), bvec(b, 2), bvec(b, 3)};
+  double *ZR[2] = {bvec(b, 6), bvec(b, 7)};
+  auto Y = [&](int j) -> double * { return (j == 0 || j == s) ? Y0 : R[j % 3]; };
+  auto Z = [&](int j) -> double * { return (j == 0 || j == s) ? Z0 : ZR[j & 1]; };
+  const unsigned G = (unsigned)((b->n + block - 1) / block);
+  BatchStage a{};
+  a.cnt = b->cnt; a.nb = b->nb; a.W = b->W; a.rho = b->rho; a.pid = b->pid; a.dt = b->dt;
+  a.Y0 = Y0; a.DY0 = DY0; a.n = (int)b->n; a.npad = (int)b->npad;
+  a.Yjm1 = Y0; a.Yjm2 = Y0; a.Zjm1 = Z0; a.Yj = Y(1); a.Zj = Z(1);
+  a.mhat = sts_mu_hat(1, s);
+  hipLaunchKernelGGL((k_batch_stage<true>), dim3(G), dim3(block), 0, b->stream, a);
+  for (int j = 2; j <= s; j++) {
+    a.Yjm1 = Y(j - 1); a.Yjm2 = Y(j - 2); a.Zjm1 = Z(j - 1); a.Yj = Y(j); a.Zj = Z(j);
+    a.mu = sts_mu(j); a.nu = sts_nu(j); a.c0 = 1. - a.mu - a.nu;
+    a.mhat = sts_mu_hat(j, s); a.ghat = sts_gm_hat(j, s);
+    hipLaunchKernelGGL((k_batch_stage<false>), dim3(G), dim3(block), 0, b->stream, a);
+  }
+}
+
+void batch_free(sph_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void *p[] = {b->cnt, b->pid, b->nb, b->W, b->rho, b->dt, b->orig, b->vec};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// Time one 4-stage super-step per candidate block size (on the zeroed work
+// vectors: the stage kernels' work does not depend on the values) and keep
+// the fastest.
+int batch_autotune(sph_batch *b) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int k = 0; k < 4; k++) {
+    const int blk = kBatchBlocks[k];
+    batch_super_step(b, 4, blk);  // warm-up
+    float t = 1e30f;
+    for (int r = 0; r < 3; r++) {
+      (void)hipEventRecord(e0, b->stream);
+      batch_super_step(b, 4, blk);
+      (void)hipEventRecord(e1, b->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      t = std::min(t, ms);
+    }
+    b->tune_ms[k] = t;
+    if (t < best) { best = t; b->block = blk; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(b->vec, 0, sizeof(double) * kBatchVecs * b->npad, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SPH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch **out) {
+  if (!ctxs || nprob < 1 || !K || !out) return SPH_ERR_BAD_ARG;
+  *out = nullptr;
+  int64_t n = 0, np = 0;
+  int maxc = 0;
+  for (int p = 0; p < nprob; p++) {
+    const sph_ctx *c = ctxs[p];
+    if (!c || !c->diff_ready || c->device != ctxs[0]->device) return SPH_ERR_BAD_ARG;
+    n += c->n;
+    np += c->stats.n_pairs;
+    maxc = std::max(maxc, c->last_maxcnt);
+  }
+  if (n >= ((int64_t)1 << 31) - 64) return SPH_ERR_UNSUPPORTED;  // 32-bit merged row indices
+  sph_batch *b = new sph_batch();
+  b->device = ctxs[0]->device;
+  (void)hipSetDevice(b->device);
+  b->nprob = nprob;
+  b->n = n;
+  b->n_pairs = np;
+  b->npad = (n + 63) / 64 * 64;
+  b->cap = std::max(8, (maxc + 7) / 8 * 8);
+  b->off.assign(nprob + 1, 0);
+  int rc = SPH_OK;
+  auto fail = [&](int r) { batch_free(b); return r; };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(SPH_ERR_HIP);
+  const size_t slots = (size_t)b->npad * b->cap;
+  if ((rc = dalloc(&b->cnt, b->npad)) || (rc = dalloc(&b->pid, b->npad)) || (rc = dalloc(&b->nb, slots)) ||
+      (rc = dalloc(&b->W, slots)) || (rc = dalloc(&b->rho, b->npad)) || (rc = dalloc(&b->dt, nprob)) ||
+      (rc = dalloc(&b->orig, b->npad)) || (rc = dalloc(&b->vec, kBatchVecs * (size_t)b->npad)))
+    return fail(rc);
+  if (hipMemset(b->cnt, 0, sizeof(int) * b->npad) != hipSuccess ||
+      hipMemset(b->dt, 0, sizeof(double) * nprob) != hipSuccess ||
+      hipMemset(b->vec, 0, sizeof(double) * kBatchVecs * b->npad) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return fail(SPH_ERR_HIP);
+  // each problem's weights for its tensor, then into the merged layout
+  for (int p = 0; p < nprob; p++) {
+    sph_ctx *c = ctxs[p];
+    ensure_weights(c, K + 6 * p);
+    hipLaunchKernelGGL(k_batch_merge, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, (int)c->n, (int)c->npad,
+                       b->off[p], p, c->cnt, c->nb, c->dW, (const double4 *)c->q[0], c->orig[c->cur],
+                       (int)b->npad, b->cnt, b->nb, b->W, b->rho, b->orig, b->pid);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SPH_ERR_HIP);
+    b->off[p + 1] = b->off[p] + c->n;
+  }
+  if ((rc = batch_autotune(b))) return fail(rc);
+  *out = b;
+  return SPH_OK;
+}
+
+int sph_batch_destroy(sph_batch *b) {
+  if (!b) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  return SPH_OK;
+}
+
+int sph_batch_sts(sph_batch *b, int s, const double *dt, int nsteps, const double *d_T0, double *d_T,
+                  int use_graph) {
+  if (!b || !dt || !d_T0 || !d_T || s < 2 || nsteps < 0) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(b->device);
+  const unsigned G = grid_for(b->n);
+  HIPCHK(hipMemcpyAsync(b->dt, dt, sizeof(double) * b->nprob, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_batch_in, dim3(G), dim3(kBlock), 0, b->stream, d_T0, b->orig, b->rho, (int)b->n, bvec(b, 0),
+                     bvec(b, 5));
+  if (use_graph && nsteps > 0) {
+    if (!b->gexec || b->graph_s != s || b->graph_block != b->block) {
+      if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+      if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      batch_super_step(b, s, b->block);
+      HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+      HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_s = s;
+      b->graph_block = b->block;
+    }
+    for (int st = 0; st < nsteps; st++) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+  } else {
+    for (int st = 0; st < nsteps; st++) batch_super_step(b, s, b->block);
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, b->stream, bvec(b, 0), b->orig, (int)b->n, d_T);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SPH_OK;
+}
+
+int sph_batch_get_info(sph_batch *b, sph_batch_info *info) {
+  if (!b || !info) return SPH_ERR_BAD_ARG;
+  info->n_problems = b->nprob;
+  info->n_total = b->n;
+  info->n_pairs = b->n_pairs;
+  info->list_cap = b->cap;
+  info->block = b->block;
+  for (int k = 0; k < 4; k++) {
+    info->tune_block[k] = kBatchBlocks[k];
+    info->tune_ms[k] = b->tune_ms[k];
+  }
+  return SPH_OK;
+}
+
+int sph_batch_offsets(sph_batch *b, int64_t *off) {
+  if (!b || !off) return SPH_ERR_BAD_ARG;
+  for (int p = 0; p <= b->nprob; p++) off[p] = b->off[p];
   return SPH_OK;
 }
 

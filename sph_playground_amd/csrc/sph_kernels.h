@@ -131,6 +131,23 @@ struct Loop2Args {
 };
 
 // Up to four adjoint stages' (lam, Zt) records (sph_adjoint.hip).
+// One RKL2 stage over merged problems (k_batch_stage): list-layout operator
+// (cnt, nb, W), densities, each row's problem (pid) and the per-problem time
+// steps; stage states Y_{j-1}, Y_{j-2}, Y_0, Z_{j-1} = Y_{j-1} rho, the
+// stage-1 rate DY0 and the outputs Y_j, Z_j; coefficients mu, nu,
+// 1 - mu - nu, mu_hat, gamma_hat.
+struct BatchStage {
+  const int *cnt;
+  const uint32_t *nb;
+  const double *W, *rho;
+  const int *pid;
+  const double *dt;
+  const double *Yjm1, *Yjm2, *Y0, *Zjm1;
+  double *DY0, *Yj, *Zj;
+  double mu, nu, c0, mhat, ghat;
+  int n, npad;
+};
+
 struct LZSet {
   const double2 *p[4];
 };

@@ -241,6 +241,35 @@ def diffusion2d(n_side=32, hfac=1.0, ddw="fab", b=(1.0, 0.0, 0.0), disotropic="y
     return state, st
 
 
+def batch2d_specs(nprob=256, seed=11, n_min=1024, n_max=262144, ratios=(1.0, 0.1, 0.01, 0.001)):
+    """A batched 2D anisotropic-diffusion study: problem k has
+    N_k = round(exp(U[ln n_min, ln n_max])) particles on a periodic square
+    lattice (n_side = round(sqrt(N_k))), field angle theta ~ U[0, pi) and
+    kappa_perp / kappa_par drawn from ``ratios``.  Returns a list of dicts
+    (n_side, theta, ratio)."""
+    u = uniform(seed, 3 * nprob).reshape(nprob, 3)
+    lo, hi = math.log(n_min), math.log(n_max)
+    specs = []
+    for k in range(nprob):
+        nk = round(math.exp(lo + (hi - lo) * u[k, 0]))
+        specs.append(dict(n_side=max(4, int(round(math.sqrt(nk)))), theta=math.pi * float(u[k, 1]),
+                          ratio=float(ratios[min(int(u[k, 2] * len(ratios)), len(ratios) - 1)])))
+    return specs
+
+
+def batch2d_problem(spec):
+    """One problem of ``batch2d_specs``: 2D diffusion in periodic [0,1]^2 with
+    T = sin(2 pi x) sin(2 pi y), b = (cos theta, sin theta, 0), kappa_par = 1,
+    the full tensor (kappa_perp = ratio).  Returns (state, store, K) with
+    K = kappa_perp I + (kappa_par - kappa_perp) b b as (xx, yy, zz, xy, xz, yz)."""
+    t, r = spec["theta"], spec["ratio"]
+    b = (math.cos(t), math.sin(t), 0.0)
+    state, st = diffusion2d(spec["n_side"], ddw="fab", b=b, disotropic="no", dcond=1.0)
+    K = np.array([r + (1.0 - r) * b[0] * b[0], r + (1.0 - r) * b[1] * b[1], r,
+                  (1.0 - r) * b[0] * b[1], 0.0, 0.0])
+    return state, st, K
+
+
 def random_slab3d(n, seed=3, box=(-1.0, 1.0, 0.0, 0.125, 0.0, 0.125), hfac=1.2,
                   equations="magnetohydro", b=(0.75, 1.0, 0.0), kernel=C.KERNEL_M6,
                   fld=False, adden="yes", artts="yes", au=1.0, process="fullyperiodic",

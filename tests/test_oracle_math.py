@@ -207,3 +207,24 @@ def test_neighbour_sets_equal_minimum_image_brute_force():
                     if sd <= r2:
                         exp.add((j, (sx + 1) + 3 * (sy + 1) + 9))
         assert got[i] == exp, i
+
+
+def test_batch2d_specs_follow_the_study_distribution():
+    # N log-uniform in [32^2, 512^2] on square lattices, theta in [0, pi),
+    # kappa_perp / kappa_par in {1, 0.1, 0.01, 0.001}; deterministic per seed
+    from sph_playground_amd import problems as P
+    a = P.batch2d_specs(256, seed=11)
+    assert a == P.batch2d_specs(256, seed=11)
+    N = np.array([x["n_side"] ** 2 for x in a])
+    assert N.min() >= 31 ** 2 and N.max() <= 512 ** 2
+    assert 0.5e7 < N.sum() < 2.0e7  # E[sum N] = 256 (262144 - 1024) / ln 256 = 1.2e7
+    ln = np.log(N)
+    assert abs(np.median(ln) - 0.5 * (np.log(1024) + np.log(262144))) < 0.5
+    assert all(0.0 <= x["theta"] < math.pi for x in a)
+    assert {x["ratio"] for x in a} == {1.0, 0.1, 0.01, 0.001}
+    state, st, K = P.batch2d_problem(a[0])
+    t, r = a[0]["theta"], a[0]["ratio"]
+    b = np.array([math.cos(t), math.sin(t), 0.0])
+    Km = r * np.eye(3) + (1 - r) * np.outer(b, b)
+    assert np.allclose(K, [Km[0, 0], Km[1, 1], Km[2, 2], Km[0, 1], Km[0, 2], Km[1, 2]], rtol=0, atol=1e-15)
+    assert st.shape[0] == a[0]["n_side"] ** 2
