@@ -300,11 +300,15 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 // span).  Candidates inside the band, and any lane whose radius is not a
 // finite number, take the exact FP64 rule on pm, so the sets are those of
 // the FP64 test.
+// pending list entries per lane in k_neigh (a power of two, whole chunks)
+constexpr int kNbRing = 16;
+
 template <int DIM, int RS>
 __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
                                                const float4 *__restrict__ pf, uint32_t *__restrict__ nb,
                                                int *__restrict__ cnt) {
   __shared__ long long lds[4];
+  __shared__ __align__(16) uint32_t nb_ring[256 * kNbRing];
   int s = gtid();
   int count = 0;
   bool live = s < a.n && a.real[s];
@@ -320,9 +324,22 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     const bool finite_r = rr == rr && rr < HUGE_VAL;
     const float hi_thr = finite_r ? (float)(r2 * (1.0 + 1e-5) + mabs) : HUGE_VALF;
     const float lo_thr = finite_r ? (float)(r2 * (1.0 - 1e-5) - mabs) : -HUGE_VALF;
-    // list slot of entry `count` (nb_slot), advanced incrementally
-    size_t widx = (size_t)s * 8;
-    const size_t wstep = (size_t)a.npad * 8 - 7;
+    // entries [done, count) wait in the lane's ring and leave as whole
+    // 32-byte chunks: after each stencil cell (the same point for every
+    // lane, so the lanes' chunk stores share instructions), when the ring
+    // is full, and the last partial chunk at the end
+    uint32_t *ring = nb_ring + threadIdx.x * kNbRing;
+    int done = 0;
+    auto flush_chunk = [&]() {
+      if (done < a.cap) {  // cap is whole chunks: the chunk fits
+        const uint4 *src = reinterpret_cast<const uint4 *>(ring + (done & (kNbRing - 1)));
+        uint4 *dst = reinterpret_cast<uint4 *>(nb + nb_slot(done, s, a.npad));
+        const uint4 lo = src[0], hi = src[1];
+        dst[0] = lo;
+        dst[1] = hi;
+      }
+      done += 8;
+    };
     int ci[3];
     cell_of(a.g, pi.x, pi.y, pi.z, ci);
     const double *cs = a.g.cs;
@@ -385,9 +402,9 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
               hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
             }
             if (hit) {
-              if (count < a.cap) nb[widx] = nb_pack((uint32_t)tt, code);
+              ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
               count++;
-              widx += (count & 7) ? 1 : wstep;
+              if (count - done == kNbRing) flush_chunk();
             }
           };
           // image existence as (band & bm) == bv; self (unshifted) excluded
@@ -405,9 +422,11 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
             for (int u = 0; u < 4; u++)
               test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
           }
+          while (count - done >= 8) flush_chunk();
         }
       }
     }
+    if (count > done) flush_chunk();  // slots past count in this chunk are never read
     cnt[s] = count;
     if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
     if (count > a.cap) atomicOr(a.err, EB_OVERFLOW);
