@@ -30,7 +30,7 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_range(const uint32_t *, int, Grid, int2 *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, double *);
+                       uint8_t *, double4 *, double *, double4 *, double *);
 template <int DIM, int RS>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -137,6 +137,7 @@ struct sph_ctx {
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
   double *tr[2] = {nullptr, nullptr};  // T den of the loop-1 sweep state st[k]
+  double4 *jr[2] = {nullptr, nullptr};  // mixed path: (x, y, z, [m, T den] FP32) of sweep state st[k]
   double4 *pd = nullptr;                // loop 2, diffusion mode: x, y, z, m/den
   double *trd = nullptr;                // loop 2, diffusion mode: T den
   double *h0 = nullptr, *eps = nullptr;
@@ -279,6 +280,7 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
     RCHK(dalloc(&c->orig[b], npad));
     RCHK(dalloc(&c->st[b], npad));
     RCHK(dalloc(&c->tr[b], npad));
+    if (c->opt.precision == 1) RCHK(dalloc(&c->jr[b], npad));
   }
   RCHK(dalloc(&c->pd, npad));
   RCHK(dalloc(&c->trd, npad));
@@ -490,7 +492,7 @@ static int build_neighbours(sph_ctx *c) {
   HIPCHK(hipMemsetAsync(c->crange, 0, sizeof(int2) * ncell, c->stream));
   hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->crange);
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->eps);
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -598,6 +600,8 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
     a.next = c->st[it & 1];
     a.trc = c->tr[(it - 1) & 1];
     a.trn = c->tr[it & 1];
+    a.jrc = c->jr[(it - 1) & 1];
+    a.jrn = c->jr[it & 1];
     HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
@@ -898,7 +902,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->pd, c->trd,
+                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->jr[0], c->jr[1], c->pd, c->trd,
                   c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,

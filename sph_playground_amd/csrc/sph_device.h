@@ -31,6 +31,14 @@ __device__ __forceinline__ uint32_t nb_pack(uint32_t j, uint32_t code) { return 
 __device__ __forceinline__ uint32_t nb_index(uint32_t e) { return e >> kCodeBits; }
 __device__ __forceinline__ uint32_t nb_code(uint32_t e) { return e & 31u; }
 
+// Two FP32 values in the 8 bytes of one double slot (the mixed path's gathered
+// j-records carry (x, y, z) in FP64 and two FP32 scalars in the fourth slot).
+__device__ __forceinline__ double pack_f2(float lo, float hi) {
+  return __hiloint2double(__float_as_int(hi), __float_as_int(lo));
+}
+__device__ __forceinline__ float f2_lo(double d) { return __int_as_float(__double2loint(d)); }
+__device__ __forceinline__ float f2_hi(double d) { return __int_as_float(__double2hiint(d)); }
+
 // Periodic image coordinate exactly as the reference ghost copies compute it
 // (BC.f90:339,366,396,419,445,461): +1 -> emax + (x - emin), -1 -> emin + (x - emax).
 __device__ __forceinline__ double image_coord(double x, int s, double emin, double emax) {

@@ -97,6 +97,8 @@ struct Loop1Args {
   V4 *next;                // sweep k+1
   const double *trc;       // T den of sweep k (the only j-side value grad T needs)
   double *trn;             // T den of sweep k+1
+  const double4 *jrc;      // mixed path, grad T: (x, y, z, [m, T den] as FP32) of sweep k
+  double4 *jrn;            // ... of sweep k+1
   double *eps;
   double *dtdx, *dtdy, *dtdz;
   long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs

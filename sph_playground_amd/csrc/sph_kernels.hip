@@ -219,7 +219,7 @@ __global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int2 *crange) 
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
-                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double *eps) {
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double4 *jr, double *eps) {
   int s = gtid();
   if (s >= n) return;
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
@@ -230,6 +230,7 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
   real[s] = r ? 1 : 0;
   st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
   tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
+  if (jr) jr[s] = make_double4(x, y, z, pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s])));
   eps[s] = r ? 1.0 : 0.0;
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
@@ -453,9 +454,13 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   bool act = inb && a.real[s] && a.eps[s] > a.tol;
   bool still = false;
   int nn = 0;
+  // the FP32 path gathers one 32-byte record per pair: FP64 (x, y, z) and the
+  // FP32 m and T den it converts them to anyway
+  constexpr bool PACK = GRAD == 1 && std::is_same_v<PT, float>;
   if (inb && !act) {
     a.next[s] = c;
-    if constexpr (GRAD == 1) a.trn[s] = a.trc[s];
+    if constexpr (PACK) a.jrn[s] = a.jrc[s];
+    else if constexpr (GRAD == 1) a.trn[s] = a.trc[s];
   }
   if (act) {
     using K = KernelFn<KT, PT>;
@@ -474,8 +479,12 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
-      d.pb = a.pm[j];
-      if constexpr (GRAD == 1) d.tr = a.trc[j];
+      if constexpr (PACK) {
+        d.pb = a.jrc[j];
+      } else {
+        d.pb = a.pm[j];
+        if constexpr (GRAD == 1) d.tr = a.trc[j];
+      }
       if constexpr (GRAD == 2) d.sb = a.cur[j];
       return d;
     };
@@ -488,14 +497,19 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       PT q = dr * inv_ha;
       PT f, df;
       K::f_df(q, f, df);
-      PT mb = (PT)d.pb.w;
+      PT mb;
+      if constexpr (PACK) mb = f2_lo(d.pb.w);
+      else mb = (PT)d.pb.w;
       dsum += mb * f;
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
         const double4 sb = d.sb;
         PT inv_dr = PT(1) / dr;
         if constexpr (GRAD == 1) {
-          PT w = mb * ((PT)d.tr - (PT)ta * (PT)rhoa) * df * inv_dr;
+          PT trb;
+          if constexpr (PACK) trb = f2_hi(d.pb.w);
+          else trb = (PT)d.tr;
+          PT w = mb * (trb - (PT)ta * (PT)rhoa) * df * inv_dr;
           g0 += w * r0; g1 += w * r1; g2 += w * r2;
         } else {
           PT inv_hb = (PT)(1.0 / sb.x);
@@ -528,7 +542,8 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     a.eps[s] = ep;
     still = ep > a.tol;
     a.next[s] = make_double4(hn, den, osm, ta);
-    if constexpr (GRAD == 1) a.trn[s] = ta * den;
+    if constexpr (PACK) a.jrn[s] = make_double4(pa.x, pa.y, pa.z, pack_f2((float)ma, (float)(ta * den)));
+    else if constexpr (GRAD == 1) a.trn[s] = ta * den;
     double gx = 0, gy = 0, gz = 0;
     if constexpr (GRAD == 1) {
       double cN = wcv * ipow<DIM + 1>(ih) / rhoa / oma;
@@ -658,8 +673,13 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
     if (q4) q4[s] = V4{(T)bx, (T)by, (T)bz, (T)0};
     if (pd) {
-      pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
-      trd[s] = F.f[F_T][s] * den;
+      if constexpr (std::is_same_v<V4, float4>) {  // FP32 path: m/den and T den as FP32 in the fourth slot
+        pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s],
+                             pack_f2((float)(F.f[F_M][s] / den), (float)(F.f[F_T][s] * den)));
+      } else {
+        pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
+        trd[s] = F.f[F_T][s] * den;
+      }
     }
   }
   long long t = block_reduce<R_OR>(fl, lds);
@@ -768,8 +788,10 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
     const int nn = a.cnt[s];
     // batched: generic mode gathers positions/mass and {1/h, den, om, T};
     // MODE_DIFF gathers only what the Hessian term reads, {x, y, z, m/den}
-    // and T den (the same FP64 values the term forms from m, den, T).  The
+    // and T den (the same values the term forms from m, den, T).  The
     // optional packs (v, c/u/kappa, grad T, B) are gathered in the body.
+    // the FP32 path's record holds m/den and T den as FP32 in its fourth slot
+    constexpr bool PACK = std::is_same_v<V4, float4>;
     struct JG { double4 pb; V4 q0; uint32_t j, code; };
     struct JD { double4 pb; double tr; uint32_t j, code; };
     using J = std::conditional_t<MODE == MODE_DIFF, JD, JG>;
@@ -779,7 +801,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
       d.code = nb_code(ent);
       if constexpr (MODE == MODE_DIFF) {
         d.pb = a.pd[d.j];
-        d.tr = a.trd[d.j];
+        if constexpr (!PACK) d.tr = a.trd[d.j];
       } else {
         d.pb = a.pm[d.j];
         d.q0 = a.q0[d.j];
@@ -796,8 +818,13 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
       // j-side scalars; in MODE_DIFF mb_rhob = m_b / rho_b and trb = T_b rho_b
       PT mb = 0, inv_hb = 0, rhob = 0, omb = 0, tb = 0, mb_rhob = 0, trb = 0;
       if constexpr (MODE == MODE_DIFF) {
-        mb_rhob = (PT)pb.w;
-        trb = (PT)d.tr;
+        if constexpr (PACK) {
+          mb_rhob = f2_lo(pb.w);
+          trb = f2_hi(pb.w);
+        } else {
+          mb_rhob = (PT)pb.w;
+          trb = (PT)d.tr;
+        }
         if (need_v && P.adden) mb = (PT)a.pm[d.j].w;  // d rho/dt of moving particles only
       } else {
         const V4 B0 = d.q0;
