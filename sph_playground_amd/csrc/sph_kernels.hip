@@ -402,11 +402,11 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
               const double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
               hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
             }
-            if (hit) {
-              ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
-              count++;
-              if (count - done == kNbRing) flush_chunk();
-            }
+            // every lane writes the next free ring slot and keeps it on a hit
+            // (at most kNbRing - 1 entries are pending here, so that slot
+            // holds none of them)
+            ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
+            count += hit ? 1 : 0;
           };
           // image existence as (band & bm) == bv; self (unshifted) excluded
           uint32_t bm, bv;
@@ -419,9 +419,17 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
             float4 q[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) q[u] = p[u];
+            if (t + 4 <= t1) {  // a full batch: no range test
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-              test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+              for (int u = 0; u < 4; u++)
+                test(t + u, q[u], (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+            } else {
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+            }
+            // at most 11 pending before the next batch's 4 candidates
+            if (count - done > kNbRing - 5) flush_chunk();
           }
           while (count - done >= 8) flush_chunk();
         }
