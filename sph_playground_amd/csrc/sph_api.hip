@@ -30,7 +30,7 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_range(const uint32_t *, int, Grid, int2 *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, double4 *, double *);
+                       uint8_t *, double4 *, double *, double4 *, double *, long long *);
 template <int DIM, int RS>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -91,6 +91,8 @@ __global__ void k_batch_in(const double *, const int64_t *, const double *, int,
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kFlagSlots = 4 * kStripes;  // d_stripes slots of k_prep's flags
+constexpr int kLoop2ErrSlot = 255;        // h_int slot of loop 2's error bits (last of 256)
 constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fields
 
 // statevars slots (0-based), const.f90:14-66
@@ -146,6 +148,9 @@ struct sph_ctx {
   uint32_t *nb = nullptr;
   int nb_cap = 0;
   int last_maxcnt = 0;  // longest neighbour list of the last search
+  long long step_flags = 0;       // k_prep's v / B flags of this step
+  double *h_energy = nullptr;     // pinned loop-2 energy partials (energy_cap)
+  bool loop2_pending = false;     // loop 2's read-back still to be taken
   int *cnt = nullptr;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *energy_part = nullptr;
@@ -299,6 +304,9 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
   for (int k = 0; k < 5; k++) RCHK(dev_alloc(&c->q[k], v4 * npad));
   c->energy_cap = (npad + kBlock - 1) / kBlock;
   RCHK(dalloc(&c->energy_part, c->energy_cap));
+  if (c->h_energy) (void)hipHostFree(c->h_energy);
+  c->h_energy = nullptr;
+  if (hipHostMalloc((void **)&c->h_energy, sizeof(double) * c->energy_cap) != hipSuccess) return SPH_ERR_NOMEM;
   // radix-sort scratch for n keys
   size_t bytes = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->keys, c->keys2, c->vals, c->vals2, (int)npad, 0, 32,
@@ -491,8 +499,10 @@ static int build_neighbours(sph_ctx *c) {
   }
   HIPCHK(hipMemsetAsync(c->crange, 0, sizeof(int2) * ncell, c->stream));
   hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->crange);
+  HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps);
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
+                     c->d_stripes + kFlagSlots);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -524,9 +534,12 @@ static int build_neighbours(sph_ctx *c) {
     }
 #undef NEIGH
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
-                          c->stream));
+    // pair counts, list lengths and k_prep's flags in one read-back
+    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
+                          hipMemcpyDeviceToHost, c->stream));
     RCHK(read_ints(c, 0, 1));
+    c->step_flags = 0;
+    for (int k = 0; k < kStripes; k++) c->step_flags |= c->h_ll[kFlagSlots + k];
     c->stats.n_pairs = stripe_sum(c->h_ll, 0);
     int maxcnt = (int)stripe_max(c->h_ll, 1);
     c->last_maxcnt = maxcnt;
@@ -673,16 +686,13 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   V4 *q2 = want_q2 ? (V4 *)c->q[2] : nullptr;
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
   V4 *q4 = want_q4 ? (V4 *)c->q[4] : nullptr;
-  HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * kStripes, c->stream));
+  long long *none = nullptr;
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
                      c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
-                     generic ? nullptr : c->trd, c->d_stripes);
+                     generic ? nullptr : c->trd, none);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
-  HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * kStripes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  long long fl = 0;
-  for (int k = 0; k < kStripes; k++) fl |= c->h_ll[k];
+  const long long fl = c->step_flags;  // from k_prep: v and B are what they were there
   Loop2Args<V4> a{};
   a.pm = c->pm; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
@@ -697,14 +707,24 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
   else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
   HIPCHK(hipEventRecord(c->ev[6], c->stream));
-  std::vector<double> parts(G);
-  HIPCHK(hipMemcpyAsync(parts.data(), c->energy_part, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  RCHK(read_ints(c, 0, 1));
+  // energy partials and error bits come back with the step's closing sync
+  HIPCHK(hipMemcpyAsync(c->h_energy, c->energy_part, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_int + kLoop2ErrSlot, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  c->loop2_pending = true;
+  return SPH_OK;
+}
+
+// After the step's closing sync: loop 2's energy sum (block partials summed
+// in block order) and its error bits.
+static int finish_loop2(sph_ctx *c) {
+  if (!c->loop2_pending) return SPH_OK;
+  c->loop2_pending = false;
+  const unsigned G = grid_for(c->n);
   double e = 0;
-  for (unsigned b = 0; b < G; b++) e += parts[b];
+  for (unsigned b = 0; b < G; b++) e += c->h_energy[b];
   c->stats.energy = e;
   c->stats.ms_k_loop2 = elapsed(c, c->ev[5], c->ev[6]);
-  return map_err(c->h_int[0]);
+  return map_err(c->h_int[kLoop2ErrSlot]);
 }
 
 static Phys call_phys(const sph_ctx *c, const double *state) {
@@ -736,6 +756,8 @@ static int run_step(sph_ctx *c, const double *state) {
                        c->gravity);
   HIPCHK(hipEventRecord(c->ev[3], c->stream));
   HIPCHK(hipEventSynchronize(c->ev[3]));
+  const int rc2 = finish_loop2(c);
+  if (rc == SPH_OK) rc = rc2;
   float t01 = 0, t12 = 0, t23 = 0;
   (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
@@ -912,6 +934,7 @@ int sph_ctx_destroy(sph_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
   if (c->h_ll) (void)hipHostFree(c->h_ll);
+  if (c->h_energy) (void)hipHostFree(c->h_energy);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -218,10 +218,9 @@ __global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int2 *crange) 
 // to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
-__global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
-                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double4 *jr, double *eps) {
-  int s = gtid();
-  if (s >= n) return;
+__device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, int64_t n_real, Box b, Grid g,
+                                         int dim, double4 *pm, double *h0, float4 *pf, uint8_t *real, double4 *st,
+                                         double *tr, double4 *jr, double *eps) {
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
   pm[s] = make_double4(x, y, z, F.f[F_M][s]);
   double h = F.f[F_H][s];
@@ -251,6 +250,25 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
     }
   }
   pf[s] = make_float4((float)(x - g.lo[0]), (float)(y - g.lo[1]), (float)(z - g.lo[2]), __uint_as_float(code));
+}
+
+// flags: OR of bit 0 = some velocity is non-zero, bit 1 = some B differs from
+// particle 0's (loop 2 skips terms that are exactly zero otherwise; neither
+// changes before loop 2).
+__global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double4 *jr, double *eps,
+                       long long *flags) {
+  __shared__ long long lds[4];
+  int s = gtid();
+  int fl = 0;
+  if (s < n) {
+    const double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
+    if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
+    if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
+    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, jr, eps);
+  }
+  long long t = block_reduce<R_OR>(fl, lds);
+  if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
 }
 
 // Does the reference hold a ghost of j shifted by (sx, sy, sz)?  The first
@@ -673,8 +691,10 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
     }
     double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
     double bx = F.f[F_BX][s], by = F.f[F_BY][s], bz = F.f[F_BZ][s];
-    if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
-    if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;
+    if (flags) {
+      if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
+      if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;
+    }
     q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_T][s]};
     if (q1) q1[s] = V4{(T)vx, (T)vy, (T)vz, (T)F.f[F_P][s]};
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
@@ -690,8 +710,10 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
       }
     }
   }
-  long long t = block_reduce<R_OR>(fl, lds);
-  if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
+  if (flags) {
+    long long t = block_reduce<R_OR>(fl, lds);
+    if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
+  }
 }
 
 // ---------------------------------------------------------------- loop 2
