@@ -39,6 +39,13 @@ __device__ __forceinline__ double pack_f2(float lo, float hi) {
 __device__ __forceinline__ float f2_lo(double d) { return __int_as_float(__double2loint(d)); }
 __device__ __forceinline__ float f2_hi(double d) { return __int_as_float(__double2hiint(d)); }
 
+// Reciprocal and square root of the pair math: FP32 uses the hardware
+// instructions (v_rcp_f32, v_sqrt_f32, within 1 ulp), FP64 the IEEE operations.
+__device__ __forceinline__ float rcp_pair(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_pair(double x) { return 1.0 / x; }
+__device__ __forceinline__ float sqrt_pair(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double sqrt_pair(double x) { return sqrt(x); }
+
 // Periodic image coordinate exactly as the reference ghost copies compute it
 // (BC.f90:339,366,396,419,445,461): +1 -> emax + (x - emin), -1 -> emin + (x - emax).
 __device__ __forceinline__ double image_coord(double x, int s, double emin, double emax) {

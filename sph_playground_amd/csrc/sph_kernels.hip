@@ -514,12 +514,13 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       if constexpr (GRAD == 2) d.sb = a.cur[j];
       return d;
     };
-    for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
+    // FP32 path: a whole chunk of 8 gathers in flight (measured faster than 4)
+    for_neighbours<(std::is_same_v<PT, float> ? 8 : 4)>(a.nb, s, a.npad, nn, load, [&](const J &d) {
       double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
       if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
       PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
-      PT dr = sqrt(rsq);
+      PT dr = sqrt_pair(rsq);
       PT q = dr * inv_ha;
       PT f, df;
       K::f_df(q, f, df);
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
         const double4 sb = d.sb;
-        PT inv_dr = PT(1) / dr;
+        PT inv_dr = rcp_pair(dr);
         if constexpr (GRAD == 1) {
           PT trb;
           if constexpr (PACK) trb = f2_hi(d.pb.w);
@@ -792,6 +793,7 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
     // i-side constants
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
+    const PT inv_rhoa = rcp_pair(rhoa);      // FP32 path: 1/rho_a once per particle
     const PT rho2oma = rhoa * rhoa * oma;
     const PT ub2 = bax * bax + bay * bay + baz * baz;
     Sym3<PT> kta = {0, 0, 0, 0, 0, 0}, ktb_fixed = {0, 0, 0, 0, 0, 0};
@@ -862,8 +864,8 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
         inv_hb = (PT)B0.x; rhob = (PT)B0.y; omb = (PT)B0.z; tb = (PT)B0.w;
       }
       const PT r2 = rx * rx + ry * ry + rz * rz;
-      const PT inv_r2 = PT(1) / r2;
-      const PT dr = sqrt(r2);
+      const PT inv_r2 = rcp_pair(r2);
+      const PT dr = sqrt_pair(r2);
       const PT inv_dr = dr * inv_r2;
       const PT qa = dr * inv_ha;
       PT dfa, ddfa = 0, fa = 0;
@@ -957,7 +959,10 @@ __global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
                                      : -wcv * dfa * ihd1 * inv_dr;  // 0.5*Gab
             contr = half * (PT(DIM + 2) * S - Tr);
           }
-          if constexpr (MODE == MODE_DIFF) ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+          if constexpr (MODE == MODE_DIFF) {
+            if constexpr (std::is_same_v<PT, float>) ddt += mb_rhob * inv_rhoa * (trb - ta * rhoa) * contr;
+            else ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+          }
           else ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
         } else if (MODE == MODE_GENERIC) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
