@@ -681,18 +681,20 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   const bool want_q2 = generic && ((P.diff && P.fld) || P.artts);
   const bool want_q3 = generic && P.diff && (P.fld || P.ddw == 403 || P.ddw == 404);
   const bool want_q4 = P.magneto || (P.diff && !P.fld && P.diso == 0);
+  // k_prep's flags (v and B are what they were there): the diffusion-only
+  // loop 2 reads v only where it is non-zero and B only where it varies
+  const long long fl = c->step_flags;
   V4 *q0 = (V4 *)c->q[0];
-  V4 *q1 = want_v ? (V4 *)c->q[1] : nullptr;
+  V4 *q1 = want_v && (generic || (fl & 1)) ? (V4 *)c->q[1] : nullptr;
   V4 *q2 = want_q2 ? (V4 *)c->q[2] : nullptr;
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
-  V4 *q4 = want_q4 ? (V4 *)c->q[4] : nullptr;
+  V4 *q4 = want_q4 && (generic || (fl & 2)) ? (V4 *)c->q[4] : nullptr;
   long long *none = nullptr;
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
                      c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
                      generic ? nullptr : c->trd, none);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
-  const long long fl = c->step_flags;  // from k_prep: v and B are what they were there
   Loop2Args<V4> a{};
   a.pm = c->pm; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt;
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;

@@ -139,7 +139,7 @@ __device__ __forceinline__ void cell_of(const Grid &g, double x, double y, doubl
 // red[v * kStripes + stripe]: v = 0..2 min, 3..5 max (ordered int64), 6 max h.
 __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_real, Box b,
                                                      int dim, long long *red, int *err) {
-  __shared__ long long lds[4];
+  __shared__ long long lds7[7 * 4];  // per-wave partials of the 256-thread block
   int s = gtid();
   long long mn[3], mx[3], hm;
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
@@ -164,14 +164,24 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
     for (int a = 0; a < 3; a++) { mn[a] = mx[a] = d2ord(x[a]); }
     if (real) hm = d2ord(F.f[F_H][s]);
   }
-  for (int a = 0; a < 3; a++) {
-    long long v = block_reduce<R_MIN>(mn[a], lds);
-    if (threadIdx.x == 0) atomicMin(&red[a * kStripes + stripe()], v);
-    v = block_reduce<R_MAX>(mx[a], lds);
-    if (threadIdx.x == 0) atomicMax(&red[(3 + a) * kStripes + stripe()], v);
+  // the seven block reductions in one pass: per wave, then one barrier, then
+  // thread v combines value v over the block's waves
+  long long w7[7] = {wave_min(mn[0]), wave_min(mn[1]), wave_min(mn[2]), wave_max(mx[0]), wave_max(mx[1]),
+                     wave_max(mx[2]), wave_max(hm)};
+  const int nw = (int)(blockDim.x >> 6);
+  if ((threadIdx.x & 63) == 0)
+    for (int v = 0; v < 7; v++) lds7[v * 4 + (threadIdx.x >> 6)] = w7[v];
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const int v = threadIdx.x;
+    long long t = lds7[v * 4];
+    for (int w = 1; w < nw; w++) {
+      const long long u = lds7[v * 4 + w];
+      t = v < 3 ? (u < t ? u : t) : (u > t ? u : t);
+    }
+    if (v < 3) atomicMin(&red[v * kStripes + stripe()], t);
+    else atomicMax(&red[v * kStripes + stripe()], t);
   }
-  long long v = block_reduce<R_MAX>(hm, lds);
-  if (threadIdx.x == 0) atomicMax(&red[6 * kStripes + stripe()], v);
 }
 
 __global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) {
