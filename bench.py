@@ -166,6 +166,21 @@ def batched_cost(nprob=256, s=10, nsteps=5, reps=3):
         ts.append(time.perf_counter() - a)
     t_single = float(np.median(ts))
     same = all(torch.equal(x, y) for x, y in zip(batch.split(out), outp))
+    # the same study with FP32-stored pair weights
+    b32 = Batch(engines, np.stack(Ks), precision="mixed")
+    out32 = torch.empty_like(T0)
+    b32.sts(T0, s, dts, nsteps, graph=True, out=out32)
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        b32.sts(T0, s, dts, nsteps, graph=True, out=out32)
+        ts.append(time.perf_counter() - a)
+    t32 = float(np.median(ts))
+    dev32 = max(float((x - y).abs().max() / max((y - z).abs().max(), 1e-300))
+                for x, y, z in zip(b32.split(out32), batch.split(out), batch.split(T0)))
+    info32 = b32.info()
+    b32.close()
     P_ = info["n_pairs"]
     N_ = info["n_total"]
     inter = P_ * s * nsteps
@@ -184,7 +199,12 @@ def batched_cost(nprob=256, s=10, nsteps=5, reps=3):
             "stage_ms": stage_ms, "stage_algorithmic_bytes": byts,
             "stage_hbm_gbs": byts / (stage_ms * 1e-3) / 1e9,
             "autotune_block": info["block"], "autotune_ms_per_4_stages": info["tune"],
-            "prepare_s": t_prep}
+            "prepare_s": t_prep,
+            "fp32_weights": {"ms_graph": t32 * 1e3, "pair_interactions_per_s": inter / t32,
+                             "stage_ms": t32 * 1e3 / (s * nsteps),
+                             "stage_algorithmic_bytes": 8 * P_ + 80 * N_,
+                             "stage_hbm_gbs": (8 * P_ + 80 * N_) / (t32 / (s * nsteps)) / 1e9,
+                             "max_dev_rel_to_change_vs_fp64": dev32, "autotune_block": info32["block"]}}
 
 
 def cpu_baseline(size):

@@ -163,15 +163,17 @@ typedef struct {
   float tune_ms[4];   /* ... and their time for a 4-stage super-step */
 } sph_batch_info;
 /* Merge ctxs[0..nprob) (same device, sph_diff_prepare done) with tensors
- * K[6 p .. 6 p + 5] (xx, yy, zz, xy, xz, yz); autotunes the stage launch. */
-int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch **out);
+ * K[6 p .. 6 p + 5] (xx, yy, zz, xy, xz, yz); autotunes the stage launch.
+ * precision 0: FP64 pair weights (every problem bitwise its sph_diff_forward);
+ * 1: pair weights stored in FP32, products, sums and stage states FP64. */
+int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, int precision, sph_batch **out);
 int sph_batch_destroy(sph_batch *batch);
 /* nsteps RKL2 super-steps of s stages (sts_integrator.f90:87-124), problem p
  * with time step dt[p] (host array).  d_T0 / d_T: device vectors of n_total
  * doubles, problem p's particles (original order) at offset off[p]
  * (sph_batch_offsets).  use_graph: replay one captured super-step per step
- * (1) or launch the stages directly (0); both give the same bits, which are
- * each problem's sph_diff_forward bits. */
+ * (1) or launch the stages directly (0); both give the same bits (with
+ * precision 0 these are each problem's sph_diff_forward bits). */
 int sph_batch_sts(sph_batch *batch, int s, const double *dt, int nsteps, const double *d_T0, double *d_T,
                   int use_graph);
 int sph_batch_get_info(sph_batch *batch, sph_batch_info *info);

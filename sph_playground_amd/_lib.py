@@ -83,7 +83,7 @@ def lib():
     L.sph_sts_coeffs.argtypes = [ct.c_int, ct.c_int, D]
     L.sph_get_stats.argtypes = [P, ct.POINTER(Stats)]
     L.sph_diff_prepare.argtypes = [P, D]
-    L.sph_batch_create.argtypes = [ct.POINTER(P), ct.c_int, D, ct.POINTER(P)]
+    L.sph_batch_create.argtypes = [ct.POINTER(P), ct.c_int, D, ct.c_int, ct.POINTER(P)]
     L.sph_batch_destroy.argtypes = [P]
     L.sph_batch_sts.argtypes = [P, ct.c_int, D, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]
     L.sph_batch_get_info.argtypes = [P, ct.POINTER(BatchInfo)]

@@ -17,16 +17,23 @@ from ._lib import BatchInfo, check, lib
 _D = ct.POINTER(ct.c_double)
 
 
+_PREC = {"fp64": 0, "mixed": 1}
+
+
 class Batch:
     """Merged operator of ``engines`` (each already prepared) with the
-    per-problem tensors ``K`` ((nprob, 6): xx, yy, zz, xy, xz, yz)."""
+    per-problem tensors ``K`` ((nprob, 6): xx, yy, zz, xy, xz, yz).
+    ``precision``: 'fp64' (each problem bitwise its unbatched super-steps) or
+    'mixed' (pair weights stored in FP32; products, sums and states FP64)."""
 
-    def __init__(self, engines, K):
+    def __init__(self, engines, K, precision="fp64"):
         self.nprob = len(engines)
+        self.precision = precision
         Kh = np.ascontiguousarray(np.asarray(K, dtype=np.float64).reshape(self.nprob, 6))
         hs = (ct.c_void_p * self.nprob)(*[e._h.value for e in engines])
         h = ct.c_void_p()
-        check(lib().sph_batch_create(hs, self.nprob, Kh.ctypes.data_as(_D), ct.byref(h)), "sph_batch_create")
+        check(lib().sph_batch_create(hs, self.nprob, Kh.ctypes.data_as(_D), _PREC[precision], ct.byref(h)),
+              "sph_batch_create")
         self._h = h
         off = np.zeros(self.nprob + 1, dtype=np.int64)
         check(lib().sph_batch_offsets(self._h, off.ctypes.data_as(ct.POINTER(ct.c_int64))), "sph_batch_offsets")

@@ -20,6 +20,8 @@
 // contracts with G once per backward pass.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sph_kernels.h"
 
 namespace sph {
@@ -375,7 +377,7 @@ __global__ void k_rev_chunk(const int *roff, const uint32_t *flat, int n, int np
 // RKL2 stage of every problem is then one launch of k_batch_stage.
 __global__ void k_batch_merge(int n, int npad, int64_t off, int pid, const int *cnt, const uint32_t *nb,
                               const double *W, const double4 *q0, const int64_t *orig, int npad_g, int *mcnt,
-                              uint32_t *mnb, double *mW, double *mrho, int64_t *morig, int *mpid) {
+                              uint32_t *mnb, double *mW, float *mW32, double *mrho, int64_t *morig, int *mpid) {
   int s = gtid_a();
   if (s >= n) return;
   const int64_t g = off + s;
@@ -387,7 +389,8 @@ __global__ void k_batch_merge(int n, int npad, int64_t off, int pid, const int *
   for (int k = 0; k < c; k++) {
     const size_t src = nb_slot(k, s, npad), dst = nb_slot(k, (int)g, npad_g);
     mnb[dst] = (uint32_t)(off + nb_index(nb[src]));
-    mW[dst] = W[src];
+    if (mW32) mW32[dst] = (float)W[src];
+    else mW[dst] = W[src];
   }
 }
 
@@ -417,7 +420,7 @@ __global__ void k_batch_in(const double *T0, const int64_t *orig, const double *
 // then explicit fused multiply-adds in the same order), so every problem's
 // result is bitwise the unbatched super-step's.  Y_j may be Y_0's buffer:
 // row s reads only its own Y_0 entry before writing it.
-template <bool FIRST>
+template <bool FIRST, typename WT>
 __global__ __launch_bounds__(512) void k_batch_stage(BatchStage a) {
   int s = gtid_a();
   if (s >= a.n) return;
@@ -428,11 +431,20 @@ __global__ __launch_bounds__(512) void k_batch_stage(BatchStage a) {
   for (int k0 = 0; k0 < nn; k0 += 8) {
     const size_t base = nb_slot(k0, s, a.npad);
     const uint4 *pe = reinterpret_cast<const uint4 *>(a.nb + base);
-    const double4 *pw = reinterpret_cast<const double4 *>(a.W + base);
     const uint4 e0 = pe[0], e1 = pe[1];
-    const double4 w0 = pw[0], w1 = pw[1];
+    double w8[8];
+    if constexpr (std::is_same_v<WT, float>) {  // FP32 weights, FP64 products and sums
+      const float4 *pw = reinterpret_cast<const float4 *>(a.W32 + base);
+      const float4 w0 = pw[0], w1 = pw[1];
+      w8[0] = w0.x; w8[1] = w0.y; w8[2] = w0.z; w8[3] = w0.w;
+      w8[4] = w1.x; w8[5] = w1.y; w8[6] = w1.z; w8[7] = w1.w;
+    } else {
+      const double4 *pw = reinterpret_cast<const double4 *>(a.W + base);
+      const double4 w0 = pw[0], w1 = pw[1];
+      w8[0] = w0.x; w8[1] = w0.y; w8[2] = w0.z; w8[3] = w0.w;
+      w8[4] = w1.x; w8[5] = w1.y; w8[6] = w1.z; w8[7] = w1.w;
+    }
     const uint32_t e8[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
-    const double w8[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
     const int m = min(8, nn - k0);
     double z8[8];
 #pragma unroll
@@ -495,8 +507,10 @@ __global__ void k_lincomb(int n, double *y, double c0, const double *x0, double 
   y[s] = v;
 }
 
-template __global__ void k_batch_stage<true>(BatchStage);
-template __global__ void k_batch_stage<false>(BatchStage);
+template __global__ void k_batch_stage<true, double>(BatchStage);
+template __global__ void k_batch_stage<false, double>(BatchStage);
+template __global__ void k_batch_stage<true, float>(BatchStage);
+template __global__ void k_batch_stage<false, float>(BatchStage);
 
 #define SPH_INST_ADJ(DIM, KT)                                                                  \
   template __global__ void k_diff_weights<DIM, KT>(DiffOpArgs, double *, double *);                 \

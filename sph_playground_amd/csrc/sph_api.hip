@@ -82,8 +82,8 @@ __global__ void k_from_sorted(const double *, const int64_t *, int, double *);
 __global__ void k_lincomb(int, double *, double, const double *, double, const double *, double, const double *,
                           double, const double *, double, const double *);
 __global__ void k_batch_merge(int, int, int64_t, int, const int *, const uint32_t *, const double *, const double4 *,
-                              const int64_t *, int, int *, uint32_t *, double *, double *, int64_t *, int *);
-template <bool FIRST>
+                              const int64_t *, int, int *, uint32_t *, double *, float *, double *, int64_t *, int *);
+template <bool FIRST, typename WT>
 __global__ void k_batch_stage(BatchStage);
 __global__ void k_batch_in(const double *, const int64_t *, const double *, int, double *, double *);
 }  // namespace sph
@@ -1573,6 +1573,8 @@ struct sph_batch {
   int *cnt = nullptr, *pid = nullptr;  // list lengths, problem of each row
   uint32_t *nb = nullptr;              // merged neighbour indices (list layout)
   double *W = nullptr, *rho = nullptr; // merged pair weights, densities
+  float *W32 = nullptr;                // merged pair weights in FP32 (precision 1)
+  int precision = 0;
   double *dt = nullptr;                // per-problem time steps
   int64_t *orig = nullptr;             // merged row -> position in the concatenated vectors
   double *vec = nullptr;               // Y0, three stage states, DY0, three Z (npad each)
@@ -1600,23 +1602,26 @@ void batch_super_step(sph_batch *b, int s, int block) {
   auto Z = [&](int j) -> double * { return (j == 0 || j == s) ? Z0 : ZR[j & 1]; };
   const unsigned G = (unsigned)((b->n + block - 1) / block);
   BatchStage a{};
-  a.cnt = b->cnt; a.nb = b->nb; a.W = b->W; a.rho = b->rho; a.pid = b->pid; a.dt = b->dt;
+  a.cnt = b->cnt; a.nb = b->nb; a.W = b->W; a.W32 = b->W32; a.rho = b->rho; a.pid = b->pid; a.dt = b->dt;
   a.Y0 = Y0; a.DY0 = DY0; a.n = (int)b->n; a.npad = (int)b->npad;
   a.Yjm1 = Y0; a.Yjm2 = Y0; a.Zjm1 = Z0; a.Yj = Y(1); a.Zj = Z(1);
   a.mhat = sts_mu_hat(1, s);
-  hipLaunchKernelGGL((k_batch_stage<true>), dim3(G), dim3(block), 0, b->stream, a);
+  const bool f32 = b->precision == 1;
+  if (f32) hipLaunchKernelGGL((k_batch_stage<true, float>), dim3(G), dim3(block), 0, b->stream, a);
+  else hipLaunchKernelGGL((k_batch_stage<true, double>), dim3(G), dim3(block), 0, b->stream, a);
   for (int j = 2; j <= s; j++) {
     a.Yjm1 = Y(j - 1); a.Yjm2 = Y(j - 2); a.Zjm1 = Z(j - 1); a.Yj = Y(j); a.Zj = Z(j);
     a.mu = sts_mu(j); a.nu = sts_nu(j); a.c0 = 1. - a.mu - a.nu;
     a.mhat = sts_mu_hat(j, s); a.ghat = sts_gm_hat(j, s);
-    hipLaunchKernelGGL((k_batch_stage<false>), dim3(G), dim3(block), 0, b->stream, a);
+    if (f32) hipLaunchKernelGGL((k_batch_stage<false, float>), dim3(G), dim3(block), 0, b->stream, a);
+    else hipLaunchKernelGGL((k_batch_stage<false, double>), dim3(G), dim3(block), 0, b->stream, a);
   }
 }
 
 void batch_free(sph_batch *b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void *p[] = {b->cnt, b->pid, b->nb, b->W, b->rho, b->dt, b->orig, b->vec};
+  void *p[] = {b->cnt, b->pid, b->nb, b->W, b->W32, b->rho, b->dt, b->orig, b->vec};
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1658,8 +1663,8 @@ int batch_autotune(sph_batch *b) {
 
 extern "C" {
 
-int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch **out) {
-  if (!ctxs || nprob < 1 || !K || !out) return SPH_ERR_BAD_ARG;
+int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, int precision, sph_batch **out) {
+  if (!ctxs || nprob < 1 || !K || !out || (precision != 0 && precision != 1)) return SPH_ERR_BAD_ARG;
   *out = nullptr;
   int64_t n = 0, np = 0;
   int maxc = 0;
@@ -1675,6 +1680,7 @@ int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch
   b->device = ctxs[0]->device;
   (void)hipSetDevice(b->device);
   b->nprob = nprob;
+  b->precision = precision;
   b->n = n;
   b->n_pairs = np;
   b->npad = (n + 63) / 64 * 64;
@@ -1685,7 +1691,8 @@ int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(SPH_ERR_HIP);
   const size_t slots = (size_t)b->npad * b->cap;
   if ((rc = dalloc(&b->cnt, b->npad)) || (rc = dalloc(&b->pid, b->npad)) || (rc = dalloc(&b->nb, slots)) ||
-      (rc = dalloc(&b->W, slots)) || (rc = dalloc(&b->rho, b->npad)) || (rc = dalloc(&b->dt, nprob)) ||
+      (rc = precision == 1 ? dalloc(&b->W32, slots) : dalloc(&b->W, slots)) || (rc = dalloc(&b->rho, b->npad)) ||
+      (rc = dalloc(&b->dt, nprob)) ||
       (rc = dalloc(&b->orig, b->npad)) || (rc = dalloc(&b->vec, kBatchVecs * (size_t)b->npad)))
     return fail(rc);
   if (hipMemset(b->cnt, 0, sizeof(int) * b->npad) != hipSuccess ||
@@ -1698,7 +1705,7 @@ int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, sph_batch
     ensure_weights(c, K + 6 * p);
     hipLaunchKernelGGL(k_batch_merge, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, (int)c->n, (int)c->npad,
                        b->off[p], p, c->cnt, c->nb, c->dW, (const double4 *)c->q[0], c->orig[c->cur],
-                       (int)b->npad, b->cnt, b->nb, b->W, b->rho, b->orig, b->pid);
+                       (int)b->npad, b->cnt, b->nb, b->W, b->W32, b->rho, b->orig, b->pid);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SPH_ERR_HIP);
     b->off[p + 1] = b->off[p] + c->n;
   }

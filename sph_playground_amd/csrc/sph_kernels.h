@@ -142,6 +142,7 @@ struct BatchStage {
   const int *cnt;
   const uint32_t *nb;
   const double *W, *rho;
+  const float *W32;          // FP32 weights (precision 1) instead of W
   const int *pid;
   const double *dt;
   const double *Yjm1, *Yjm2, *Y0, *Zjm1;

@@ -138,3 +138,30 @@ def test_batch_rejects_unprepared_and_bad_arguments(gpu):
     with pytest.raises(SphError):
         batch.sts(torch.zeros(batch.n_total, dtype=torch.float64, device="cuda"), 1, [1e-5], 1)  # s < 2
     batch.close()
+
+
+def test_batch_fp32_weights_close_to_fp64(gpu):
+    # FP32-stored pair weights (6e-8 relative each), FP64 products, sums and
+    # states: the change of T over the super-steps agrees with the FP64
+    # batch to 1e-5 of its size (the sum over a smooth field cancels to
+    # about h k of its terms, k the field's wavenumber); graph replays equal
+    # direct launches bit for bit
+    import torch
+    from sph_playground_amd.batch import Batch
+    probs, engines = _problems()
+    K = np.stack([K for _, _, K, _ in probs])
+    b64 = Batch(engines, K)
+    b32 = Batch(engines, K, precision="mixed")
+    s, nsteps = 6, 3
+    dts = [_dt(st, s) for _, st, _, _ in probs]
+    T0 = torch.from_numpy(np.concatenate([st[:, C.es_t - 1] for _, st, _, _ in probs])).cuda()
+    ref = b64.sts(T0, s, dts, nsteps)
+    got = b32.sts(T0, s, dts, nsteps, graph=False)
+    assert torch.equal(b32.sts(T0, s, dts, nsteps, graph=True), got)
+    for p in range(len(probs)):
+        r, g, t0 = (x[p].cpu().numpy() for x in (b64.split(ref), b32.split(got), b64.split(T0)))
+        change = np.abs(r - t0).max()
+        assert change > 0
+        assert np.abs(g - r).max() <= 1e-5 * change, (p, np.abs(g - r).max() / change)
+    b64.close()
+    b32.close()
