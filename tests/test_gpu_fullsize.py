@@ -1,0 +1,94 @@
+"""The bench workload at its full size (128^3 = 2,097,152 particles,
+P = 239,075,328 pairs): the HIP step against the oracle on the same input.
+
+The oracle finishes this size in seconds on the host's cores, so the FP64
+step is compared field by field (1e-12); neighbour sets, too many to compare
+as Python sets, are compared per particle through their length and an
+order-independent checksum of (j, image code) hashes.  The mixed path's
+density stays within 1e-5 of the FP64 oracle's.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sph_playground_amd import consts as C
+from sph_playground_amd import problems as P
+from tests import goldens as G
+
+pytestmark = pytest.mark.gpu
+
+N_SIDE = 128
+
+
+def set_checksums(off, j, code):
+    """Per particle: sum (mod 2^64) of a 64-bit mix of (j, code) over its
+    neighbours (independent of their order), and the list lengths."""
+    x = j.astype(np.uint64) * np.uint64(32) + code.astype(np.uint64)
+    x ^= x >> np.uint64(33)
+    x *= np.uint64(0xFF51AFD7ED558CCD)
+    x ^= x >> np.uint64(33)
+    x *= np.uint64(0xC4CEB9FE1A85EC53)
+    x ^= x >> np.uint64(33)
+    cnt = np.diff(off)
+    sums = np.zeros(cnt.shape[0], dtype=np.uint64)
+    nz = cnt > 0
+    sums[nz] = np.add.reduceat(x, off[:-1][nz])
+    return cnt, sums
+
+
+def test_checksum_detects_a_changed_pair():
+    # the checksum separates sets that differ in one neighbour or one code
+    off = np.array([0, 3, 5])
+    j = np.array([4, 9, 2, 7, 1], dtype=np.int32)
+    code = np.array([13, 13, 4, 13, 22], dtype=np.int8)
+    c0, s0 = set_checksums(off, j, code)
+    perm = np.array([2, 0, 1, 4, 3])
+    c1, s1 = set_checksums(off, j[perm], code[perm])
+    assert np.array_equal(c0, c1) and np.array_equal(s0, s1)  # order does not matter
+    j2 = j.copy(); j2[1] = 8
+    assert set_checksums(off, j2, code)[1][0] != s0[0]
+    code2 = code.copy(); code2[4] = 13
+    assert set_checksums(off, j, code2)[1][1] != s0[1]
+
+
+@pytest.fixture(scope="module")
+def fullsize():
+    state, st = P.diffusion3d(N_SIDE, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    print("oracle at %d particles" % st.shape[0], flush=True)
+    ref = oracle.iterate(state, st, kernel=1, tensor_mode=1, kappa_perp=1e-3)
+    return state, st, ref
+
+
+def test_fullsize_fp64_step_matches_oracle(gpu, fullsize):
+    from sph_playground_amd.engine import Engine
+    state, st, ref = fullsize
+    n = st.shape[0]
+    eng = Engine(state, kernel="m6", tensor="full", kappa_perp=1e-3)
+    got = st.copy()
+    eng.iterate(got)
+    errs = G.field_errors(ref["store"], got, n)
+    assert max(errs.values()) <= 1e-12, errs
+    s = eng.stats()
+    assert s["n_pairs"] == ref["off"][-1] == 114 * n
+    assert s["sweeps"] == ref["sweeps"]
+    off, j, code = eng.neighbours()
+    c_got, h_got = set_checksums(off, j, code)
+    c_ref, h_ref = set_checksums(ref["off"], ref["j"], ref["code"])
+    assert np.array_equal(c_got, c_ref)
+    bad = np.nonzero(h_got != h_ref)[0]
+    assert bad.size == 0, bad[:10]
+    eng.close()
+
+
+def test_fullsize_mixed_density_within_1e5(gpu, fullsize):
+    from sph_playground_amd.engine import Engine
+    state, st, ref = fullsize
+    n = st.shape[0]
+    eng = Engine(state, kernel="m6", precision="mixed", tensor="full", kappa_perp=1e-3)
+    got = st.copy()
+    eng.iterate(got)
+    den = ref["store"][:n, C.es_den - 1]
+    assert (np.abs(got[:n, C.es_den - 1] - den) / den).max() <= 1e-5
+    ddt = ref["store"][:n, C.es_ddt - 1]
+    assert np.abs(got[:n, C.es_ddt - 1] - ddt).max() <= 1e-3 * np.abs(ddt).max()
+    eng.close()
