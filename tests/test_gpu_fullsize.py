@@ -92,3 +92,77 @@ def test_fullsize_mixed_density_within_1e5(gpu, fullsize):
     ddt = ref["store"][:n, C.es_ddt - 1]
     assert np.abs(got[:n, C.es_ddt - 1] - ddt).max() <= 1e-3 * np.abs(ddt).max()
     eng.close()
+
+
+def _slab_vs_oracle(state, st, cap):
+    from sph_playground_amd.engine import Engine
+    n = st.shape[0]
+    ref = oracle.iterate(state, st, kernel=1, neigh_cap=cap)
+    eng = Engine(state, kernel="m6", neigh_cap=cap)
+    got = st.copy()
+    eng.iterate(got)
+    errs = G.field_errors(ref["store"], got, n)
+    assert max(errs.values()) <= 1e-12, errs
+    assert eng.stats()["n_pairs"] == ref["off"][-1]
+    off, j, code = eng.neighbours()
+    c_got, h_got = set_checksums(off, j, code)
+    c_ref, h_ref = set_checksums(ref["off"], ref["j"], ref["code"])
+    assert np.array_equal(c_got, c_ref) and np.array_equal(h_got, h_ref)
+    eng.close()
+
+
+# the shock-tube slab at its size: 1,048,576 uniform-random particles in
+# [-1,1] x [0,0.125]^2, periodic in y and z.  At this size some particles have
+# more neighbours than the reference's cap (raised on both sides), and with
+# FLD at hfac 1.0 one particle's Newton step goes negative in the oracle as in
+# the reference (NEG_H), so the FLD slab uses hfac 1.2
+SLAB_N = 1048576
+
+
+def test_fullsize_fld_slab_matches_oracle(gpu):
+    state, st = P.random_slab3d(SLAB_N, seed=5, hfac=1.2, fld=True, velocity_seed=2)
+    print("fld slab", st.shape[0], flush=True)
+    _slab_vs_oracle(state, st, 600)
+
+
+def test_fullsize_mhd_slab_matches_oracle(gpu):
+    state, st = P.random_slab3d(SLAB_N, seed=4, hfac=1.0, equations="mhd", velocity_seed=2)
+    state[C.ec_disotropic - 1] = 0.0
+    print("mhd slab", st.shape[0], flush=True)
+    _slab_vs_oracle(state, st, 400)
+
+
+def test_fullsize_adjoint_matches_central_differences(gpu):
+    # the differentiation workload at its size: 64^3 particles, 20 RKL2
+    # super-steps of 5 stages, checkpoints every 5, loss 0.5 |T - T*|^2 with a
+    # seeded target; gradients w.r.t. kappa_par, kappa_perp, the field angles
+    # and (along a random direction) T0 against central differences
+    import math
+    import torch
+    from sph_playground_amd import autograd as AG
+    from sph_playground_amd.engine import Engine
+    from tests.test_gpu_adjoint import _loss_and_grads
+    state, st = P.diffusion3d(64, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    eng = Engine(state, kernel="m6", tensor="full", kappa_perp=1e-3)
+    eng.load(st)
+    AG.prepare(eng)
+    n = st.shape[0]
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda()
+    target = torch.from_numpy(P.gaussian(5, n) * 0.1).cuda()
+    params = [1.0, 1e-3, math.pi / 2, math.pi / 4]
+    s, nsteps, every, dt = 5, 20, 5, 1e-6
+    _, g, gT0 = _loss_and_grads(AG, eng, T0, params, target, s, dt, nsteps, every)
+    eps = 1e-4
+    for k in range(4):
+        p1, p2 = list(params), list(params)
+        p1[k] += eps
+        p2[k] -= eps
+        fd = (_loss_and_grads(AG, eng, T0, p1, target, s, dt, nsteps, every, False)
+              - _loss_and_grads(AG, eng, T0, p2, target, s, dt, nsteps, every, False)) / (2 * eps)
+        print("param", k, g[k], fd, flush=True)
+        assert abs(g[k] - fd) <= 1e-6 * max(1.0, abs(fd)), (k, g[k], fd)
+    d = torch.from_numpy(np.random.default_rng(6).standard_normal(n)).cuda()
+    fd = (_loss_and_grads(AG, eng, T0 + eps * d, params, target, s, dt, nsteps, every, False)
+          - _loss_and_grads(AG, eng, T0 - eps * d, params, target, s, dt, nsteps, every, False)) / (2 * eps)
+    assert abs(float(torch.dot(gT0, d)) - fd) <= 1e-6 * max(1.0, abs(fd))
+    eng.close()
