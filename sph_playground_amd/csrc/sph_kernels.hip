@@ -772,8 +772,13 @@ __device__ __forceinline__ void tensor_contract(const Sym3<T> &kt, T rx, T ry, T
 
 // c2 (circuit2.f90:58-505) for one real particle per lane over the loop-1
 // neighbour list, post-loop-1 h/den/om and post-EOS p/c.
+// The FP64 diffusion-mode loop is latency-bound: at 176 VGPRs it runs 2 waves
+// per SIMD, capped to 168 it runs 3 (a few registers spill) and measured 17 %
+// faster; the other variants are fastest uncapped.
 template <int DIM, int KT, typename PT, typename V4, int MODE>
-__global__ __launch_bounds__(256) void k_loop2(Loop2Args<V4> a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((MODE == MODE_DIFF && std::is_same_v<PT, double>) ? 3 : 1)))
+void k_loop2(Loop2Args<V4> a) {
   int s = gtid();
   double e_part = 0.0;
   if (s < a.n && a.real[s]) {
