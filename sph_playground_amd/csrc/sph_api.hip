@@ -28,10 +28,10 @@ __global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, l
 __global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
 __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
-__global__ void k_cell_range(const uint32_t *, int, Grid, int2 *);
+__global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
                        uint8_t *, double4 *, double *, double4 *, double *, long long *);
-template <int DIM, int RS>
+template <int DIM>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
 template <int DIM, int KT, int GRAD, typename PT>
@@ -135,7 +135,7 @@ struct sph_ctx {
   uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  int2 *crange = nullptr;
+  int *cstart = nullptr;   // first sorted index of every cell key
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
   double *tr[2] = {nullptr, nullptr};  // T den of the loop-1 sweep state st[k]
@@ -328,10 +328,10 @@ static int ensure_nb(sph_ctx *c, int cap) {
   return SPH_OK;
 }
 
-static int ensure_cells(sph_ctx *c, int64_t ncell) {
-  if (ncell <= c->ncell_cap) return SPH_OK;
-  int64_t cap = std::max<int64_t>(ncell, 1024);
-  RCHK(dalloc(&c->crange, cap));
+static int ensure_cells(sph_ctx *c, int64_t nkeys) {
+  if (nkeys + 1 <= c->ncell_cap) return SPH_OK;
+  int64_t cap = std::max<int64_t>(nkeys + 1, 1024);
+  RCHK(dalloc(&c->cstart, cap));
   c->ncell_cap = cap;
   return SPH_OK;
 }
@@ -376,7 +376,7 @@ static long long stripe_min(const long long *h, int v) {
 
 // Stencil half-width of the neighbour search in cells (cells rmax/RS wide).
 #ifndef SPH_NEIGH_RS
-#define SPH_NEIGH_RS 1
+#define SPH_NEIGH_RS 2
 #endif
 static const int kNeighRS = SPH_NEIGH_RS;
 
@@ -404,12 +404,10 @@ static int build_neighbours(sph_ctx *c) {
   if (!(hmax > 0.0) && c->n_real > 0) return SPH_ERR_Q_NAN_OR_NEG;
   double rmax = c->frozen.krad * hmax * (1.0 + 1e-9);
   if (!(rmax > 0.0)) rmax = 1.0;
-  // 2. cell grid: cells at least rmax/RS wide, searched over +-RS cells;
-  //    imaged axes tile [emin, emax] and need 2 RS + 1 cells for every
-  //    (cell, image) pair to be distinct, else RS drops to 1.  Per-axis cell
-  //    counts stay within the 30-bit Morton key.  RS = kNeighRS by default:
-  //    one-radius cells measured faster than half-radius cells on MI355X (the
-  //    search is bound by cell-range loads and gathers, not by tests).
+  // 2. cell grid: cells at least rmax/RS wide; imaged axes tile
+  //    [emin, emax] and need 2 RS + 1 cells, else RS drops to 1.  The search
+  //    reads each (y, z) row as whole ranges along x, so half-radius cells
+  //    (RS = 2) follow the sphere closely at a few more row ranges.
   const int axis_cap = dim == 3 ? 1024 : (dim == 2 ? 32768 : (1 << 30));
   Grid g{};
   double cs[3];
@@ -459,13 +457,20 @@ static int build_neighbours(sph_ctx *c) {
     g.cs[a] = cs[a];
     g.inv_cs[a] = cs[a] > 0 ? 1.0 / cs[a] : 0.0;
   }
+  // sort keys: row (y, z) times nx plus x; 3D rows in 8 x 8 tiles of (y, z)
+  int64_t nrow = 1;
+  if (dim == 3) {
+    g.ty = (g.nc[1] + 7) / 8;
+    nrow = (int64_t)g.ty * ((g.nc[2] + 7) / 8) * 64;
+  } else if (dim == 2) {
+    nrow = g.nc[1];
+  }
+  const int64_t nkeys = nrow * g.nc[0];
   c->grid = g;
-  int64_t ncell = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
-  RCHK(ensure_cells(c, ncell));
-  int axis_bits = 1;
-  while ((1 << axis_bits) < std::max(g.nc[0], std::max(g.nc[1], g.nc[2]))) axis_bits++;
-  const int key_bits = std::min(32, axis_bits * dim);
-  // 3. Morton keys, sortedness, radix sort + reorder when needed
+  RCHK(ensure_cells(c, nkeys));
+  int key_bits = 1;
+  while (key_bits < 32 && ((int64_t)1 << key_bits) < nkeys) key_bits++;
+  // 3. cell keys, sortedness, radix sort + reorder when needed
   hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
   hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
   RCHK(read_ints(c, 2, 1));
@@ -497,8 +502,8 @@ static int build_neighbours(sph_ctx *c) {
     F = c->fields();
     c->stats.sorted = 1;
   }
-  HIPCHK(hipMemsetAsync(c->crange, 0, sizeof(int2) * ncell, c->stream));
-  hipLaunchKernelGGL(k_cell_range, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, g, c->crange);
+  hipLaunchKernelGGL(k_cell_start, dim3(grid_for(n + 1)), dim3(kBlock), 0, c->stream, c->keys, n, (uint32_t)nkeys,
+                     c->cstart);
   HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
@@ -509,7 +514,15 @@ static int build_neighbours(sph_ctx *c) {
     HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 2, c->stream));
     HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
     NeighArgs a;
-    a.hs = c->h0; a.real = c->real; a.crange = c->crange;
+    a.hs = c->h0; a.real = c->real; a.cstart = c->cstart;
+    // rows and cells within r_max (1 + 1e-6) of a cell; imaged axes at most
+    // one wrap away
+    int rc[3];
+    for (int d = 0; d < 3; d++) {
+      rc[d] = d < dim && cs[d] > 0 ? (int)std::ceil(rmax * (1.0 + 1e-6) / cs[d]) : 0;
+      if (g.periodic[d]) rc[d] = std::min(rc[d], g.nc[d]);
+    }
+    a.rx = rc[0]; a.ry = rc[1]; a.rz = rc[2];
     a.g = g; a.b = c->box; a.krad = c->frozen.krad; a.n = n; a.npad = (int)c->npad; a.cap = c->nb_cap;
     // FP32 search records: coordinates relative to the grid origin lie within
     // the grid span S (imaged axes: the box), images within 2 S; each of the
@@ -522,16 +535,10 @@ static int build_neighbours(sph_ctx *c) {
       a.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
     a.maxneib = c->maxneib; a.err = c->d_int; a.stripes = c->d_stripes;
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-#define NEIGH(D, R) hipLaunchKernelGGL((k_neigh<D, R>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
-    if (RS == 2) {
-      if (dim == 1) NEIGH(1, 2);
-      else if (dim == 2) NEIGH(2, 2);
-      else NEIGH(3, 2);
-    } else {
-      if (dim == 1) NEIGH(1, 1);
-      else if (dim == 2) NEIGH(2, 1);
-      else NEIGH(3, 1);
-    }
+#define NEIGH(D) hipLaunchKernelGGL((k_neigh<D>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
+    if (dim == 1) NEIGH(1);
+    else if (dim == 2) NEIGH(2);
+    else NEIGH(3);
 #undef NEIGH
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     // pair counts, list lengths and k_prep's flags in one read-back
@@ -926,7 +933,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->crange, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->jr[0], c->jr[1], c->pd, c->trd,
+                  c->sort_tmp, c->cstart, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->jr[0], c->jr[1], c->pd, c->trd,
                   c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,

@@ -49,7 +49,8 @@ enum { MODE_GENERIC = 0, MODE_DIFF = 1 };  // counters are striped over 64 slots
 struct NeighArgs {
   const double *hs;        // h used for the search radius (pre-loop-1 h)
   const uint8_t *real;     // 1 = real particle
-  const int2 *crange;      // [first, last) sorted index of every cell
+  const int *cstart;       // first sorted index of every cell key (nkeys + 1 entries)
+  int rx, ry, rz;          // cells within r_max (1 + 1e-6) of a cell along x, y, z
   Grid g;
   Box b;
   float lenf[3];           // box lengths in FP32 (image shifts of the search records)

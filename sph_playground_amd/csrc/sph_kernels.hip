@@ -118,6 +118,16 @@ __device__ __forceinline__ uint32_t morton_key(const int c[3], int mdim) {
   if (mdim == 2) return spread2(c[0]) | (spread2(c[1]) << 1);
   return (uint32_t)c[0];
 }
+// Sort key of a cell: its row (y, z) times nx plus x, so the cells of a row
+// are consecutive keys; in 3D the rows follow each other in 8 x 8 tiles of
+// the (y, z) plane, which keeps the neighbours of a block of particles close
+// in memory.
+__device__ __forceinline__ uint32_t row_of(const Grid &g, int cy, int cz) {
+  if (g.mdim == 3)
+    return ((uint32_t)(cz >> 3) * (uint32_t)g.ty + (uint32_t)(cy >> 3)) * 64u + (uint32_t)(((cz & 7) << 3) | (cy & 7));
+  return (uint32_t)cy;
+}
+
 __device__ __forceinline__ void morton_cell(uint32_t k, int mdim, int c[3]) {
   if (mdim == 3) { c[0] = compact3(k); c[1] = compact3(k >> 1); c[2] = compact3(k >> 2); }
   else if (mdim == 2) { c[0] = compact2(k); c[1] = compact2(k >> 1); c[2] = 0; }
@@ -189,7 +199,7 @@ __global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) 
   if (s >= n) return;
   int c[3];
   cell_of(g, F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], c);
-  keys[s] = morton_key(c, g.mdim);
+  keys[s] = row_of(g, c[1], c[2]) * (uint32_t)g.nc[0] + (uint32_t)c[0];
   vals[s] = (uint32_t)s;
 }
 
@@ -210,18 +220,15 @@ __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const in
   orig_dst[s] = orig_src[p];
 }
 
-// Particle range of every occupied cell (cells indexed row-major).
-__global__ void k_cell_range(const uint32_t *keys, int n, Grid g, int2 *crange) {
+// First sorted index of every cell key (cstart[K] = number of particles with
+// a key below K, for every K in [0, nkeys]); thread s fills the keys between
+// its predecessor's and its own.
+__global__ void k_cell_start(const uint32_t *keys, int n, uint32_t nkeys, int *cstart) {
   int s = gtid();
-  if (s >= n) return;
-  uint32_t k = keys[s];
-  bool first = (s == 0 || keys[s - 1] != k), last = (s == n - 1 || keys[s + 1] != k);
-  if (!first && !last) return;
-  int c[3];
-  morton_cell(k, g.mdim, c);
-  int rm = (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
-  if (first) crange[rm].x = s;
-  if (last) crange[rm].y = s + 1;
+  if (s > n) return;
+  const int64_t prev = s == 0 ? -1 : (int64_t)keys[s - 1];
+  const int64_t cur = s == n ? (int64_t)nkeys : (int64_t)keys[s];
+  for (int64_t k = prev + 1; k <= cur; k++) cstart[k] = s;
 }
 
 // Positions/masses, search h, the FP32 search record (coordinates relative
@@ -315,11 +322,13 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 }
 
 // findneighboursKDT (neighboursearch.f90:74-194) as a cell-list search:
-// every real i tests the particles of the (2 RS + 1)^DIM surrounding cells
-// (cells of about r_max / RS; cells wrap with an image shift along imaged
-// axes) with the kd-tree's FP64 inclusion rule sd <= (krad*h_i)^2; self
-// (unshifted) is dropped.  Cells whose box lies farther than the radius
-// (with a safety margin) are skipped.
+// particles are sorted so that the cells of one (y, z) row lie contiguously
+// along x.  Every real i visits the rows within r_max in y and z (rows wrap
+// with an image shift along imaged axes; rows whose box lies farther than its
+// radius are skipped) and, in each, the one contiguous range of records of
+// the cells its sphere reaches along x at that row's distance (one more
+// range per wrapped part), and tests them with the kd-tree's FP64 inclusion
+// rule sd <= (krad*h_i)^2; self (unshifted) is dropped.
 //
 // Candidates arrive as one 16-byte record each (pf: FP32 coordinates
 // relative to the grid origin, and the band bits), four at a time.  The FP32
@@ -332,7 +341,7 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 // pending list entries per lane in k_neigh (a power of two, whole chunks)
 constexpr int kNbRing = 16;
 
-template <int DIM, int RS>
+template <int DIM>
 __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
                                                const float4 *__restrict__ pf, uint32_t *__restrict__ nb,
                                                int *__restrict__ cnt) {
@@ -354,7 +363,7 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     const float hi_thr = finite_r ? (float)(r2 * (1.0 + 1e-5) + mabs) : HUGE_VALF;
     const float lo_thr = finite_r ? (float)(r2 * (1.0 - 1e-5) - mabs) : -HUGE_VALF;
     // entries [done, count) wait in the lane's ring and leave as whole
-    // 32-byte chunks: after each stencil cell (the same point for every
+    // 32-byte chunks: after each stencil row (the same point for every
     // lane, so the lanes' chunk stores share instructions), when the ring
     // is full, and the last partial chunk at the end
     uint32_t *ring = nb_ring + threadIdx.x * kNbRing;
@@ -369,98 +378,118 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
       }
       done += 8;
     };
+    // every candidate of one contiguous record range [t0, t1) of one image
+    // shift: FP32 decision, the exact FP64 rule inside the band
+    auto scan = [&](int t0, int t1, int sx, int sy, int sz) {
+      const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+      // the image shift in the FP32 frame, and the lane position in it
+      const float ox = pif.x - (float)sx * a.lenf[0], oy = pif.y - (float)sy * a.lenf[1],
+                  oz = pif.z - (float)sz * a.lenf[2];
+      auto test = [&](int tt, const float4 &q, bool ok) {
+        const float fx = q.x - ox, fy = q.y - oy, fz = q.z - oz;
+        const float sd32 = fx * fx + fy * fy + fz * fz;
+        bool hit = ok && sd32 <= hi_thr;
+        if (hit && !(sd32 < lo_thr)) {
+          const double4 pj = pm[tt];
+          const double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
+          const double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
+          const double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
+          hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
+        }
+        // every lane writes the next free ring slot and keeps it on a hit
+        // (at most kNbRing - 1 entries are pending here, so that slot
+        // holds none of them)
+        ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
+        count += hit ? 1 : 0;
+      };
+      // image existence as (band & bm) == bv; self (unshifted) excluded
+      uint32_t bm, bv;
+      image_mask(sx, sy, sz, bm, bv);
+      const int self_t = code == kNoImage ? s : -1;
+      // records t..t+3 (pf holds 4 records of padding past n; the ones
+      // past t1 belong to other cells and are masked)
+      for (int t = t0; t < t1; t += 4) {
+        const float4 *p = pf + t;
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = p[u];
+        if (t + 4 <= t1) {  // a full batch: no range test
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            test(t + u, q[u], (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+        }
+        // at most 11 pending before the next batch's 4 candidates
+        if (count - done > kNbRing - 5) flush_chunk();
+      }
+    };
     int ci[3];
     cell_of(a.g, pi.x, pi.y, pi.z, ci);
     const double *cs = a.g.cs;
-    const double px[3] = {pi.x, pi.y, pi.z};
-    const int rz = DIM == 3 ? RS : 0, ry = DIM >= 2 ? RS : 0;
+    const int nx = a.g.nc[0];
+    const int rz = DIM == 3 ? a.rz : 0, ry = DIM >= 2 ? a.ry : 0;
     for (int dz = -rz; dz <= rz; dz++) {
-      int czr = ci[2] + dz, cz = czr, sz = 0;
-      if (a.g.periodic[2]) {
-        if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
-        else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
-      } else if (cz < 0 || cz >= a.g.nc[2]) continue;
-      double gz = 0.0;
-      if (DIM == 3) {
-        double lo = a.g.lo[2] + czr * cs[2];
-        gz = fmax(fmax(lo - px[2], px[2] - (lo + cs[2])), 0.0);
-        gz *= gz;
-        if (gz > r2m) continue;
-      }
       for (int dy = -ry; dy <= ry; dy++) {
-        int cyr = ci[1] + dy, cy = cyr, sy = 0;
-        if (a.g.periodic[1]) {
-          if (cy < 0) { cy += a.g.nc[1]; sy = -1; }
-          else if (cy >= a.g.nc[1]) { cy -= a.g.nc[1]; sy = 1; }
-        } else if (cy < 0 || cy >= a.g.nc[1]) continue;
-        double gy = 0.0;
+        // the row (cy, cz) with its image shift; rows outside a non-imaged
+        // axis, or farther than the radius, are skipped
+        int cy = ci[1] + dy, cz = ci[2] + dz, sy = 0, sz = 0;
+        bool use = true;
+        double gyz = 0.0;
+        if (DIM == 3) {
+          const int czr = cz;
+          if (a.g.periodic[2]) {
+            if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
+            else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
+          } else if (cz < 0 || cz >= a.g.nc[2]) use = false;
+          const double lo = a.g.lo[2] + czr * cs[2];
+          const double gz = fmax(fmax(lo - pi.z, pi.z - (lo + cs[2])), 0.0);
+          gyz = gz * gz;
+        }
         if (DIM >= 2) {
-          double lo = a.g.lo[1] + cyr * cs[1];
-          gy = fmax(fmax(lo - px[1], px[1] - (lo + cs[1])), 0.0);
-          gy = gy * gy + gz;
-          if (gy > r2m) continue;
+          const int cyr = cy;
+          if (a.g.periodic[1]) {
+            if (cy < 0) { cy += a.g.nc[1]; sy = -1; }
+            else if (cy >= a.g.nc[1]) { cy -= a.g.nc[1]; sy = 1; }
+          } else if (cy < 0 || cy >= a.g.nc[1]) use = false;
+          const double lo = a.g.lo[1] + cyr * cs[1];
+          const double gy = fmax(fmax(lo - pi.y, pi.y - (lo + cs[1])), 0.0);
+          gyz += gy * gy;
         }
-        for (int dx = -RS; dx <= RS; dx++) {
-          int cxr = ci[0] + dx, cx = cxr, sx = 0;
+        use = use && gyz <= r2m;
+        if (use) {
+          // the sphere's x extent at this row (never tighter than the exact
+          // test): cells [xl, xh] before wrapping, at most one wrap away
+          const double hx = sqrt(fmax(r2m - gyz, 0.0)) * (1.0 + 1e-9);
+          int xl = (int)floor((pi.x - hx - a.g.lo[0]) * a.g.inv_cs[0]);
+          int xh = (int)floor((pi.x + hx - a.g.lo[0]) * a.g.inv_cs[0]);
+          if (!finite_r) { xl = ci[0] - a.rx; xh = ci[0] + a.rx; }
+          xl = max(xl, ci[0] - a.rx);
+          xh = min(xh, ci[0] + a.rx);
+          if (a.g.periodic[0]) {  // at most one wrap
+            xl = max(xl, -nx);
+            xh = min(xh, 2 * nx - 1);
+          }
+          const uint32_t row = row_of(a.g, cy, cz) * (uint32_t)nx;
           if (a.g.periodic[0]) {
-            if (cx < 0) { cx += a.g.nc[0]; sx = -1; }
-            else if (cx >= a.g.nc[0]) { cx -= a.g.nc[0]; sx = 1; }
-          } else if (cx < 0 || cx >= a.g.nc[0]) continue;
-          {
-            double lo = a.g.lo[0] + cxr * cs[0];
-            double gx = fmax(fmax(lo - px[0], px[0] - (lo + cs[0])), 0.0);
-            if (gx * gx + gy > r2m) continue;
-          }
-          const int c = (cz * a.g.nc[1] + cy) * a.g.nc[0] + cx;
-          const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
-          // the image shift in the FP32 frame, and the lane position in it
-          const float ox = pif.x - (float)sx * a.lenf[0], oy = pif.y - (float)sy * a.lenf[1],
-                      oz = pif.z - (float)sz * a.lenf[2];
-          const int2 tr = a.crange[c];
-          const int t0 = tr.x, t1 = tr.y;
-          // one candidate: FP32 decision, the exact FP64 rule inside the band
-          auto test = [&](int tt, const float4 &q, bool ok) {
-            const float fx = q.x - ox, fy = q.y - oy, fz = q.z - oz;
-            const float sd32 = fx * fx + fy * fy + fz * fz;
-            bool hit = ok && sd32 <= hi_thr;
-            if (hit && !(sd32 < lo_thr)) {
-              const double4 pj = pm[tt];
-              const double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
-              const double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
-              const double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
-              hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
+            if (xl < 0) {  // left of the box: images shifted by -L
+              const int l = xl + nx, h = min(xh, -1) + nx;
+              scan(a.cstart[row + l], a.cstart[row + h + 1], -1, sy, sz);
             }
-            // every lane writes the next free ring slot and keeps it on a hit
-            // (at most kNbRing - 1 entries are pending here, so that slot
-            // holds none of them)
-            ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
-            count += hit ? 1 : 0;
-          };
-          // image existence as (band & bm) == bv; self (unshifted) excluded
-          uint32_t bm, bv;
-          image_mask(sx, sy, sz, bm, bv);
-          const int self_t = code == kNoImage ? s : -1;
-          // records t..t+3 (pf holds 4 records of padding past n; the ones
-          // past t1 belong to other cells and are masked)
-          for (int t = t0; t < t1; t += 4) {
-            const float4 *p = pf + t;
-            float4 q[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) q[u] = p[u];
-            if (t + 4 <= t1) {  // a full batch: no range test
-#pragma unroll
-              for (int u = 0; u < 4; u++)
-                test(t + u, q[u], (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
-            } else {
-#pragma unroll
-              for (int u = 0; u < 4; u++)
-                test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+            const int l = max(xl, 0), h = min(xh, nx - 1);
+            if (l <= h) scan(a.cstart[row + l], a.cstart[row + h + 1], 0, sy, sz);
+            if (xh >= nx) {  // right of the box: images shifted by +L
+              const int l2 = max(xl, nx) - nx, h2 = xh - nx;
+              scan(a.cstart[row + l2], a.cstart[row + h2 + 1], 1, sy, sz);
             }
-            // at most 11 pending before the next batch's 4 candidates
-            if (count - done > kNbRing - 5) flush_chunk();
+          } else {
+            const int l = max(xl, 0), h = min(xh, nx - 1);
+            if (l <= h) scan(a.cstart[row + l], a.cstart[row + h + 1], 0, sy, sz);
           }
-          while (count - done >= 8) flush_chunk();
         }
+        while (count - done >= 8) flush_chunk();
       }
     }
     if (count > done) flush_chunk();  // slots past count in this chunk are never read
@@ -1234,15 +1263,12 @@ SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
 SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
 
-#define SPH_INST_NEIGH(DIM, RS)                                                                       \
-  template __global__ void k_neigh<DIM, RS>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
-                                            uint32_t *__restrict__, int *__restrict__);
-SPH_INST_NEIGH(1, 1)
-SPH_INST_NEIGH(2, 1)
-SPH_INST_NEIGH(3, 1)
-SPH_INST_NEIGH(1, 2)
-SPH_INST_NEIGH(2, 2)
-SPH_INST_NEIGH(3, 2)
+#define SPH_INST_NEIGH(DIM)                                                                      \
+  template __global__ void k_neigh<DIM>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
+                                        uint32_t *__restrict__, int *__restrict__);
+SPH_INST_NEIGH(1)
+SPH_INST_NEIGH(2)
+SPH_INST_NEIGH(3)
 template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
                                           double, double4 *, double4 *, double4 *, double4 *, double4 *, double4 *,
                                           double *, long long *);

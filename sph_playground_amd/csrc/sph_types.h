@@ -20,7 +20,8 @@ struct Grid {
   double cs[3];      // cell widths (the box of cell k spans lo + [k, k+1) cs)
   int nc[3];
   int periodic[3];  // 1 if the axis is imaged (cells wrap with a shift)
-  int mdim;         // axes interleaved into the Morton sort key (= dim)
+  int mdim;         // dimension of the sort key (= dim)
+  int ty;           // 3D: tiles of 8 rows along y in the sort key
 };
 
 // Per-call physics flags and constants (circuit1.f90:38-51, circuit2.f90:45-56,86-90).
