@@ -460,11 +460,11 @@ static int build_neighbours(sph_ctx *c) {
     g.cs[a] = cs[a];
     g.inv_cs[a] = cs[a] > 0 ? 1.0 / cs[a] : 0.0;
   }
-  // sort keys: row (y, z) times nx plus x; 3D rows in 8 x 8 tiles of (y, z)
+  // sort keys: row (y, z) times nx plus x; 3D rows in tiles of (y, z)
   int64_t nrow = 1;
   if (dim == 3) {
-    g.ty = (g.nc[1] + 7) / 8;
-    nrow = (int64_t)g.ty * ((g.nc[2] + 7) / 8) * 64;
+    g.ty = (g.nc[1] + kRowTile - 1) / kRowTile;
+    nrow = (int64_t)g.ty * ((g.nc[2] + kRowTile - 1) / kRowTile) * kRowTile * kRowTile;
   } else if (dim == 2) {
     nrow = g.nc[1];
   }

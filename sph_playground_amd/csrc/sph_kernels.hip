@@ -119,12 +119,14 @@ __device__ __forceinline__ uint32_t morton_key(const int c[3], int mdim) {
   return (uint32_t)c[0];
 }
 // Sort key of a cell: its row (y, z) times nx plus x, so the cells of a row
-// are consecutive keys; in 3D the rows follow each other in 8 x 8 tiles of
+// are consecutive keys; in 3D the rows follow each other in square tiles of
 // the (y, z) plane, which keeps the neighbours of a block of particles close
 // in memory.
 __device__ __forceinline__ uint32_t row_of(const Grid &g, int cy, int cz) {
+  constexpr int L = kRowTileLog2, T = kRowTile;
   if (g.mdim == 3)
-    return ((uint32_t)(cz >> 3) * (uint32_t)g.ty + (uint32_t)(cy >> 3)) * 64u + (uint32_t)(((cz & 7) << 3) | (cy & 7));
+    return ((uint32_t)(cz >> L) * (uint32_t)g.ty + (uint32_t)(cy >> L)) * (uint32_t)(T * T) +
+           (uint32_t)(((cz & (T - 1)) << L) | (cy & (T - 1)));
   return (uint32_t)cy;
 }
 

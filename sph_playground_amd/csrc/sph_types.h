@@ -21,8 +21,15 @@ struct Grid {
   int nc[3];
   int periodic[3];  // 1 if the axis is imaged (cells wrap with a shift)
   int mdim;         // dimension of the sort key (= dim)
-  int ty;           // 3D: tiles of 8 rows along y in the sort key
+  int ty;           // 3D: tiles of kRowTile rows along y in the sort key
 };
+
+// 3D sort key: (y, z) rows in tiles of kRowTile x kRowTile rows.
+#ifndef SPH_ROW_TILE_LOG2
+#define SPH_ROW_TILE_LOG2 3
+#endif
+constexpr int kRowTileLog2 = SPH_ROW_TILE_LOG2;
+constexpr int kRowTile = 1 << kRowTileLog2;
 
 // Per-call physics flags and constants (circuit1.f90:38-51, circuit2.f90:45-56,86-90).
 struct Phys {
