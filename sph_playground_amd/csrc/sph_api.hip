@@ -526,6 +526,9 @@ static int build_neighbours(sph_ctx *c) {
       if (g.periodic[d]) rc[d] = std::min(rc[d], g.nc[d]);
     }
     a.rx = rc[0]; a.ry = rc[1]; a.rz = rc[2];
+    // a position in cell units is FP32: its rounding stays below 2^-23 of the
+    // cell count along the axis
+    for (int d = 0; d < 3; d++) a.mcell[d] = (float)(1e-4 + 1e-6 * g.nc[d]);
     a.g = g; a.b = c->box; a.krad = c->frozen.krad; a.n = n; a.npad = (int)c->npad; a.cap = c->nb_cap;
     // FP32 search records: coordinates relative to the grid origin lie within
     // the grid span S (imaged axes: the box), images within 2 S; each of the

@@ -51,6 +51,7 @@ struct NeighArgs {
   const uint8_t *real;     // 1 = real particle
   const int *cstart;       // first sorted index of every cell key (nkeys + 1 entries)
   int rx, ry, rz;          // cells within r_max (1 + 1e-6) of a cell along x, y, z
+  float mcell[3];          // margin (cells) covering FP32 rounding of a position in cell units
   Grid g;
   Box b;
   float lenf[3];           // box lengths in FP32 (image shifts of the search records)

@@ -430,44 +430,52 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     };
     int ci[3];
     cell_of(a.g, pi.x, pi.y, pi.z, ci);
-    const double *cs = a.g.cs;
     const int nx = a.g.nc[0];
+    // row geometry in FP32 and in cell units, widened by margins that only
+    // add rows and cells: positions shrink towards the row by mcell, the
+    // squared radius grows by 2e-5
+    const float pcx = (float)((pi.x - a.g.lo[0]) * a.g.inv_cs[0]);
+    const float pcy = (float)((pi.y - a.g.lo[1]) * a.g.inv_cs[1]);
+    const float pcz = (float)((pi.z - a.g.lo[2]) * a.g.inv_cs[2]);
+    const float csy = (float)a.g.cs[1], csz = (float)a.g.cs[2], icx = (float)a.g.inv_cs[0];
+    const float r2g = finite_r ? (float)r2m * 1.00002f : HUGE_VALF;
     const int rz = DIM == 3 ? a.rz : 0, ry = DIM >= 2 ? a.ry : 0;
     for (int dz = -rz; dz <= rz; dz++) {
+      // the rows (cy, cz) with their image shifts; rows outside a non-imaged
+      // axis, or farther than the radius, are skipped
+      int cz = ci[2] + dz, sz = 0;
+      bool usez = true;
+      float gz2 = 0.f;
+      if (DIM == 3) {
+        const float t = (float)cz;
+        gz2 = fmaxf(fmaxf(t - pcz, pcz - (t + 1.f)) - a.mcell[2], 0.f) * csz;
+        gz2 *= gz2;
+        if (a.g.periodic[2]) {
+          if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
+          else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
+        } else if (cz < 0 || cz >= a.g.nc[2]) usez = false;
+      }
       for (int dy = -ry; dy <= ry; dy++) {
-        // the row (cy, cz) with its image shift; rows outside a non-imaged
-        // axis, or farther than the radius, are skipped
-        int cy = ci[1] + dy, cz = ci[2] + dz, sy = 0, sz = 0;
-        bool use = true;
-        double gyz = 0.0;
-        if (DIM == 3) {
-          const int czr = cz;
-          if (a.g.periodic[2]) {
-            if (cz < 0) { cz += a.g.nc[2]; sz = -1; }
-            else if (cz >= a.g.nc[2]) { cz -= a.g.nc[2]; sz = 1; }
-          } else if (cz < 0 || cz >= a.g.nc[2]) use = false;
-          const double lo = a.g.lo[2] + czr * cs[2];
-          const double gz = fmax(fmax(lo - pi.z, pi.z - (lo + cs[2])), 0.0);
-          gyz = gz * gz;
-        }
+        int cy = ci[1] + dy, sy = 0;
+        bool use = usez;
+        float gyz = gz2;
         if (DIM >= 2) {
-          const int cyr = cy;
+          const float t = (float)cy;
+          const float gy = fmaxf(fmaxf(t - pcy, pcy - (t + 1.f)) - a.mcell[1], 0.f) * csy;
+          gyz += gy * gy;
           if (a.g.periodic[1]) {
             if (cy < 0) { cy += a.g.nc[1]; sy = -1; }
             else if (cy >= a.g.nc[1]) { cy -= a.g.nc[1]; sy = 1; }
           } else if (cy < 0 || cy >= a.g.nc[1]) use = false;
-          const double lo = a.g.lo[1] + cyr * cs[1];
-          const double gy = fmax(fmax(lo - pi.y, pi.y - (lo + cs[1])), 0.0);
-          gyz += gy * gy;
         }
-        use = use && gyz <= r2m;
+        use = use && gyz <= r2g;
         if (use) {
-          // the sphere's x extent at this row (never tighter than the exact
-          // test): cells [xl, xh] before wrapping, at most one wrap away
-          const double hx = sqrt(fmax(r2m - gyz, 0.0)) * (1.0 + 1e-9);
-          int xl = (int)floor((pi.x - hx - a.g.lo[0]) * a.g.inv_cs[0]);
-          int xh = (int)floor((pi.x + hx - a.g.lo[0]) * a.g.inv_cs[0]);
-          if (!finite_r) { xl = ci[0] - a.rx; xh = ci[0] + a.rx; }
+          // the sphere's x extent at this row: cells [xl, xh] before
+          // wrapping, at most one wrap away (a radius that is not finite
+          // takes the whole stencil)
+          const float hxc = fminf(sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], (float)(a.rx + 1));
+          int xl = (int)floorf(pcx - hxc);
+          int xh = (int)floorf(pcx + hxc);
           xl = max(xl, ci[0] - a.rx);
           xh = min(xh, ci[0] + a.rx);
           if (a.g.periodic[0]) {  // at most one wrap
