@@ -342,6 +342,16 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 // the FP64 test.
 // pending list entries per lane in k_neigh (a power of two, whole chunks)
 constexpr int kNbRing = 16;
+static_assert(kNbRing == 16, "ring_slot masks a 64-byte ring");
+
+// byte offset of ring slot (c4 / 4) mod 16 in a lane's 64-byte-aligned ring
+// at ring_off: (c4 & 60) | ring_off in one instruction (the compiler splits
+// the same expression into an and and an add)
+__device__ __forceinline__ uint32_t ring_slot(int c4, uint32_t ring_off) {
+  uint32_t r;
+  asm("v_and_or_b32 %0, %1, 60, %2" : "=v"(r) : "v"(c4), "v"(ring_off));
+  return r;
+}
 
 template <int DIM>
 __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
@@ -369,6 +379,11 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     // lane, so the lanes' chunk stores share instructions), when the ring
     // is full, and the last partial chunk at the end
     uint32_t *ring = nb_ring + threadIdx.x * kNbRing;
+    // the ring position in bytes: c4 = 4 x entries found, so a slot's LDS
+    // offset is the lane's ring offset ORed with c4's low bits
+    const uint32_t ring_off = (uint32_t)threadIdx.x * (kNbRing * 4);
+    char *const ring_bytes = reinterpret_cast<char *>(nb_ring);
+    int c4 = 0;
     int done = 0;
     auto flush_chunk = [&]() {
       if (done < a.cap) {  // cap is whole chunks: the chunk fits
@@ -381,53 +396,59 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
       done += 8;
     };
     // every candidate of one contiguous record range [t0, t1) of one image
-    // shift: FP32 decision, the exact FP64 rule inside the band
-    auto scan = [&](int t0, int t1, int sx, int sy, int sz) {
+    // shift: FP32 decision, the exact FP64 rule inside the band.  IMG is
+    // false for the unshifted records, which always exist and never hold the
+    // lane's own record (the caller splits the range around it); IMG true
+    // tests the records' band bits for the image's existence.  One code path
+    // for full and partial batches keeps the wave's lanes together.
+    auto scan = [&](auto img, int t0, int t1, int sx, int sy, int sz) {
+      constexpr bool IMG = decltype(img)::value;
       const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
       // the image shift in the FP32 frame, and the lane position in it
       const float ox = pif.x - (float)sx * a.lenf[0], oy = pif.y - (float)sy * a.lenf[1],
                   oz = pif.z - (float)sz * a.lenf[2];
-      auto test = [&](int tt, const float4 &q, bool ok) {
-        const float fx = q.x - ox, fy = q.y - oy, fz = q.z - oz;
-        const float sd32 = fx * fx + fy * fy + fz * fz;
-        bool hit = ok && sd32 <= hi_thr;
-        if (hit && !(sd32 < lo_thr)) {
-          const double4 pj = pm[tt];
-          const double xj = image_coord(pj.x, sx, a.b.emin[0], a.b.emax[0]);
-          const double yj = image_coord(pj.y, sy, a.b.emin[1], a.b.emax[1]);
-          const double zj = image_coord(pj.z, sz, a.b.emin[2], a.b.emax[2]);
-          hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
-        }
-        // every lane writes the next free ring slot and keeps it on a hit
-        // (at most kNbRing - 1 entries are pending here, so that slot
-        // holds none of them)
-        ring[count & (kNbRing - 1)] = nb_pack((uint32_t)tt, code);
-        count += hit ? 1 : 0;
-      };
-      // image existence as (band & bm) == bv; self (unshifted) excluded
-      uint32_t bm, bv;
-      image_mask(sx, sy, sz, bm, bv);
-      const int self_t = code == kNoImage ? s : -1;
-      // records t..t+3 (pf holds 4 records of padding past n; the ones
-      // past t1 belong to other cells and are masked)
+      // image existence as (band & bm) == bv
+      uint32_t bm = 0, bv = 0;
+      if (IMG) image_mask(sx, sy, sz, bm, bv);
       for (int t = t0; t < t1; t += 4) {
+        // records t..t+3 (pf holds 4 records of padding past n; the ones
+        // past t1 belong to other cells and are masked)
+        const int rem = t1 - t;
         const float4 *p = pf + t;
         float4 q[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = p[u];
-        if (t + 4 <= t1) {  // a full batch: no range test
+        const uint32_t e0 = nb_pack((uint32_t)t, code);
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            test(t + u, q[u], (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
-        } else {
-#pragma unroll
-          for (int u = 0; u < 4; u++)
-            test(t + u, q[u], (t + u < t1) & (t + u != self_t) & ((__float_as_uint(q[u].w) & bm) == bv));
+        for (int u = 0; u < 4; u++) {
+          bool ok = u < rem;
+          if (IMG) ok = ok && (__float_as_uint(q[u].w) & bm) == bv;
+          const float fx = q[u].x - ox, fy = q[u].y - oy, fz = q[u].z - oz;
+          const float sd32 = fx * fx + fy * fy + fz * fz;
+          bool hit = ok && sd32 <= hi_thr;
+          if (hit && !(sd32 < lo_thr)) {
+            const double4 pj = pm[t + u];
+            double xj = pj.x, yj = pj.y, zj = pj.z;
+            if (IMG) {
+              xj = image_coord(xj, sx, a.b.emin[0], a.b.emax[0]);
+              yj = image_coord(yj, sy, a.b.emin[1], a.b.emax[1]);
+              zj = image_coord(zj, sz, a.b.emin[2], a.b.emax[2]);
+            }
+            hit = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
+          }
+          // every lane writes the next free ring slot and keeps it on a hit
+          // (at most kNbRing - 1 entries are pending here, so that slot
+          // holds none of them)
+          *reinterpret_cast<uint32_t *>(ring_bytes + ring_slot(c4, ring_off)) =
+              e0 + ((uint32_t)u << kCodeBits);
+          c4 += hit ? 4 : 0;
         }
         // at most 11 pending before the next batch's 4 candidates
-        if (count - done > kNbRing - 5) flush_chunk();
+        if ((c4 >> 2) - done > kNbRing - 5) flush_chunk();
       }
     };
+    using Img = std::true_type;
+    using Own = std::false_type;
     int ci[3];
     cell_of(a.g, pi.x, pi.y, pi.z, ci);
     const int nx = a.g.nc[0];
@@ -483,25 +504,33 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
             xh = min(xh, 2 * nx - 1);
           }
           const uint32_t row = row_of(a.g, cy, cz) * (uint32_t)nx;
-          if (a.g.periodic[0]) {
-            if (xl < 0) {  // left of the box: images shifted by -L
-              const int l = xl + nx, h = min(xh, -1) + nx;
-              scan(a.cstart[row + l], a.cstart[row + h + 1], -1, sy, sz);
+          if (a.g.periodic[0] && xl < 0) {  // left of the box: images shifted by -L
+            const int l = xl + nx, h = min(xh, -1) + nx;
+            scan(Img{}, a.cstart[row + l], a.cstart[row + h + 1], -1, sy, sz);
+          }
+          const int l = max(xl, 0), h = min(xh, nx - 1);
+          if (l <= h) {
+            const int t0 = a.cstart[row + l], t1 = a.cstart[row + h + 1];
+            if ((sy | sz) != 0) {
+              scan(Img{}, t0, t1, 0, sy, sz);
+            } else if (dy == 0 && dz == 0) {
+              // the lane's own row: the records on either side of its own
+              const int m = min(max(s, t0), t1);
+              scan(Own{}, t0, m, 0, 0, 0);
+              scan(Own{}, m == s ? m + 1 : m, t1, 0, 0, 0);
+            } else {
+              scan(Own{}, t0, t1, 0, 0, 0);
             }
-            const int l = max(xl, 0), h = min(xh, nx - 1);
-            if (l <= h) scan(a.cstart[row + l], a.cstart[row + h + 1], 0, sy, sz);
-            if (xh >= nx) {  // right of the box: images shifted by +L
-              const int l2 = max(xl, nx) - nx, h2 = xh - nx;
-              scan(a.cstart[row + l2], a.cstart[row + h2 + 1], 1, sy, sz);
-            }
-          } else {
-            const int l = max(xl, 0), h = min(xh, nx - 1);
-            if (l <= h) scan(a.cstart[row + l], a.cstart[row + h + 1], 0, sy, sz);
+          }
+          if (a.g.periodic[0] && xh >= nx) {  // right of the box: images shifted by +L
+            const int l2 = max(xl, nx) - nx, h2 = xh - nx;
+            scan(Img{}, a.cstart[row + l2], a.cstart[row + h2 + 1], 1, sy, sz);
           }
         }
-        while (count - done >= 8) flush_chunk();
+        while ((c4 >> 2) - done >= 8) flush_chunk();
       }
     }
+    count = c4 >> 2;
     if (count > done) flush_chunk();  // slots past count in this chunk are never read
     cnt[s] = count;
     if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
