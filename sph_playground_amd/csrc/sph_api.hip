@@ -34,6 +34,8 @@ __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, do
 template <int DIM>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
+template <int DIM, int KT, typename PT, typename V4>
+__global__ void k_loop2_fab(Loop2Args<V4>);
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1(Loop1Args<double4>);
 template <int DIM, int KT, int GRAD, typename PT>
@@ -651,12 +653,20 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
 template <int DIM, int KT, int MODE>
 static void launch_loop2(sph_ctx *c, void *args) {
   unsigned G = grid_for(c->n);
-  if (c->opt.precision == 1)
+  if constexpr (MODE == MODE_FAB) {
+    if (c->opt.precision == 1)
+      hipLaunchKernelGGL((k_loop2_fab<DIM, KT, float, float4>), dim3(G), dim3(kBlock), 0, c->stream,
+                         *(Loop2Args<float4> *)args);
+    else
+      hipLaunchKernelGGL((k_loop2_fab<DIM, KT, double, double4>), dim3(G), dim3(kBlock), 0, c->stream,
+                         *(Loop2Args<double4> *)args);
+  } else if (c->opt.precision == 1) {
     hipLaunchKernelGGL((k_loop2<DIM, KT, float, float4, MODE>), dim3(G), dim3(kBlock), 0, c->stream,
                        *(Loop2Args<float4> *)args);
-  else
+  } else {
     hipLaunchKernelGGL((k_loop2<DIM, KT, double, double4, MODE>), dim3(G), dim3(kBlock), 0, c->stream,
                        *(Loop2Args<double4> *)args);
+  }
 }
 template <int DIM, int MODE>
 static void launch_loop2_k(sph_ctx *c, void *args) {
@@ -719,7 +729,11 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   a.npad = (int)c->npad;
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
   HIPCHK(hipEventRecord(c->ev[5], c->stream));
+  // fab with one tensor for every pair and no velocity term: the loop
+  // without run-time choices
+  const bool fab = !generic && P.ddw == 400 && q1 == nullptr && q4 == nullptr;
   if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
+  else if (fab) launch_loop2_d<MODE_FAB>(c, P.dim, &a);
   else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
   HIPCHK(hipEventRecord(c->ev[6], c->stream));
   // energy partials and error bits come back with the step's closing sync

@@ -42,7 +42,9 @@ constexpr int kStripes = 64;
 // Loop-2 kernel variants.  MODE_GENERIC: every term under run-time flags.
 // MODE_DIFF: diffusion with a Hessian operator (fab/fw/n2w) and no
 // hydro/AV/MHD/FLD terms, the density+force step of the diffusion workloads.
-enum { MODE_GENERIC = 0, MODE_DIFF = 1 };  // counters are striped over 64 slots to avoid same-address atomics
+// MODE_FAB: MODE_DIFF with the fab operator, one tensor for every pair and
+// no velocity term (k_loop2_fab).
+enum { MODE_GENERIC = 0, MODE_DIFF = 1, MODE_FAB = 2 };  // counters are striped over 64 slots to avoid same-address atomics
 
 // Positions, search records, lists and counts travel as separate
 // __restrict__ kernel arguments.

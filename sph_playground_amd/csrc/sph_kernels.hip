@@ -1106,6 +1106,117 @@ void k_loop2(Loop2Args<V4> a) {
   }
 }
 
+// c2 (circuit2.f90:58-505) of a diffusion-only step with the fab Hessian
+// operator (hessian_fab_cart, kernel.f90:460-539) and one conduction tensor
+// for every pair: isotropic conduction, or every particle carrying the same B
+// bit for bit, and no velocity term (adden off, or every v exactly 0).  That
+// is k_loop2's MODE_DIFF with all of its run-time choices fixed, so none of
+// their branches remain in the pair loop; the other outputs of c2 are the
+// zeros MODE_DIFF writes.  FP64 evaluates the pair term with MODE_DIFF's
+// expressions; FP32 takes 1/|r| from one reciprocal square root, folds
+// (dim + 2) and the off-diagonal 2 into the tensor, and applies the
+// particle's constant factors -wcv / (h^(dim+1) rho_a) after the sum.
+template <int DIM, int KT, typename PT, typename V4>
+__global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
+  int s = gtid();
+  double e_part = 0.0;
+  if (s < a.n && a.real[s]) {
+    using K = KernelFn<KT, PT>;
+    constexpr bool F32 = std::is_same_v<PT, float>;
+    const Phys &P = a.p;
+    const double4 pa = a.pm[s];
+    const double *const *Fi = a.out.f;
+    const double ha_d = Fi[F_H][s];
+    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)Fi[F_DEN][s], oma = (PT)Fi[F_OM][s], ta = (PT)Fi[F_T][s];
+    const PT wcv = (PT)P.wcv;
+    const PT ihd1 = ipow<DIM + 1>(inv_ha);
+    const Sym3<PT> kt = tensor_of_b<PT>(P, (PT)Fi[F_BX][s], (PT)Fi[F_BY][s], (PT)Fi[F_BZ][s]);
+    const PT tra = ta * rhoa;
+    // FP32: (dim + 2) k with the off-diagonal terms doubled, and the trace
+    constexpr PT D2 = PT(DIM + 2);
+    const PT cxx = D2 * kt.xx, cyy = D2 * kt.yy, czz = D2 * kt.zz;
+    const PT cxy = PT(2) * D2 * kt.xy, cxz = PT(2) * D2 * kt.xz, cyz = PT(2) * D2 * kt.yz;
+    PT tr = kt.xx;
+    if (DIM >= 2) tr += kt.yy;
+    if (DIM == 3) tr += kt.zz;
+    PT ddt = 0;
+    bool qbad = !(inv_ha > PT(0));
+    const int nn = a.cnt[s];
+    // the FP32 record holds m/den and T den as FP32 in its fourth slot
+    struct J { double4 pb; double tr; uint32_t code; };
+    auto load = [&](uint32_t ent) {
+      J d;
+      const uint32_t j = nb_index(ent);
+      d.code = nb_code(ent);
+      d.pb = a.pd[j];
+      if constexpr (!F32) d.tr = a.trd[j];
+      return d;
+    };
+    for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
+      const double4 pb = d.pb;
+      double xb = pb.x, yb = pb.y, zb = pb.z;
+      if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
+      const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
+      if constexpr (F32) {
+        const float mb_rhob = f2_lo(pb.w), trb = f2_hi(pb.w);
+        const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+        const float inv_dr = __builtin_amdgcn_rsqf(r2);
+        const float qa = r2 * inv_dr * inv_ha;
+        const float dfa = K::df(qa);
+        qbad |= !(qa >= 0.f);
+        // (dim + 2) sum_lm k_lm r_l r_m over the dim x dim block
+        float sx = cxx * rx, sy = 0.f;
+        if (DIM >= 2) { sx = fmaf(cxy, ry, sx); sy = cyy * ry; }
+        if (DIM == 3) { sx = fmaf(cxz, rz, sx); sy = fmaf(cyz, rz, sy); }
+        float S = rx * sx;
+        if (DIM >= 2) S = fmaf(ry, sy, S);
+        if (DIM == 3) S = fmaf(czz * rz, rz, S);
+        const float x = fmaf(S, inv_dr * inv_dr, -tr);
+        ddt = fmaf(mb_rhob * (trb - tra), dfa * inv_dr * x, ddt);
+      } else {
+        const PT mb_rhob = (PT)pb.w, trb = (PT)d.tr;
+        const PT r2 = rx * rx + ry * ry + rz * rz;
+        const PT inv_r2 = rcp_pair(r2);
+        const PT dr = sqrt_pair(r2);
+        const PT inv_dr = dr * inv_r2;
+        const PT qa = dr * inv_ha;
+        const PT dfa = K::df(qa);
+        qbad |= !(qa >= PT(0));
+        PT S, Tr;
+        tensor_contract<DIM, PT>(kt, rx, ry, rz, inv_r2, S, Tr);
+        const PT half = -wcv * dfa * ihd1 * inv_dr;  // 0.5*Gab
+        const PT contr = half * (PT(DIM + 2) * S - Tr);
+        ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+      }
+    });
+    if constexpr (F32) ddt *= -wcv * ihd1 * rcp_pair(rhoa);
+    if (qbad) atomicOr(a.err, EB_Q);
+    const PT zero = 0;
+    double dh = 0.0;
+    if (P.adden) dh = -DIM * ha_d / (double)rhoa * (double)zero / (double)oma;
+    a.out.f[F_AX][s] = 0.0; a.out.f[F_AY][s] = 0.0; a.out.f[F_AZ][s] = 0.0;
+    a.out.f[F_DH][s] = dh;
+    a.out.f[F_DDT][s] = (double)ddt;
+    a.out.f[F_DBX][s] = 0.0; a.out.f[F_DBY][s] = 0.0; a.out.f[F_DBZ][s] = 0.0;
+    a.out.f[F_DU][s] = 0.0;
+    // FP64 energy sum (circuit2.f90:490-492) with zero dv, du, dB, drho
+    double vx = Fi[F_VX][s], vy = Fi[F_VY][s], vz = Fi[F_VZ][s];
+    double bx = Fi[F_BX][s], by = Fi[F_BY][s], bz = Fi[F_BZ][s], rh = Fi[F_DEN][s];
+    const double z0 = 0.0;
+    e_part = pa.w * (vx * z0 + vy * z0 + vz * z0 + z0 + (bx * z0 + by * z0 + bz * z0) / rh -
+                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * z0 / Fi[F_OM][s]);
+  }
+  __shared__ double red[4];
+  double w = wave_sum(e_part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i];
+    a.energy_part[xcd_block(blockIdx.x, gridDim.x)] = t;
+  }
+}
+
 // ------------------------------------------------------------- transfers
 // AoS store rows (any leading dimension) <-> SoA fields in sorted order.
 __global__ void k_aos_to_soa(const double *aos, int64_t ld, int n, Fields F, int64_t *orig) {
@@ -1296,7 +1407,9 @@ __global__ void k_axpy_copy(double *dst, const double *x, const double *y, doubl
   template __global__ void k_loop2<DIM, KT, double, double4, MODE_GENERIC>(Loop2Args<double4>); \
   template __global__ void k_loop2<DIM, KT, float, float4, MODE_GENERIC>(Loop2Args<float4>);    \
   template __global__ void k_loop2<DIM, KT, double, double4, MODE_DIFF>(Loop2Args<double4>);    \
-  template __global__ void k_loop2<DIM, KT, float, float4, MODE_DIFF>(Loop2Args<float4>);
+  template __global__ void k_loop2<DIM, KT, float, float4, MODE_DIFF>(Loop2Args<float4>);       \
+  template __global__ void k_loop2_fab<DIM, KT, double, double4>(Loop2Args<double4>);         \
+  template __global__ void k_loop2_fab<DIM, KT, float, float4>(Loop2Args<float4>);
 
 SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)

@@ -111,6 +111,51 @@ def test_diffusion3d_operators_vs_oracle(gpu, ddw):
     _oracle_compare(state, st, tensor_mode=1, kappa_perp=0.1)
 
 
+def _diffusion1d(n=96, kernel=1):
+    # 1D diffusion on periodic [0, 1]: T = sin(2 pi x), isotropic conduction
+    box = (0.0, 1.0, 0.0, 0.0, 0.0, 0.0)
+    pos, dx = P.lattice_positions(1, n, box)
+    st = P.empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = 1.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = 1.2 * dx
+    st[:, C.es_t - 1] = np.sin(2 * np.pi * pos[:, 0])
+    st[:, C.es_kappa - 1] = 1.0
+    st[:, C.es_bx - 1] = 1.0
+    state = P.make_state(dim=1, equations="diffusion", ddw="fab", artts="no", disotropic="yes",
+                         hfac=1.2, box=box, spacing=dx, n_real=n, kernel=kernel)
+    return state, st
+
+
+@pytest.mark.parametrize("dim,kernel", [(1, 1), (1, 0), (2, 0), (2, 2), (3, 0)])
+def test_fab_fixed_tensor_loop_vs_oracle(gpu, dim, kernel):
+    # loop 2 without run-time choices (fab, one tensor for every pair, no
+    # velocity term) in every dimension and kernel family: FP64 within 1e-12,
+    # mixed within 1e-5 (density) and 1e-3 (ddt) of the oracle
+    kw = {}
+    if dim == 1:
+        state, st = _diffusion1d(kernel=kernel)
+    elif dim == 2:
+        state, st = P.diffusion2d(24, ddw="fab", disotropic="no", b=(1.0, 0.5, 0.0), kernel=kernel,
+                                  seed=3, jitter=0.05)
+        kw = dict(tensor_mode=1, kappa_perp=0.1)
+    else:
+        state, st = P.diffusion3d(10, ddw="fab", b=(1.0, 0.3, 0.2), kernel=kernel, h_scale=1.03)
+        kw = dict(tensor_mode=1, kappa_perp=1e-2)
+    print("case dim", dim, "kernel", kernel, flush=True)
+    eng, ref, got = _oracle_compare(state, st, kernel=kernel, **kw)
+    nr = int(state[C.ec_realpn - 1])
+    ekw = dict(tensor="full", kappa_perp=kw["kappa_perp"]) if kw else {}
+    mix = _engine(state, kernel, precision="mixed", **ekw)
+    gm = st.copy()
+    mix.iterate(gm)
+    den = ref["store"][:nr, C.es_den - 1]
+    assert (np.abs(gm[:nr, C.es_den - 1] - den) / den).max() <= 1e-5
+    ddt = ref["store"][:nr, C.es_ddt - 1]
+    assert np.abs(gm[:nr, C.es_ddt - 1] - ddt).max() <= 1e-3 * np.abs(ddt).max()
+
+
 def test_mhd_fld_slab_vs_oracle(gpu):
     state, st = P.random_slab3d(4000, seed=5, hfac=1.0, fld=True, velocity_seed=2)
     _oracle_compare(state, st)
