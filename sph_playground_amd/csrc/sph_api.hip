@@ -153,6 +153,7 @@ struct sph_ctx {
   long long step_flags = 0;       // k_prep's v / B flags of this step
   double *h_energy = nullptr;     // pinned loop-2 energy partials (energy_cap)
   bool loop2_pending = false;     // loop 2's read-back still to be taken
+  bool neigh_pending = false;     // the search's read-back still to be taken (loop 1's first sync brings it)
   int *cnt = nullptr;
   void *q[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *energy_part = nullptr;
@@ -162,7 +163,7 @@ struct sph_ctx {
   int *d_int = nullptr;  // 0 err, 1 maxcnt, 2 sorted flag, 3 pos, 8.. sweep counters (max_sweeps+2)
   long long *d_stripes = nullptr;         // striped counters (kStripes slots per value)
   int *h_int = nullptr;                   // pinned mirror of d_int
-  long long *h_ll = nullptr;              // pinned scratch for small host<->device values (1024)
+  long long *h_ll = nullptr;              // pinned scratch for small host<->device values (2048)
   double *d_staging = nullptr;
   int64_t staging_bytes = 0;
   double4 *final_st = nullptr;
@@ -382,7 +383,37 @@ static long long stripe_min(const long long *h, int v) {
 #endif
 static const int kNeighRS = SPH_NEIGH_RS;
 
-static int build_neighbours(sph_ctx *c) {
+// The search's read-back (pair counts, list lengths, k_prep's flags and the
+// error bits) can be left to a later synchronisation: copies to these pinned
+// slots, taken by finish_neighbours.
+constexpr int kNeighLL = 1024;   // h_ll slots of the deferred read-back
+constexpr int kNeighInt = 16;    // h_int slot of its error bits
+constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
+
+static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
+  c->step_flags = 0;
+  for (int k = 0; k < kStripes; k++) c->step_flags |= ll[kFlagSlots + k];
+  c->stats.n_pairs = stripe_sum(ll, 0);
+  int maxcnt = (int)stripe_max(ll, 1);
+  c->last_maxcnt = maxcnt;
+  c->stats.ms_k_neigh = elapsed(c, c->ev[4], c->ev[7]);
+  if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
+  if (!(bits & EB_OVERFLOW)) return SPH_OK;
+  RCHK(ensure_nb(c, maxcnt + 8));
+  return kRedoNeigh;
+}
+
+// A deferred read-back, once the stream has passed its copies.
+static int take_neighbours(sph_ctx *c) {
+  if (!c->neigh_pending) return SPH_OK;
+  c->neigh_pending = false;
+  return finish_neighbours(c, c->h_ll + kNeighLL, c->h_int[kNeighInt]);
+}
+
+// defer: launch the search and its read-back copies without waiting; the
+// lists then hold at most nb_cap entries per particle, so what runs on them
+// before take_neighbours stays in bounds.
+static int build_neighbours(sph_ctx *c, bool defer = false) {
   const int n = (int)c->n;
   const int dim = c->frozen.dim;
   Fields F = c->fields();
@@ -542,27 +573,27 @@ static int build_neighbours(sph_ctx *c) {
     for (int d = 0; d < 3; d++)  // only imaged axes shift (elsewhere the shift is 0, keep 0 * lenf finite)
       a.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
     a.maxneib = c->maxneib; a.err = c->d_int; a.stripes = c->d_stripes;
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    // the search's own events (loop 1 reuses 5 and 6 before a deferred read-back is taken)
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
 #define NEIGH(D) hipLaunchKernelGGL((k_neigh<D>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
     if (dim == 1) NEIGH(1);
     else if (dim == 2) NEIGH(2);
     else NEIGH(3);
 #undef NEIGH
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // pair counts, list lengths and k_prep's flags in one read-back
+    if (defer) {
+      HIPCHK(hipMemcpyAsync(c->h_ll + kNeighLL, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_int + kNeighInt, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      c->neigh_pending = true;
+      return SPH_OK;
+    }
     HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
                           hipMemcpyDeviceToHost, c->stream));
     RCHK(read_ints(c, 0, 1));
-    c->step_flags = 0;
-    for (int k = 0; k < kStripes; k++) c->step_flags |= c->h_ll[kFlagSlots + k];
-    c->stats.n_pairs = stripe_sum(c->h_ll, 0);
-    int maxcnt = (int)stripe_max(c->h_ll, 1);
-    c->last_maxcnt = maxcnt;
-    c->stats.ms_k_neigh = elapsed(c, c->ev[5], c->ev[6]);
-    int bits = c->h_int[0];
-    if (bits & EB_TRUNC) return SPH_ERR_NEIGH_TRUNCATED;
-    if (!(bits & EB_OVERFLOW)) return SPH_OK;
-    RCHK(ensure_nb(c, maxcnt + 8));
+    const int fin = finish_neighbours(c, c->h_ll, c->h_int[0]);
+    if (fin != kRedoNeigh) return fin;
   }
   return SPH_ERR_HIP;
 }
@@ -615,6 +646,7 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
     c->final_st = c->st[0];
     c->stats.sweeps = 0;
     RCHK(read_ints(c, 0, 1));
+    RCHK(take_neighbours(c));
     c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
     c->stats.loop1_pairs = c->stats.n_pairs;
     return map_err(c->h_int[0]);
@@ -640,6 +672,7 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
     HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
                           c->stream));
     RCHK(read_ints(c, 0, 1));
+    RCHK(take_neighbours(c));
     c->stats.ms_k_loop1 += elapsed(c, c->ev[5], c->ev[6]);
     c->stats.loop1_pairs += stripe_sum(c->h_ll, 1);
     if (c->h_int[0]) break;  // circuit1.f90:202-204 stops inside the sweep
@@ -707,7 +740,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   // k_prep's flags (v and B are what they were there): the diffusion-only
   // loop 2 reads v only where it is non-zero and B only where it varies
   const long long fl = c->step_flags;
-  V4 *q0 = (V4 *)c->q[0];
+  // q0 feeds the generic loop, and in FP64 also the adjoint and batched
+  // operators built after the step; the FP32 diffusion loops read pd alone
+  V4 *q0 = std::is_same_v<V4, float4> && !generic ? nullptr : (V4 *)c->q[0];
   V4 *q1 = want_v && (generic || (fl & 1)) ? (V4 *)c->q[1] : nullptr;
   V4 *q2 = want_q2 ? (V4 *)c->q[2] : nullptr;
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
@@ -775,9 +810,24 @@ static int run_step(sph_ctx *c, const double *state) {
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
-  RCHK(build_neighbours(c));
+  // loop 1 follows the search without a host round trip; its first
+  // synchronisation brings the search's read-back, and lists that overflowed
+  // send the step back to a full search with room for them
+  RCHK(build_neighbours(c, true));
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
-  RCHK(run_loop1(c, P));
+  int r1 = run_loop1(c, P);
+  if (r1 == SPH_OK && c->neigh_pending) {  // loop 1 ran no sweep (no real particles)
+    HIPCHK(hipStreamSynchronize(c->stream));
+    r1 = take_neighbours(c);
+  }
+  if (r1 == kRedoNeigh) {
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    RCHK(build_neighbours(c));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    r1 = run_loop1(c, P);
+  }
+  c->neigh_pending = false;
+  RCHK(r1);
   HIPCHK(hipEventRecord(c->ev[2], c->stream));
   int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
   if (rc == SPH_OK && c->gravity)
@@ -939,7 +989,7 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
       dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
       dev_alloc((void **)&c->d_stripes, 8 * kStripes * sizeof(long long)) != SPH_OK ||
       hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_ll, 1024 * sizeof(long long)) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_ll, 2048 * sizeof(long long)) != hipSuccess) {
     sph_ctx_destroy(c);
     return SPH_ERR_NOMEM;
   }

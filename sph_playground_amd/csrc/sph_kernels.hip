@@ -223,14 +223,30 @@ __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const in
 }
 
 // First sorted index of every cell key (cstart[K] = number of particles with
-// a key below K, for every K in [0, nkeys]); thread s fills the keys between
-// its predecessor's and its own.
-__global__ void k_cell_start(const uint32_t *keys, int n, uint32_t nkeys, int *cstart) {
-  int s = gtid();
-  if (s > n) return;
-  const int64_t prev = s == 0 ? -1 : (int64_t)keys[s - 1];
-  const int64_t cur = s == n ? (int64_t)nkeys : (int64_t)keys[s];
-  for (int64_t k = prev + 1; k <= cur; k++) cstart[k] = s;
+// a key below K, for every K in [0, nkeys]); index s fills the keys between
+// its predecessor's and its own.  Short ranges are filled by their own lane;
+// long ones (empty rows: the tiles of the key order pad each axis to a
+// multiple of their side) by the whole wave, one range at a time.
+__global__ __launch_bounds__(256) void k_cell_start(const uint32_t *keys, int n, uint32_t nkeys, int *cstart) {
+  const int s = gtid();
+  const int lane = threadIdx.x & 63;
+  int64_t lo = 0, hi = -1;  // keys [lo, hi] take the value s
+  if (s <= n) {
+    lo = (s == 0 ? -1 : (int64_t)keys[s - 1]) + 1;
+    hi = s == n ? (int64_t)nkeys : (int64_t)keys[s];
+  }
+  constexpr int64_t kShort = 8;
+  const bool lng = hi - lo >= kShort;
+  if (!lng)
+    for (int64_t k = lo; k <= hi; k++) cstart[k] = s;
+  unsigned long long m = __ballot(lng);
+  while (m) {
+    const int l = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    const int64_t l0 = __shfl(lo, l), l1 = __shfl(hi, l);
+    const int v = __shfl(s, l);
+    for (int64_t k = l0 + lane; k <= l1; k += 64) cstart[k] = v;
+  }
 }
 
 // Positions/masses, search h, the FP32 search record (coordinates relative
@@ -532,7 +548,7 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     }
     count = c4 >> 2;
     if (count > done) flush_chunk();  // slots past count in this chunk are never read
-    cnt[s] = count;
+    cnt[s] = min(count, a.cap);  // an overflowed list is searched again; what reads it first stays in bounds
     if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
     if (count > a.cap) atomicOr(a.err, EB_OVERFLOW);
   } else if (s < a.n) {
@@ -774,7 +790,7 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;
     }
-    q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_T][s]};
+    if (q0) q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_T][s]};
     if (q1) q1[s] = V4{(T)vx, (T)vy, (T)vz, (T)F.f[F_P][s]};
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
