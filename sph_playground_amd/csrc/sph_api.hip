@@ -425,8 +425,9 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = v < 3 ? BIG : SMALL;
   HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7 * kStripes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
-  hipLaunchKernelGGL(k_wrap_reduce, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box,
-                     dim, c->d_red, c->d_int);
+  // at most 1024 blocks (four per CU), each striding over the particles
+  hipLaunchKernelGGL(k_wrap_reduce, dim3(std::min(G, 1024u)), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n,
+                     c->n_real, c->box, dim, c->d_red, c->d_int);
   long long *red = c->h_ll + 512;
   HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7 * kStripes, hipMemcpyDeviceToHost, c->stream));
   RCHK(read_ints(c, 0, 1));

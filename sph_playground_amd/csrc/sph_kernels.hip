@@ -152,12 +152,14 @@ __device__ __forceinline__ void cell_of(const Grid &g, double x, double y, doubl
 __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *orig, int n, int64_t n_real, Box b,
                                                      int dim, long long *red, int *err) {
   __shared__ long long lds7[7 * 4];  // per-wave partials of the 256-thread block
-  int s = gtid();
   long long mn[3], mx[3], hm;
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
   for (int a = 0; a < 3; a++) { mn[a] = BIG; mx[a] = SMALL; }
   hm = SMALL;
-  if (s < n) {
+  bool outside = false;
+  // grid-stride: a few blocks' worth of particles per block keeps the
+  // blocks' closing atomics few
+  for (int s = gtid(); s < n; s += (int)(gridDim.x * blockDim.x)) {
     bool real = orig[s] < n_real;
     double x[3] = {F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s]};
     if (real && b.wrap_mask) {
@@ -172,10 +174,18 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
       }
     }
     for (int a = 0; a < dim; a++)
-      if (((b.ghost_mask >> a) & 1) && !(x[a] >= b.emin[a] && x[a] <= b.emax[a])) atomicOr(err, EB_OUTSIDE);
-    for (int a = 0; a < 3; a++) { mn[a] = mx[a] = d2ord(x[a]); }
-    if (real) hm = d2ord(F.f[F_H][s]);
+      if (((b.ghost_mask >> a) & 1) && !(x[a] >= b.emin[a] && x[a] <= b.emax[a])) outside = true;
+    for (int a = 0; a < 3; a++) {
+      const long long o = d2ord(x[a]);
+      mn[a] = o < mn[a] ? o : mn[a];
+      mx[a] = o > mx[a] ? o : mx[a];
+    }
+    if (real) {
+      const long long o = d2ord(F.f[F_H][s]);
+      hm = o > hm ? o : hm;
+    }
   }
+  if (outside) atomicOr(err, EB_OUTSIDE);
   // the seven block reductions in one pass: per wave, then one barrier, then
   // thread v combines value v over the block's waves
   long long w7[7] = {wave_min(mn[0]), wave_min(mn[1]), wave_min(mn[2]), wave_max(mx[0]), wave_max(mx[1]),
@@ -614,7 +624,11 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
       PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
-      PT dr = sqrt_pair(rsq);
+      // FP32 with grad T: 1/|r| from one reciprocal square root, |r| from it
+      constexpr bool RSQ = GRAD != 0 && std::is_same_v<PT, float>;
+      PT inv_dr_rsq = 0;
+      if constexpr (RSQ) inv_dr_rsq = __builtin_amdgcn_rsqf(rsq);
+      PT dr = RSQ ? rsq * inv_dr_rsq : sqrt_pair(rsq);
       PT q = dr * inv_ha;
       PT f, df;
       K::f_df(q, f, df);
@@ -625,7 +639,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
         const double4 sb = d.sb;
-        PT inv_dr = rcp_pair(dr);
+        PT inv_dr = RSQ ? inv_dr_rsq : rcp_pair(dr);
         if constexpr (GRAD == 1) {
           PT trb;
           if constexpr (PACK) trb = f2_hi(d.pb.w);
