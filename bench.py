@@ -35,7 +35,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the 128^3 workload's kernels, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the counters read
 # for a plain 1 GiB copy (x2 and x1): profiles/hbm_traffic_diffusion3d_128cube.md
-MEASURED_TRAFFIC_128 = {"neigh": 1.546e9, "loop1_mixed": 1.503e9, "loop2_mixed": 1.658e9}
+MEASURED_TRAFFIC_128 = {"neigh": 1.204e9, "loop1_mixed": 1.514e9, "loop2_mixed": 1.631e9}
 
 
 def log(msg):
@@ -283,7 +283,7 @@ def main():
     if dom == "ms_k_loop2":
         flops = pairs * FLOP_PER_PAIR_LOOP2_FAB
         ach = flops / (kmx[dom] * 1e-3) / 1e12
-        roof = dict(kernel="k_loop2<3,M6,float> (loop 2, fab, full tensor)", bound="valu", achieved=ach,
+        roof = dict(kernel="k_loop2_fab<3,M6,float> (loop 2, fab, full tensor)", bound="valu", achieved=ach,
                     peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=traffic("loop2_mixed"),
                     flop_per_pair=FLOP_PER_PAIR_LOOP2_FAB, pairs_per_launch=pairs, ms_per_launch=kmx[dom])
     elif dom == "ms_k_loop1":
@@ -298,7 +298,7 @@ def main():
         # neighbour build: algorithmic bytes = read x,y,z,m,h (40 B) + write the list (4 B/pair) + count
         byts = n * 44 + pairs * 4
         ach = byts / (kmx[dom] * 1e-3) / 1e9
-        roof = dict(kernel="k_neigh<3,1> (cell-list neighbour build)", bound="hbm", achieved=ach,
+        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm", achieved=ach,
                     peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=traffic("neigh"),
                     bytes_per_launch=byts, ms_per_launch=kmx[dom])
 

@@ -156,6 +156,24 @@ def test_fab_fixed_tensor_loop_vs_oracle(gpu, dim, kernel):
     assert np.abs(gm[:nr, C.es_ddt - 1] - ddt).max() <= 1e-3 * np.abs(ddt).max()
 
 
+@pytest.mark.parametrize("process,ics", [(C.epc_borderless, 0), (C.epc_backcompatibility, C.ett_shock12),
+                                         (C.epc_backcompatibility, C.ett_pulse),
+                                         (C.epc_backcompatibility, C.ett_soundwave),
+                                         (C.epc_backcompatibility, C.ett_hydroshock),
+                                         (C.epc_backcompatibility, C.ett_mti),
+                                         (C.epc_backcompatibility, C.ett_mtilowres),
+                                         (C.epc_backcompatibility, C.ett_OTvortex)])
+def test_iterate_process_branches_vs_oracle(gpu, process, ics):
+    # every branch of iterate (iterator.f90:64-175): which axes wrap and carry
+    # images, EOS adiabatic / isothermal / none, the mti gravity terms
+    state, st = P.hydro2d(16)
+    state[C.ec_process - 1] = process
+    if ics:
+        state[C.ec_ics - 1] = ics
+    print("case process", process, "ics", ics, flush=True)
+    _oracle_compare(state, st)
+
+
 def test_mhd_fld_slab_vs_oracle(gpu):
     state, st = P.random_slab3d(4000, seed=5, hfac=1.0, fld=True, velocity_seed=2)
     _oracle_compare(state, st)
