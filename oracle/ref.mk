@@ -7,8 +7,11 @@
 #
 # Only the modules on the two-loop path are compiled (the reference Makefile's
 # SRC list minus dumper/printer/IC/args/err*/main, which need HDF5 or are not on
-# the path).  The reference Makefile itself is not run: it hard-codes gfortran
-# and a Homebrew HDF5 path.
+# the path); args.f90 (the ./execute flag parser, which needs only const and
+# state) is built for oracle/args_harness.f90.  The reference Makefile itself
+# is not run: it hard-codes gfortran and a Homebrew HDF5 path.
+#
+#   make -f oracle/ref.mk KERNEL=m6 args   (the flag-parser driver)
 #
 #   make -f oracle/ref.mk KERNEL=m6      (also m4, gauss)
 #
@@ -44,10 +47,17 @@ $(OUT)/stamp: oracle/ref.mk | $(OUT)
 $(OUT)/ref_harness: $(OUT)/stamp oracle/ref_harness.f90
 	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) oracle/ref_harness.f90 $(OBJS) -o $@
 
+# ./execute flag parser (args.f90) with this repository's driver
+$(OUT)/args_harness: $(OUT)/stamp oracle/args_harness.f90
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) -c $(REFSRC)/args.f90 -o $(OUT)/args.o
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -I $(OUT) oracle/args_harness.f90 $(OUT)/args.o $(OBJS) -o $@
+
+args: $(OUT)/args_harness
+
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all args clean
 
 # Side-by-side program: reference iterate() and the HIP drop-in (needs libsph_hip.so)
 HIPLIB := sph_playground_amd/libsph_hip.so
