@@ -54,7 +54,31 @@ def build(verbose=False, jobs=4):
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_dump(verbose)
     return LIB
+
+
+# The state dump (include/sph_dump.h) is host code on the HDF5 C library; it
+# is built where the library is found and is not part of the GPU path.
+HDF5_PREFIX = os.environ.get("SPH_HDF5_PREFIX", "/opt/conda")
+DUMP_LIB = os.path.join(HERE, "libsph_dump.so")
+
+
+def build_dump(verbose=False):
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print("libsph_dump.so not built: no hdf5.h under " + inc, file=sys.stderr)
+        return None
+    tmp = DUMP_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + inc, os.path.join(CSRC, "sph_dump.cpp"), "-o", tmp, "-L" + lib, "-lhdf5",
+           "-Wl,-rpath," + lib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DUMP_LIB)
+    return DUMP_LIB
 
 
 if __name__ == "__main__":
