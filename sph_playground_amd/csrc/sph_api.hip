@@ -168,6 +168,15 @@ struct sph_ctx {
   int64_t staging_bytes = 0;
   double4 *final_st = nullptr;
   Grid grid{};
+  // cell keys kept from an earlier step: valid while the positions they were
+  // computed from are unchanged (pos_version) and the grid is the same
+  uint64_t pos_version = 1;       // bumped by every write of positions (load, leapfrog, wrap)
+  uint64_t vb_version = 1;        // bumped by every write of v or B (load, leapfrog)
+  uint64_t flags_version = 0;     // vb_version step_flags were computed from
+  bool flags_pending = false;     // this step's k_prep computes them (read with the search)
+  uint64_t keys_version = 0;      // pos_version the keys and cell starts belong to
+  Grid keys_grid{};
+  int64_t keys_n = -1, keys_nkeys = -1;
   sph_stats stats{};
   bool have_step = false;
   hipEvent_t ev[8]{};
@@ -383,6 +392,17 @@ static long long stripe_min(const long long *h, int v) {
 #endif
 static const int kNeighRS = SPH_NEIGH_RS;
 
+// d_int slot k_wrap_reduce sets when it moved a particle back into the box
+constexpr int kWrapFlag = 4;
+
+static bool same_grid(const Grid &a, const Grid &b) {
+  for (int d = 0; d < 3; d++)
+    if (a.lo[d] != b.lo[d] || a.inv_cs[d] != b.inv_cs[d] || a.cs[d] != b.cs[d] || a.nc[d] != b.nc[d] ||
+        a.periodic[d] != b.periodic[d])
+      return false;
+  return a.mdim == b.mdim && a.ty == b.ty;
+}
+
 // The search's read-back (pair counts, list lengths, k_prep's flags and the
 // error bits) can be left to a later synchronisation: copies to these pinned
 // slots, taken by finish_neighbours.
@@ -391,8 +411,12 @@ constexpr int kNeighInt = 16;    // h_int slot of its error bits
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
 
 static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
-  c->step_flags = 0;
-  for (int k = 0; k < kStripes; k++) c->step_flags |= ll[kFlagSlots + k];
+  if (c->flags_pending) {  // k_prep's v / B flags of this step
+    c->step_flags = 0;
+    for (int k = 0; k < kStripes; k++) c->step_flags |= ll[kFlagSlots + k];
+    c->flags_version = c->vb_version;
+    c->flags_pending = false;
+  }
   c->stats.n_pairs = stripe_sum(ll, 0);
   int maxcnt = (int)stripe_max(ll, 1);
   c->last_maxcnt = maxcnt;
@@ -430,8 +454,9 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
                      c->n_real, c->box, dim, c->d_red, c->d_int);
   long long *red = c->h_ll + 512;
   HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7 * kStripes, hipMemcpyDeviceToHost, c->stream));
-  RCHK(read_ints(c, 0, 1));
+  RCHK(read_ints(c, 0, kWrapFlag + 1));
   if (c->h_int[0] & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
+  if (c->h_int[kWrapFlag]) c->pos_version++;  // a particle was wrapped back into the box
   double lo[3], hi[3];
   for (int a = 0; a < 3; a++) { lo[a] = ord2d(stripe_min(red, a)); hi[a] = ord2d(stripe_max(red, 3 + a)); }
   double hmax = c->n_real > 0 ? ord2d(stripe_max(red, 6)) : 0.0;
@@ -507,44 +532,55 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   RCHK(ensure_cells(c, nkeys));
   int key_bits = 1;
   while (key_bits < 32 && ((int64_t)1 << key_bits) < nkeys) key_bits++;
-  // 3. cell keys, sortedness, radix sort + reorder when needed
-  hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
-  hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
-  RCHK(read_ints(c, 2, 1));
+  // 3. cell keys, sortedness, radix sort + reorder when needed; keys and
+  //    cell starts of the same positions on the same grid are still valid
   c->stats.sorted = 0;
-  if (c->h_int[2]) {
-    size_t bytes = c->sort_tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, bytes, c->keys, c->keys2, c->vals, c->vals2, n, 0,
-                                              key_bits, c->stream));
-    int nxt = 1 - c->cur;
-    Fields src = c->fields();
-    Fields dst;
-    for (int f = 0; f < F_TOTAL; f++) dst.f[f] = c->fbuf[nxt] + (size_t)f * c->npad;
-    hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, src, dst, c->vals2, c->orig[c->cur],
-                       c->orig[nxt], n);
-    // auxiliary arrays ride along in groups of 32
-    for (int k0 = 0; k0 < c->naux; k0 += F_TOTAL) {
-      Fields as{}, ad{};
-      int m = std::min((int)F_TOTAL, c->naux - k0);
-      for (int f = 0; f < F_TOTAL; f++) {
-        int k = k0 + std::min(f, m - 1);
-        as.f[f] = c->fbuf[c->cur] + (size_t)(F_TOTAL + k) * c->npad;
-        ad.f[f] = c->fbuf[nxt] + (size_t)(F_TOTAL + k) * c->npad;
-      }
-      hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, as, ad, c->vals2, c->orig[c->cur],
+  const bool keys_kept = c->keys_version == c->pos_version && c->keys_n == n && c->keys_nkeys == nkeys &&
+                         same_grid(g, c->keys_grid);
+  if (!keys_kept) {
+    hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
+    hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
+    RCHK(read_ints(c, 2, 1));
+    if (c->h_int[2]) {
+      size_t bytes = c->sort_tmp_bytes;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, bytes, c->keys, c->keys2, c->vals, c->vals2, n, 0,
+                                                key_bits, c->stream));
+      int nxt = 1 - c->cur;
+      Fields src = c->fields();
+      Fields dst;
+      for (int f = 0; f < F_TOTAL; f++) dst.f[f] = c->fbuf[nxt] + (size_t)f * c->npad;
+      hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, src, dst, c->vals2, c->orig[c->cur],
                          c->orig[nxt], n);
+      // auxiliary arrays ride along in groups of 32
+      for (int k0 = 0; k0 < c->naux; k0 += F_TOTAL) {
+        Fields as{}, ad{};
+        int m = std::min((int)F_TOTAL, c->naux - k0);
+        for (int f = 0; f < F_TOTAL; f++) {
+          int k = k0 + std::min(f, m - 1);
+          as.f[f] = c->fbuf[c->cur] + (size_t)(F_TOTAL + k) * c->npad;
+          ad.f[f] = c->fbuf[nxt] + (size_t)(F_TOTAL + k) * c->npad;
+        }
+        hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, as, ad, c->vals2, c->orig[c->cur],
+                           c->orig[nxt], n);
+      }
+      c->cur = nxt;
+      std::swap(c->keys, c->keys2);
+      F = c->fields();
+      c->stats.sorted = 1;
     }
-    c->cur = nxt;
-    std::swap(c->keys, c->keys2);
-    F = c->fields();
-    c->stats.sorted = 1;
+    hipLaunchKernelGGL(k_cell_start, dim3(grid_for(n + 1)), dim3(kBlock), 0, c->stream, c->keys, n, (uint32_t)nkeys,
+                       c->cstart);
+    c->keys_version = c->pos_version;
+    c->keys_grid = g;
+    c->keys_n = n;
+    c->keys_nkeys = nkeys;
   }
-  hipLaunchKernelGGL(k_cell_start, dim3(grid_for(n + 1)), dim3(kBlock), 0, c->stream, c->keys, n, (uint32_t)nkeys,
-                     c->cstart);
-  HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
+  // the v / B flags are computed again only when v or B may have changed
+  c->flags_pending = c->flags_version != c->vb_version;
+  if (c->flags_pending) HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
-                     c->d_stripes + kFlagSlots);
+                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -869,6 +905,8 @@ static int load_impl(sph_ctx *c, int64_t n, const double *store, int64_t ld, int
     src = c->d_staging;
   }
   c->cur = 0;
+  c->pos_version++;
+  c->vb_version++;
   hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, ld, (int)n, c->fields(),
                      c->orig[0]);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1148,10 +1186,14 @@ int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_us
   c->naux = 9;
   hipLaunchKernelGGL(k_predict, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, 1,
                      c->aux(0), (int)c->npad);
+  c->pos_version++;
+  c->vb_version++;
   int rc = run_step(c, state);
   if (rc == SPH_OK)
     hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, 1,
                        c->aux(0), (int)c->npad);
+  c->pos_version++;
+  c->vb_version++;
   c->naux = 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   return rc;

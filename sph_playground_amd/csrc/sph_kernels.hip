@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
   for (int a = 0; a < 3; a++) { mn[a] = BIG; mx[a] = SMALL; }
   hm = SMALL;
-  bool outside = false;
+  bool outside = false, wrapped = false;
   // grid-stride: a few blocks' worth of particles per block keeps the
   // blocks' closing atomics few
   for (int s = gtid(); s < n; s += (int)(gridDim.x * blockDim.x)) {
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
         double nx = x[a];
         if (dxmax[a] > 0) nx = b.emin[a] + dxmax[a];
         if (dxmin[a] > 0) nx = b.emax[a] - dxmin[a];
-        if (nx != x[a]) { x[a] = nx; F.f[F_RX + a][s] = nx; }
+        if (nx != x[a]) { x[a] = nx; F.f[F_RX + a][s] = nx; wrapped = true; }
       }
     }
     for (int a = 0; a < dim; a++)
@@ -186,6 +186,7 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
     }
   }
   if (outside) atomicOr(err, EB_OUTSIDE);
+  if (wrapped) atomicOr(err + 4, 1);  // positions changed: cell keys must be recomputed
   // the seven block reductions in one pass: per wave, then one barrier, then
   // thread v combines value v over the block's waves
   long long w7[7] = {wave_min(mn[0]), wave_min(mn[1]), wave_min(mn[2]), wave_max(mx[0]), wave_max(mx[1]),
@@ -273,8 +274,9 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
   bool r = orig[s] < n_real;
   real[s] = r ? 1 : 0;
   st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
-  tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
+  // T den for grad T: FP64 in tr, or in the FP32 path's packed record alone
   if (jr) jr[s] = make_double4(x, y, z, pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s])));
+  else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
   eps[s] = r ? 1.0 : 0.0;
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
@@ -307,13 +309,17 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
   int s = gtid();
   int fl = 0;
   if (s < n) {
-    const double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
-    if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
-    if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
+    if (flags) {  // null: the caller still holds the flags of unchanged v and B
+      const double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
+      if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
+      if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
+    }
     prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, jr, eps);
   }
-  long long t = block_reduce<R_OR>(fl, lds);
-  if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
+  if (flags) {
+    long long t = block_reduce<R_OR>(fl, lds);
+    if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
+  }
 }
 
 // Does the reference hold a ghost of j shifted by (sx, sy, sz)?  The first
@@ -798,8 +804,10 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
         F.f[F_C][s] = sqrt(gamma * p / den);
       }
     }
-    double vx = F.f[F_VX][s], vy = F.f[F_VY][s], vz = F.f[F_VZ][s];
-    double bx = F.f[F_BX][s], by = F.f[F_BY][s], bz = F.f[F_BZ][s];
+    // v and B only where a pack or the flags need them
+    double vx = 0, vy = 0, vz = 0, bx = 0, by = 0, bz = 0;
+    if (flags || q1) { vx = F.f[F_VX][s]; vy = F.f[F_VY][s]; vz = F.f[F_VZ][s]; }
+    if (flags || q4) { bx = F.f[F_BX][s]; by = F.f[F_BY][s]; bz = F.f[F_BZ][s]; }
     if (flags) {
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;

@@ -209,6 +209,29 @@ def test_resident_steps_equal_host_iterate(gpu):
     assert max(errs.values()) <= FP64_TOL, errs
 
 
+def test_kept_cell_keys_vs_oracle(gpu):
+    # a resident second step on particles that did not move keeps the first
+    # step's cell keys and cell starts (no key pass, no order check); its
+    # neighbour sets and fields must still be the oracle's on the first step's
+    # output
+    state, st = P.diffusion3d(12, ddw="fab", b=(1.0, 0.3, 0.2), h_scale=1.03)
+    eng = _engine(state, 1, tensor="full", kappa_perp=1e-2)
+    eng.load(st)
+    eng.step()
+    mid = eng.save()
+    eng.step()
+    assert eng.stats()["sorted"] == 0
+    got = eng.save()
+    nr = int(state[C.ec_realpn - 1])
+    ref = oracle.iterate(state, mid, kernel=1, tensor_mode=1, kappa_perp=1e-2)
+    off, j, code = eng.neighbours()
+    a = G.pair_sets(ref["off"], ref["j"], ref["code"])
+    b = G.pair_sets(off, j, code)
+    assert sum(1 for x, y in zip(a, b) if x != y) == 0
+    errs = G.field_errors(ref["store"], got, nr)
+    assert max(errs.values()) <= FP64_TOL, errs
+
+
 def test_device_store_path(gpu):
     import torch
     state, st = P.hydro2d(32)
