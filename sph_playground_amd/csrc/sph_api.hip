@@ -42,10 +42,11 @@ template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1_simple(Loop1Args<double4>, Fields);
 template <typename V4>
 __global__ void k_post1(Fields, const double4 *, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
-                        V4 *, V4 *, V4 *, double4 *, double *, long long *);
+                        V4 *, V4 *, V4 *, double4 *, double *, long long *, long long *);
 template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ void k_loop2(Loop2Args<V4>);
 __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
+__global__ void k_step_close(const double *, int, const int *, const long long *, long long *);
 __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
 __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
                                const int64_t *, int32_t *, int8_t *);
@@ -94,7 +95,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kFlagSlots = 4 * kStripes;  // d_stripes slots of k_prep's flags
-constexpr int kLoop2ErrSlot = 255;        // h_int slot of loop 2's error bits (last of 256)
 constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fields
 
 // statevars slots (0-based), const.f90:14-66
@@ -151,7 +151,6 @@ struct sph_ctx {
   int nb_cap = 0;
   int last_maxcnt = 0;  // longest neighbour list of the last search
   long long step_flags = 0;       // k_prep's v / B flags of this step
-  double *h_energy = nullptr;     // pinned loop-2 energy partials (energy_cap)
   bool loop2_pending = false;     // loop 2's read-back still to be taken
   bool neigh_pending = false;     // the search's read-back still to be taken (loop 1's first sync brings it)
   int *cnt = nullptr;
@@ -174,12 +173,17 @@ struct sph_ctx {
   uint64_t vb_version = 1;        // bumped by every write of v or B (load, leapfrog)
   uint64_t flags_version = 0;     // vb_version step_flags were computed from
   bool flags_pending = false;     // this step's k_prep computes them (read with the search)
+  // the bounding box and h_max the next grid needs, when nothing but the
+  // library's own step changed the positions (bbox_version) or h (hmax_version)
+  uint64_t h_version = 1;         // bumped by every write of h other than the step's own (load, leapfrog)
+  uint64_t bbox_version = 0, hmax_version = 0;
+  double last_lo[3] = {0, 0, 0}, last_hi[3] = {0, 0, 0}, next_hmax = 0.0;
   uint64_t keys_version = 0;      // pos_version the keys and cell starts belong to
   Grid keys_grid{};
   int64_t keys_n = -1, keys_nkeys = -1;
   sph_stats stats{};
   bool have_step = false;
-  hipEvent_t ev[8]{};
+  hipEvent_t ev[9]{};             // 8: the short waits of read_ints
   // differentiable diffusion (sph_diff_*)
   bool diff_ready = false;
   Phys diff_phys{};
@@ -316,9 +320,6 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
   for (int k = 0; k < 5; k++) RCHK(dev_alloc(&c->q[k], v4 * npad));
   c->energy_cap = (npad + kBlock - 1) / kBlock;
   RCHK(dalloc(&c->energy_part, c->energy_cap));
-  if (c->h_energy) (void)hipHostFree(c->h_energy);
-  c->h_energy = nullptr;
-  if (hipHostMalloc((void **)&c->h_energy, sizeof(double) * c->energy_cap) != hipSuccess) return SPH_ERR_NOMEM;
   // radix-sort scratch for n keys
   size_t bytes = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->keys, c->keys2, c->vals, c->vals2, (int)npad, 0, 32,
@@ -362,9 +363,19 @@ static int map_err(int bits) {
   return SPH_OK;
 }
 
+// The short waits inside a step poll their event: the runtime's blocking
+// wait adds its wake-up to every host round trip of the step.
+static hipError_t wait_event(hipEvent_t e) {
+  hipError_t r;
+  while ((r = hipEventQuery(e)) == hipErrorNotReady) {
+  }
+  return r;
+}
+
 static int read_ints(sph_ctx *c, int first, int count) {
   HIPCHK(hipMemcpyAsync(c->h_int + first, c->d_int + first, sizeof(int) * count, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipEventRecord(c->ev[8], c->stream));
+  HIPCHK(wait_event(c->ev[8]));
   return SPH_OK;
 }
 
@@ -394,6 +405,12 @@ static const int kNeighRS = SPH_NEIGH_RS;
 
 // d_int slot k_wrap_reduce sets when it moved a particle back into the box
 constexpr int kWrapFlag = 4;
+// d_red stripes of k_post1's h_max (ordered int64) and their pinned h_ll slots
+constexpr int kHmaxRed = 8 * kStripes;
+// d_red slots of k_step_close's record (energy bits, loop-2 error bits, h_max)
+// and their pinned h_ll slots
+constexpr int kCloseRed = 9 * kStripes;
+constexpr int kCloseLL = 1536;
 
 static bool same_grid(const Grid &a, const Grid &b) {
   for (int d = 0; d < 3; d++)
@@ -443,23 +460,32 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   Fields F = c->fields();
   unsigned G = grid_for(n);
   const long long BIG = 0x7FFFFFFFFFFFFFFFLL, SMALL = -0x7FFFFFFFFFFFFFFFLL - 1;
-  // 1. wrap escaped real particles, bounding box, max h (striped reductions)
-  long long *init = c->h_ll;
-  for (int v = 0; v < 7; v++)
-    for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = v < 3 ? BIG : SMALL;
-  HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7 * kStripes, hipMemcpyHostToDevice, c->stream));
+  // 1. wrap escaped real particles, bounding box, max h (striped reductions).
+  //    Positions unchanged since the last pass (so none escaped) and h last
+  //    set by the previous step (whose k_post1 reduced h_max): nothing to do.
   HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
-  // at most 1024 blocks (four per CU), each striding over the particles
-  hipLaunchKernelGGL(k_wrap_reduce, dim3(std::min(G, 1024u)), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n,
-                     c->n_real, c->box, dim, c->d_red, c->d_int);
-  long long *red = c->h_ll + 512;
-  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7 * kStripes, hipMemcpyDeviceToHost, c->stream));
-  RCHK(read_ints(c, 0, kWrapFlag + 1));
-  if (c->h_int[0] & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
-  if (c->h_int[kWrapFlag]) c->pos_version++;  // a particle was wrapped back into the box
-  double lo[3], hi[3];
-  for (int a = 0; a < 3; a++) { lo[a] = ord2d(stripe_min(red, a)); hi[a] = ord2d(stripe_max(red, 3 + a)); }
-  double hmax = c->n_real > 0 ? ord2d(stripe_max(red, 6)) : 0.0;
+  double lo[3], hi[3], hmax;
+  if (c->bbox_version == c->pos_version && c->hmax_version == c->h_version && c->n_real > 0) {
+    for (int a = 0; a < 3; a++) { lo[a] = c->last_lo[a]; hi[a] = c->last_hi[a]; }
+    hmax = c->next_hmax;
+  } else {
+    long long *init = c->h_ll;
+    for (int v = 0; v < 7; v++)
+      for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = v < 3 ? BIG : SMALL;
+    HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7 * kStripes, hipMemcpyHostToDevice, c->stream));
+    // at most 1024 blocks (four per CU), each striding over the particles
+    hipLaunchKernelGGL(k_wrap_reduce, dim3(std::min(G, 1024u)), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n,
+                       c->n_real, c->box, dim, c->d_red, c->d_int);
+    long long *red = c->h_ll + 512;
+    HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(long long) * 7 * kStripes, hipMemcpyDeviceToHost, c->stream));
+    RCHK(read_ints(c, 0, kWrapFlag + 1));
+    if (c->h_int[0] & EB_OUTSIDE) return SPH_ERR_UNSUPPORTED;
+    if (c->h_int[kWrapFlag]) c->pos_version++;  // a particle was wrapped back into the box
+    for (int a = 0; a < 3; a++) { lo[a] = ord2d(stripe_min(red, a)); hi[a] = ord2d(stripe_max(red, 3 + a)); }
+    hmax = c->n_real > 0 ? ord2d(stripe_max(red, 6)) : 0.0;
+    for (int a = 0; a < 3; a++) { c->last_lo[a] = lo[a]; c->last_hi[a] = hi[a]; }
+    c->bbox_version = c->pos_version;
+  }
   if (!(hmax > 0.0) && c->n_real > 0) return SPH_ERR_Q_NAN_OR_NEG;
   double rmax = c->frozen.krad * hmax * (1.0 + 1e-9);
   if (!(rmax > 0.0)) rmax = 1.0;
@@ -785,9 +811,10 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
   V4 *q4 = want_q4 && (generic || (fl & 2)) ? (V4 *)c->q[4] : nullptr;
   long long *none = nullptr;
+  HIPCHK(hipMemsetAsync(c->d_red + kHmaxRed, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
                      c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
-                     generic ? nullptr : c->trd, none);
+                     generic ? nullptr : c->trd, none, c->d_red + kHmaxRed);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
   Loop2Args<V4> a{};
@@ -809,8 +836,11 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
   HIPCHK(hipEventRecord(c->ev[6], c->stream));
   // energy partials and error bits come back with the step's closing sync
-  HIPCHK(hipMemcpyAsync(c->h_energy, c->energy_part, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_int + kLoop2ErrSlot, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // energy, error bits and h_max in one record, one small copy
+  hipLaunchKernelGGL(k_step_close, dim3(1), dim3(256), 0, c->stream, c->energy_part, (int)G, c->d_int,
+                     c->d_red + kHmaxRed, c->d_red + kCloseRed);
+  HIPCHK(hipMemcpyAsync(c->h_ll + kCloseLL, c->d_red + kCloseRed, sizeof(long long) * 3, hipMemcpyDeviceToHost,
+                        c->stream));
   c->loop2_pending = true;
   return SPH_OK;
 }
@@ -820,12 +850,18 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
 static int finish_loop2(sph_ctx *c) {
   if (!c->loop2_pending) return SPH_OK;
   c->loop2_pending = false;
-  const unsigned G = grid_for(c->n);
-  double e = 0;
-  for (unsigned b = 0; b < G; b++) e += c->h_energy[b];
+  double e;
+  std::memcpy(&e, c->h_ll + kCloseLL, sizeof e);
   c->stats.energy = e;
+  // h_max after the step's own h update: the next step's grid needs no
+  // bounding-box pass while positions and h change no other way
+  const double hm = ord2d(c->h_ll[kCloseLL + 2]);
+  if (hm > 0.0 && std::isfinite(hm)) {
+    c->next_hmax = hm;
+    c->hmax_version = c->h_version;
+  }
   c->stats.ms_k_loop2 = elapsed(c, c->ev[5], c->ev[6]);
-  return map_err(c->h_int[kLoop2ErrSlot]);
+  return map_err((int)c->h_ll[kCloseLL + 1]);
 }
 
 static Phys call_phys(const sph_ctx *c, const double *state) {
@@ -847,10 +883,13 @@ static int run_step(sph_ctx *c, const double *state) {
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
-  // loop 1 follows the search without a host round trip; its first
-  // synchronisation brings the search's read-back, and lists that overflowed
-  // send the step back to a full search with room for them
-  RCHK(build_neighbours(c, true));
+  // the Newton loop 1 follows the search without a host round trip; its
+  // first synchronisation brings the search's read-back, and lists that
+  // overflowed send the step back to a full search with room for them (it
+  // writes nothing the search set-up reads).  c1simple writes h, rho and om
+  // in place, so its search is checked before it runs.
+  RCHK(build_neighbours(c, P.adden != 0));
+  c->hmax_version = 0;  // h changes from here; this step's k_post1 reports the new h_max
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
   int r1 = run_loop1(c, P);
   if (r1 == SPH_OK && c->neigh_pending) {  // loop 1 ran no sweep (no real particles)
@@ -871,7 +910,7 @@ static int run_step(sph_ctx *c, const double *state) {
     hipLaunchKernelGGL(k_gravity, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
                        c->gravity);
   HIPCHK(hipEventRecord(c->ev[3], c->stream));
-  HIPCHK(hipEventSynchronize(c->ev[3]));
+  HIPCHK(wait_event(c->ev[3]));
   const int rc2 = finish_loop2(c);
   if (rc == SPH_OK) rc = rc2;
   float t01 = 0, t12 = 0, t23 = 0;
@@ -907,6 +946,7 @@ static int load_impl(sph_ctx *c, int64_t n, const double *store, int64_t ld, int
   c->cur = 0;
   c->pos_version++;
   c->vb_version++;
+  c->h_version++;
   hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, ld, (int)n, c->fields(),
                      c->orig[0]);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1052,7 +1092,6 @@ int sph_ctx_destroy(sph_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
   if (c->h_ll) (void)hipHostFree(c->h_ll);
-  if (c->h_energy) (void)hipHostFree(c->h_energy);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1188,12 +1227,14 @@ int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_us
                      c->aux(0), (int)c->npad);
   c->pos_version++;
   c->vb_version++;
+  c->h_version++;
   int rc = run_step(c, state);
   if (rc == SPH_OK)
     hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n, dt, 1,
                        c->aux(0), (int)c->npad);
   c->pos_version++;
   c->vb_version++;
+  c->h_version++;
   c->naux = 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   return rc;

@@ -783,7 +783,7 @@ template <typename V4>
 __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n,
                                                int advanced, int do_eos, int iso, double c1, double gamma, V4 *q0,
                                                V4 *q1, V4 *q2, V4 *q3, V4 *q4, double4 *pd, double *trd,
-                                               long long *flags) {
+                                               long long *flags, long long *hmax_red) {
   __shared__ long long lds[4];
   using T = decltype(V4().x);
   int s = gtid();
@@ -830,6 +830,14 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, con
   if (flags) {
     long long t = block_reduce<R_OR>(fl, lds);
     if (threadIdx.x == 0 && t) atomicOr((unsigned long long *)&flags[stripe()], (unsigned long long)t);
+  }
+  // the largest h of the real particles after this step (ordered int64), for
+  // the next step's grid when nothing else changes h or the positions
+  if (hmax_red) {
+    long long hm = -0x7FFFFFFFFFFFFFFFLL - 1;
+    if (s < n && real[s]) hm = d2ord(F.f[F_H][s]);
+    const long long t = block_reduce<R_MAX>(hm, lds);
+    if (threadIdx.x == 0) atomicMax(&hmax_red[stripe()], t);
   }
 }
 
@@ -1255,6 +1263,40 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
   }
 }
 
+// The end of a step in one block: loop 2's energy partials summed in a fixed
+// order (thread t over partials t, t + 256, ..., then a fixed binary tree
+// over the threads), its error bits and k_post1's largest h, as one small
+// record the host takes with one copy.
+__global__ __launch_bounds__(256) void k_step_close(const double *part, int np, const int *err,
+                                                    const long long *hmax_red, long long *out) {
+  __shared__ double sums[256];
+  __shared__ long long hms[kStripes];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int k0 = t; k0 < np; k0 += 256 * 8) {  // eight loads in flight, added in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = k0 + 256 * u < np ? part[k0 + 256 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  sums[t] = acc;
+  if (t < kStripes) hms[t] = hmax_red[t];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sums[t] += sums[t + w];
+      if (w < kStripes) hms[t] = hms[t + w] > hms[t] ? hms[t + w] : hms[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = __double_as_longlong(sums[0]);
+    out[1] = err[0];
+    out[2] = hms[0];
+  }
+}
+
 // ------------------------------------------------------------- transfers
 // AoS store rows (any leading dimension) <-> SoA fields in sorted order.
 __global__ void k_aos_to_soa(const double *aos, int64_t ld, int n, Fields F, int64_t *orig) {
@@ -1461,9 +1503,9 @@ SPH_INST_NEIGH(2)
 SPH_INST_NEIGH(3)
 template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
                                           double, double4 *, double4 *, double4 *, double4 *, double4 *, double4 *,
-                                          double *, long long *);
+                                          double *, long long *, long long *);
 template __global__ void k_post1<float4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
                                          double, float4 *, float4 *, float4 *, float4 *, float4 *, double4 *,
-                                         double *, long long *);
+                                         double *, long long *, long long *);
 
 }  // namespace sph

@@ -209,6 +209,16 @@ def test_resident_steps_equal_host_iterate(gpu):
     assert max(errs.values()) <= FP64_TOL, errs
 
 
+def test_simple_loop1_after_list_overflow_vs_oracle(gpu):
+    # c1simple (adden off) writes h and rho in place, so lists that overflow
+    # their first storage must be searched again before it runs: Gaussian 2D
+    # lists (about 300 neighbours) overflow the first step's storage
+    state, st = P.hydro2d(16, kernel=2)
+    state[C.ec_adden - 1] = C.eif_no
+    eng, ref, got = _oracle_compare(state, st, kernel=2)
+    assert ref["off"][-1] > 48 * st.shape[0]
+
+
 def test_kept_cell_keys_vs_oracle(gpu):
     # a resident second step on particles that did not move keeps the first
     # step's cell keys and cell starts (no key pass, no order check); its
