@@ -124,8 +124,9 @@ __device__ __forceinline__ uint32_t morton_key(const int c[3], int mdim) {
 // in memory.
 __device__ __forceinline__ uint32_t row_of(const Grid &g, int cy, int cz) {
   constexpr int L = kRowTileLog2, T = kRowTile;
+  // 24-bit products: tiles and rows stay below 2^24 under the axis caps
   if (g.mdim == 3)
-    return ((uint32_t)(cz >> L) * (uint32_t)g.ty + (uint32_t)(cy >> L)) * (uint32_t)(T * T) +
+    return (__umul24((uint32_t)(cz >> L), (uint32_t)g.ty) + (uint32_t)(cy >> L)) * (uint32_t)(T * T) +
            (uint32_t)(((cz & (T - 1)) << L) | (cy & (T - 1)));
   return (uint32_t)cy;
 }
@@ -525,8 +526,10 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
         if (use) {
           // the sphere's x extent at this row: cells [xl, xh] before
           // wrapping, at most one wrap away (a radius that is not finite
-          // takes the whole stencil)
-          const float hxc = fminf(sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], (float)(a.rx + 1));
+          // takes the whole stencil); the hardware square root's ulp is far
+          // inside the margin mcell
+          const float hxc =
+              fminf(__builtin_amdgcn_sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], (float)(a.rx + 1));
           int xl = (int)floorf(pcx - hxc);
           int xh = (int)floorf(pcx + hxc);
           xl = max(xl, ci[0] - a.rx);
@@ -535,7 +538,9 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
             xl = max(xl, -nx);
             xh = min(xh, 2 * nx - 1);
           }
-          const uint32_t row = row_of(a.g, cy, cz) * (uint32_t)nx;
+          // first key of the row (rows and nx stay below 2^24 under the
+          // axis caps: 1024 cells per axis in 3D, 32768 in 2D)
+          const uint32_t row = __umul24(row_of(a.g, cy, cz), (uint32_t)nx);
           if (a.g.periodic[0] && xl < 0) {  // left of the box: images shifted by -L
             const int l = xl + nx, h = min(xh, -1) + nx;
             scan(Img{}, a.cstart[row + l], a.cstart[row + h + 1], -1, sy, sz);
