@@ -30,7 +30,7 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, double4 *, double *, long long *);
+                       uint8_t *, double4 *, double *, double4 *, double *, long long *, long long *, int);
 template <int DIM>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -94,7 +94,21 @@ __global__ void k_batch_in(const double *, const int64_t *, const double *, int,
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kFlagSlots = 4 * kStripes;  // d_stripes slots of k_prep's flags
+// Per-step counters in d_stripes (long long slots, kStripes per value): the
+// search's pair counts and list lengths, loop 1's active counts and pairs of
+// odd sweeps, k_post1's h_max, the error words (ints), k_prep's v / B flags,
+// loop 1's counters of even sweeps.  k_prep zeroes [0, kZeroLL) as it starts
+// a step, so the step's first search, first sweep, k_post1 and loop 2 find
+// their counters cleared without a fill of their own.
+constexpr int kSearchLL = 0;
+constexpr int kSweepOddLL = 2 * kStripes;
+constexpr int kHmaxLL = 4 * kStripes;
+constexpr int kErrLL = 5 * kStripes;      // ints: 0 search, 1 loop 1, 2 loop 2
+constexpr int kZeroLL = kErrLL + 4;
+constexpr int kFlagSlots = 6 * kStripes;
+constexpr int kSweepEvenLL = 7 * kStripes;
+constexpr int kStepLL = 9 * kStripes;     // d_stripes size
+enum { kErrSearch = 0, kErrLoop1 = 1, kErrLoop2 = 2 };
 constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fields
 
 // statevars slots (0-based), const.f90:14-66
@@ -405,8 +419,6 @@ static const int kNeighRS = SPH_NEIGH_RS;
 
 // d_int slot k_wrap_reduce sets when it moved a particle back into the box
 constexpr int kWrapFlag = 4;
-// d_red stripes of k_post1's h_max (ordered int64) and their pinned h_ll slots
-constexpr int kHmaxRed = 8 * kStripes;
 // d_red slots of k_step_close's record (energy bits, loop-2 error bits, h_max)
 // and their pinned h_ll slots
 constexpr int kCloseRed = 9 * kStripes;
@@ -423,9 +435,15 @@ static bool same_grid(const Grid &a, const Grid &b) {
 // The search's read-back (pair counts, list lengths, k_prep's flags and the
 // error bits) can be left to a later synchronisation: copies to these pinned
 // slots, taken by finish_neighbours.
-constexpr int kNeighLL = 1024;   // h_ll slots of the deferred read-back
-constexpr int kNeighInt = 16;    // h_int slot of its error bits
+constexpr int kNeighLL = 1024;   // h_ll slots of the search's read-back ([0, kFlagSlots + kStripes) of d_stripes)
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
+
+// the error words of the step (ints in d_stripes), and the same words in a
+// pinned copy of d_stripes starting at slot `base`
+static int *step_err(sph_ctx *c) { return reinterpret_cast<int *>(c->d_stripes + kErrLL); }
+static int copied_err(const long long *copy, int base, int word) {
+  return reinterpret_cast<const int *>(copy + (kErrLL - base))[word];
+}
 
 static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
   if (c->flags_pending) {  // k_prep's v / B flags of this step
@@ -448,7 +466,8 @@ static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
 static int take_neighbours(sph_ctx *c) {
   if (!c->neigh_pending) return SPH_OK;
   c->neigh_pending = false;
-  return finish_neighbours(c, c->h_ll + kNeighLL, c->h_int[kNeighInt]);
+  const long long *ll = c->h_ll + kNeighLL;
+  return finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
 }
 
 // defer: launch the search and its read-back copies without waiting; the
@@ -463,7 +482,6 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   // 1. wrap escaped real particles, bounding box, max h (striped reductions).
   //    Positions unchanged since the last pass (so none escaped) and h last
   //    set by the previous step (whose k_post1 reduced h_max): nothing to do.
-  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
   double lo[3], hi[3], hmax;
   if (c->bbox_version == c->pos_version && c->hmax_version == c->h_version && c->n_real > 0) {
     for (int a = 0; a < 3; a++) { lo[a] = c->last_lo[a]; hi[a] = c->last_hi[a]; }
@@ -473,6 +491,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     for (int v = 0; v < 7; v++)
       for (int k = 0; k < kStripes; k++) init[v * kStripes + k] = v < 3 ? BIG : SMALL;
     HIPCHK(hipMemcpyAsync(c->d_red, init, sizeof(long long) * 7 * kStripes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
     // at most 1024 blocks (four per CU), each striding over the particles
     hipLaunchKernelGGL(k_wrap_reduce, dim3(std::min(G, 1024u)), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n,
                        c->n_real, c->box, dim, c->d_red, c->d_int);
@@ -565,6 +584,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
                          same_grid(g, c->keys_grid);
   if (!keys_kept) {
     hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
+    HIPCHK(hipMemsetAsync(c->d_int + 2, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
     RCHK(read_ints(c, 2, 1));
     if (c->h_int[2]) {
@@ -606,12 +626,14 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   if (c->flags_pending) HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
-                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr);
+                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL);
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
-    HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 2, c->stream));
-    HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
+    if (attempt > 0) {  // k_prep cleared the first search's counters
+      HIPCHK(hipMemsetAsync(c->d_stripes + kSearchLL, 0, sizeof(long long) * 2 * kStripes, c->stream));
+      HIPCHK(hipMemsetAsync(step_err(c) + kErrSearch, 0, sizeof(int), c->stream));
+    }
     NeighArgs a;
     a.hs = c->h0; a.real = c->real; a.cstart = c->cstart;
     // rows and cells within r_max (1 + 1e-6) of a cell; imaged axes at most
@@ -635,7 +657,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     a.delta = std::ldexp(span, -21);
     for (int d = 0; d < 3; d++)  // only imaged axes shift (elsewhere the shift is 0, keep 0 * lenf finite)
       a.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
-    a.maxneib = c->maxneib; a.err = c->d_int; a.stripes = c->d_stripes;
+    a.maxneib = c->maxneib; a.err = step_err(c) + kErrSearch; a.stripes = c->d_stripes + kSearchLL;
     // the search's own events (loop 1 reuses 5 and 6 before a deferred read-back is taken)
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
 #define NEIGH(D) hipLaunchKernelGGL((k_neigh<D>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
@@ -644,18 +666,17 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     else NEIGH(3);
 #undef NEIGH
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
-    // pair counts, list lengths and k_prep's flags in one read-back
+    // pair counts, list lengths, error words and k_prep's flags in one read-back
+    HIPCHK(hipMemcpyAsync(c->h_ll + kNeighLL, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
+                          hipMemcpyDeviceToHost, c->stream));
     if (defer) {
-      HIPCHK(hipMemcpyAsync(c->h_ll + kNeighLL, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
-                            hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(c->h_int + kNeighInt, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       c->neigh_pending = true;
       return SPH_OK;
     }
-    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
-                          hipMemcpyDeviceToHost, c->stream));
-    RCHK(read_ints(c, 0, 1));
-    const int fin = finish_neighbours(c, c->h_ll, c->h_int[0]);
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
+    const long long *ll = c->h_ll + kNeighLL;
+    const int fin = finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
     if (fin != kRedoNeigh) return fin;
   }
   return SPH_ERR_HIP;
@@ -689,13 +710,14 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
   Fields F = c->fields();
   Loop1Args<double4> a{};
   a.pm = c->pm; a.h0 = c->h0; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt; a.eps = c->eps;
-  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = c->d_int; a.stripes = c->d_stripes;
+  // error word and the first sweep's counters: cleared by this step's k_prep
+  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = step_err(c) + kErrLoop1;
   a.b = c->box; a.p = P; a.n = n; a.npad = (int)c->npad;
   double tol = c->opt.newton_tol > 0 ? c->opt.newton_tol : (c->opt.precision == 1 ? 1e-6 : 1e-8);
   a.tol = tol;
   int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
-  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int) * 8, c->stream));
   c->stats.loop1_pairs = 0;
+  int bits = 0;
   if (!P.adden) {
     // c1simple: grad T only with diffusion and the 2nw operators
     int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
@@ -708,11 +730,14 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->final_st = c->st[0];
     c->stats.sweeps = 0;
-    RCHK(read_ints(c, 0, 1));
+    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
     RCHK(take_neighbours(c));
     c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
     c->stats.loop1_pairs = c->stats.n_pairs;
-    return map_err(c->h_int[0]);
+    return map_err(copied_err(c->h_ll, kErrLL, kErrLoop1));
   }
   int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
   int sweeps = 0;
@@ -725,25 +750,40 @@ static int run_loop1(sph_ctx *c, const Phys &P) {
     a.trn = c->tr[it & 1];
     a.jrc = c->jr[(it - 1) & 1];
     a.jrn = c->jr[it & 1];
-    HIPCHK(hipMemsetAsync(c->d_stripes, 0, sizeof(long long) * 2 * kStripes, c->stream));
+    // odd sweeps count into the region k_prep cleared, even ones into their own
+    const int region = (it & 1) ? kSweepOddLL : kSweepEvenLL;
+    if (it > 1) HIPCHK(hipMemsetAsync(c->d_stripes + region, 0, sizeof(long long) * 2 * kStripes, c->stream));
+    a.stripes = c->d_stripes + region;
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
     else launch_loop1_k<3>(c, a, grad, false, F);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     sweeps = it;
-    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes, sizeof(long long) * 2 * kStripes, hipMemcpyDeviceToHost,
-                          c->stream));
-    RCHK(read_ints(c, 0, 1));
+    // the sweep's counters to h_ll[0, 2 kStripes) and the error words as they
+    // lie after the odd region (one copy for odd sweeps, whose region runs on
+    // into them)
+    if (it & 1) {
+      HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kSweepOddLL, sizeof(long long) * (kZeroLL - kSweepOddLL),
+                            hipMemcpyDeviceToHost, c->stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kSweepEvenLL, sizeof(long long) * 2 * kStripes,
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_ll + (kErrLL - kSweepOddLL), c->d_stripes + kErrLL, sizeof(long long) * 4,
+                            hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
     RCHK(take_neighbours(c));
     c->stats.ms_k_loop1 += elapsed(c, c->ev[5], c->ev[6]);
     c->stats.loop1_pairs += stripe_sum(c->h_ll, 1);
-    if (c->h_int[0]) break;  // circuit1.f90:202-204 stops inside the sweep
+    bits = copied_err(c->h_ll, kSweepOddLL, kErrLoop1);
+    if (bits) break;  // circuit1.f90:202-204 stops inside the sweep
     if (stripe_sum(c->h_ll, 0) == 0) break;
   }
   c->final_st = c->st[sweeps & 1];
   c->stats.sweeps = sweeps;
-  return map_err(c->h_int[0]);
+  return map_err(bits);
 }
 
 template <int DIM, int KT, int MODE>
@@ -811,10 +851,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
   V4 *q4 = want_q4 && (generic || (fl & 2)) ? (V4 *)c->q[4] : nullptr;
   long long *none = nullptr;
-  HIPCHK(hipMemsetAsync(c->d_red + kHmaxRed, 0, sizeof(long long) * kStripes, c->stream));
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
                      c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
-                     generic ? nullptr : c->trd, none, c->d_red + kHmaxRed);
+                     generic ? nullptr : c->trd, none, c->d_stripes + kHmaxLL);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
   Loop2Args<V4> a{};
@@ -824,9 +863,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   a.trd = generic ? nullptr : c->trd;
   a.v_nonzero = (fl & 1) ? 1 : 0;
   a.b_uniform = (fl & 2) ? 0 : 1;
-  a.out = F; a.energy_part = c->energy_part; a.err = c->d_int; a.b = c->box; a.p = P; a.n = n;
+  // loop 2's error word and k_post1's h_max stripes: cleared by this step's k_prep
+  a.out = F; a.energy_part = c->energy_part; a.err = step_err(c) + kErrLoop2; a.b = c->box; a.p = P; a.n = n;
   a.npad = (int)c->npad;
-  HIPCHK(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
   HIPCHK(hipEventRecord(c->ev[5], c->stream));
   // fab with one tensor for every pair and no velocity term: the loop
   // without run-time choices
@@ -837,8 +876,8 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   HIPCHK(hipEventRecord(c->ev[6], c->stream));
   // energy partials and error bits come back with the step's closing sync
   // energy, error bits and h_max in one record, one small copy
-  hipLaunchKernelGGL(k_step_close, dim3(1), dim3(256), 0, c->stream, c->energy_part, (int)G, c->d_int,
-                     c->d_red + kHmaxRed, c->d_red + kCloseRed);
+  hipLaunchKernelGGL(k_step_close, dim3(1), dim3(256), 0, c->stream, c->energy_part, (int)G,
+                     step_err(c) + kErrLoop2, c->d_stripes + kHmaxLL, c->d_red + kCloseRed);
   HIPCHK(hipMemcpyAsync(c->h_ll + kCloseLL, c->d_red + kCloseRed, sizeof(long long) * 3, hipMemcpyDeviceToHost,
                         c->stream));
   c->loop2_pending = true;
@@ -1066,7 +1105,7 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   if (c->n_real < 0) { delete c; return SPH_ERR_BAD_ARG; }
   if (dev_alloc((void **)&c->d_red, 16 * kStripes * sizeof(long long)) != SPH_OK ||
       dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
-      dev_alloc((void **)&c->d_stripes, 8 * kStripes * sizeof(long long)) != SPH_OK ||
+      dev_alloc((void **)&c->d_stripes, kStepLL * sizeof(long long)) != SPH_OK ||
       hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_ll, 2048 * sizeof(long long)) != hipSuccess) {
     sph_ctx_destroy(c);
