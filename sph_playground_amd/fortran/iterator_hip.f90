@@ -4,8 +4,8 @@
 ! (state.f90:36, via getState) at every call and creates the device context on
 ! the first call, where the reference freezes its own (iterator.f90:31-42).
 ! Errors stop with the reference's message, as its `error stop` does.
-! Builds against the reference's state.mod (see INTEGRATION.md for the
-! Makefile lines a maintainer adds).
+! Builds against the reference's state.mod: compiled after state.f90 in the
+! reference's source list and linked with -lsph_hip.
 module iterator_hip
   use iso_c_binding
   use sph_hip_capi
