@@ -35,7 +35,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the 128^3 workload's kernels, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the counters read
 # for a plain 1 GiB copy (x2 and x1): profiles/hbm_traffic_diffusion3d_128cube.md
-MEASURED_TRAFFIC_128 = {"neigh": 1.204e9, "loop1_mixed": 1.514e9, "loop2_mixed": 1.632e9}
+MEASURED_TRAFFIC_128 = {"neigh": 1.204e9, "loop1_mixed": 1.525e9, "loop2_mixed": 1.632e9}
 
 
 def log(msg):
