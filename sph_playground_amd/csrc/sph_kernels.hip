@@ -545,13 +545,27 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
           // first key of the row (rows and nx stay below 2^24 under the
           // axis caps: 1024 cells per axis in 3D, 32768 in 2D)
           const uint32_t row = __umul24(row_of(a.g, cy, cz), (uint32_t)nx);
-          if (a.g.periodic[0] && xl < 0) {  // left of the box: images shifted by -L
-            const int l = xl + nx, h = min(xh, -1) + nx;
-            scan(Img{}, a.cstart[row + l], a.cstart[row + h + 1], -1, sy, sz);
+          // every range bound of the row is loaded before the first scan, so
+          // the loads overlap: the part left of the box (images shifted by
+          // -L) or else the part right of it (+L) into w0, w1, and the part
+          // inside into t0, t1
+          const bool wl = a.g.periodic[0] && xl < 0, wr = a.g.periodic[0] && xh >= nx;
+          int w0 = 0, w1 = 0;
+          if (wl) {
+            w0 = a.cstart[row + xl + nx];
+            w1 = a.cstart[row + min(xh, -1) + nx + 1];
+          } else if (wr) {
+            w0 = a.cstart[row + max(xl, nx) - nx];
+            w1 = a.cstart[row + xh - nx + 1];
           }
           const int l = max(xl, 0), h = min(xh, nx - 1);
+          int t0 = 0, t1 = 0;
           if (l <= h) {
-            const int t0 = a.cstart[row + l], t1 = a.cstart[row + h + 1];
+            t0 = a.cstart[row + l];
+            t1 = a.cstart[row + h + 1];
+          }
+          if (wl) scan(Img{}, w0, w1, -1, sy, sz);
+          if (l <= h) {
             if ((sy | sz) != 0) {
               scan(Img{}, t0, t1, 0, sy, sz);
             } else if (dy == 0 && dz == 0) {
@@ -563,9 +577,12 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
               scan(Own{}, t0, t1, 0, 0, 0);
             }
           }
-          if (a.g.periodic[0] && xh >= nx) {  // right of the box: images shifted by +L
-            const int l2 = max(xl, nx) - nx, h2 = xh - nx;
-            scan(Img{}, a.cstart[row + l2], a.cstart[row + h2 + 1], 1, sy, sz);
+          if (wr) {
+            if (wl) {  // both sides (a box under two stencils wide): load the right part now
+              w0 = a.cstart[row + max(xl, nx) - nx];
+              w1 = a.cstart[row + xh - nx + 1];
+            }
+            scan(Img{}, w0, w1, 1, sy, sz);
           }
         }
         while ((c4 >> 2) - done >= 8) flush_chunk();
