@@ -1185,10 +1185,11 @@ void k_loop2(Loop2Args<V4> a) {
 // bit for bit, and no velocity term (adden off, or every v exactly 0).  That
 // is k_loop2's MODE_DIFF with all of its run-time choices fixed, so none of
 // their branches remain in the pair loop; the other outputs of c2 are the
-// zeros MODE_DIFF writes.  FP64 evaluates the pair term with MODE_DIFF's
-// expressions; FP32 takes 1/|r| from one reciprocal square root, folds
-// (dim + 2) and the off-diagonal 2 into the tensor, and applies the
-// particle's constant factors -wcv / (h^(dim+1) rho_a) after the sum.
+// zeros MODE_DIFF writes.  Both precisions fold (dim + 2) and the
+// off-diagonal 2 into the tensor and apply the particle's constant factors
+// -wcv / (h^(dim+1) rho_a) after the sum, so no pair divides by rho_a; FP64
+// takes 1/|r|^2 from a division and |r| from a square root, FP32 takes 1/|r|
+// from one reciprocal square root.
 template <int DIM, int KT, typename PT, typename V4>
 __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
   int s = gtid();
@@ -1205,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     const PT ihd1 = ipow<DIM + 1>(inv_ha);
     const Sym3<PT> kt = tensor_of_b<PT>(P, (PT)Fi[F_BX][s], (PT)Fi[F_BY][s], (PT)Fi[F_BZ][s]);
     const PT tra = ta * rhoa;
-    // FP32: (dim + 2) k with the off-diagonal terms doubled, and the trace
+    // (dim + 2) k with the off-diagonal terms doubled, and the trace
     constexpr PT D2 = PT(DIM + 2);
     const PT cxx = D2 * kt.xx, cyy = D2 * kt.yy, czz = D2 * kt.zz;
     const PT cxy = PT(2) * D2 * kt.xy, cxz = PT(2) * D2 * kt.xz, cyz = PT(2) * D2 * kt.yz;
@@ -1247,22 +1248,26 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
         const float x = fmaf(S, inv_dr * inv_dr, -tr);
         ddt = fmaf(mb_rhob * (trb - tra), dfa * inv_dr * x, ddt);
       } else {
-        const PT mb_rhob = (PT)pb.w, trb = (PT)d.tr;
-        const PT r2 = rx * rx + ry * ry + rz * rz;
-        const PT inv_r2 = rcp_pair(r2);
-        const PT dr = sqrt_pair(r2);
-        const PT inv_dr = dr * inv_r2;
-        const PT qa = dr * inv_ha;
-        const PT dfa = K::df(qa);
-        qbad |= !(qa >= PT(0));
-        PT S, Tr;
-        tensor_contract<DIM, PT>(kt, rx, ry, rz, inv_r2, S, Tr);
-        const PT half = -wcv * dfa * ihd1 * inv_dr;  // 0.5*Gab
-        const PT contr = half * (PT(DIM + 2) * S - Tr);
-        ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+        const double mb_rhob = pb.w, trb = d.tr;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double inv_r2 = 1.0 / r2;
+        const double dr = sqrt(r2);
+        const double inv_dr = dr * inv_r2;
+        const double qa = dr * inv_ha;
+        const double dfa = K::df(qa);
+        qbad |= !(qa >= 0.0);
+        // (dim + 2) sum_lm k_lm r_l r_m over the dim x dim block, as above
+        double sx = cxx * rx, sy = 0.0;
+        if (DIM >= 2) { sx = fma(cxy, ry, sx); sy = cyy * ry; }
+        if (DIM == 3) { sx = fma(cxz, rz, sx); sy = fma(cyz, rz, sy); }
+        double S = rx * sx;
+        if (DIM >= 2) S = fma(ry, sy, S);
+        if (DIM == 3) S = fma(czz * rz, rz, S);
+        const double x = fma(S, inv_r2, -tr);
+        ddt = fma(mb_rhob * (trb - tra), dfa * inv_dr * x, ddt);
       }
     });
-    if constexpr (F32) ddt *= -wcv * ihd1 * rcp_pair(rhoa);
+    ddt *= -wcv * ihd1 * rcp_pair(rhoa);
     if (qbad) atomicOr(a.err, EB_Q);
     const PT zero = 0;
     double dh = 0.0;
