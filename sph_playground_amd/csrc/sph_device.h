@@ -46,6 +46,18 @@ __device__ __forceinline__ double rcp_pair(double x) { return 1.0 / x; }
 __device__ __forceinline__ float sqrt_pair(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sqrt_pair(double x) { return sqrt(x); }
 
+// 1/x in FP64 from the hardware reciprocal estimate (v_rcp_f64, about 23
+// correct bits) and two Newton-Raphson steps y <- y + y (1 - x y), each
+// doubling the correct bits: within about one ulp of the rounded quotient for
+// normal x.  x = 0 gives NaN where the division gives inf.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 // Periodic image coordinate exactly as the reference ghost copies compute it
 // (BC.f90:339,366,396,419,445,461): +1 -> emax + (x - emin), -1 -> emin + (x - emax).
 __device__ __forceinline__ double image_coord(double x, int s, double emin, double emax) {

@@ -672,7 +672,12 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
         const double4 sb = d.sb;
-        PT inv_dr = RSQ ? inv_dr_rsq : rcp_pair(dr);
+        // FP64: the refined reciprocal (r = 0 gives NaN either way, the
+        // kernels' derivative being 0 at q = 0)
+        PT inv_dr;
+        if constexpr (RSQ) inv_dr = inv_dr_rsq;
+        else if constexpr (std::is_same_v<PT, double>) inv_dr = rcp_nr(dr);
+        else inv_dr = rcp_pair(dr);
         if constexpr (GRAD == 1) {
           PT trb;
           if constexpr (PACK) trb = f2_hi(d.pb.w);
