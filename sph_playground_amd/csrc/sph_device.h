@@ -46,16 +46,14 @@ __device__ __forceinline__ double rcp_pair(double x) { return 1.0 / x; }
 __device__ __forceinline__ float sqrt_pair(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sqrt_pair(double x) { return sqrt(x); }
 
-// 1/x in FP64 from the hardware reciprocal estimate (v_rcp_f64, about 23
-// correct bits) and two Newton-Raphson steps y <- y + y (1 - x y), each
-// doubling the correct bits: within about one ulp of the rounded quotient for
-// normal x.  x = 0 gives NaN where the division gives inf.
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+// 1/sqrt(x) in FP64 from the hardware estimate (v_rsq_f64, about 23 correct
+// bits) and two Newton-Raphson steps y <- y (3/2 - x y^2 / 2), each doubling
+// the correct bits: within a few ulp for normal x > 0.  x = 0 gives NaN.
+__device__ __forceinline__ double rsq_nr(double x) {
+  const double hx = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(y, fma(-hx, y * y, 0.5), y);
+  return fma(y, fma(-hx, y * y, 0.5), y);
 }
 
 // Periodic image coordinate exactly as the reference ghost copies compute it

@@ -657,11 +657,15 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
       PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
-      // FP32 with grad T: 1/|r| from one reciprocal square root, |r| from it
-      constexpr bool RSQ = GRAD != 0 && std::is_same_v<PT, float>;
+      // with grad T: 1/|r| from one reciprocal square root (FP64: refined),
+      // |r| from it (FP64: 0 at r = 0, as the square root gives; NaN stays
+      // NaN)
+      constexpr bool RSQ = GRAD != 0;
       PT inv_dr_rsq = 0;
-      if constexpr (RSQ) inv_dr_rsq = __builtin_amdgcn_rsqf(rsq);
+      if constexpr (RSQ && std::is_same_v<PT, float>) inv_dr_rsq = __builtin_amdgcn_rsqf(rsq);
+      if constexpr (RSQ && std::is_same_v<PT, double>) inv_dr_rsq = rsq_nr(rsq);
       PT dr = RSQ ? rsq * inv_dr_rsq : sqrt_pair(rsq);
+      if constexpr (RSQ && std::is_same_v<PT, double>) dr = rsq == 0.0 ? 0.0 : dr;
       PT q = dr * inv_ha;
       PT f, df;
       K::f_df(q, f, df);
@@ -672,12 +676,9 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       osum += mb * (PT(DIM) * f + q * df);
       if constexpr (GRAD != 0) {
         const double4 sb = d.sb;
-        // FP64: the refined reciprocal (r = 0 gives NaN either way, the
-        // kernels' derivative being 0 at q = 0)
-        PT inv_dr;
-        if constexpr (RSQ) inv_dr = inv_dr_rsq;
-        else if constexpr (std::is_same_v<PT, double>) inv_dr = rcp_nr(dr);
-        else inv_dr = rcp_pair(dr);
+        // r = 0 gives NaN here, as 1/|r| times the kernels' derivative (0
+        // at q = 0) does
+        const PT inv_dr = inv_dr_rsq;
         if constexpr (GRAD == 1) {
           PT trb;
           if constexpr (PACK) trb = f2_hi(d.pb.w);
@@ -1192,9 +1193,9 @@ void k_loop2(Loop2Args<V4> a) {
 // their branches remain in the pair loop; the other outputs of c2 are the
 // zeros MODE_DIFF writes.  Both precisions fold (dim + 2) and the
 // off-diagonal 2 into the tensor and apply the particle's constant factors
-// -wcv / (h^(dim+1) rho_a) after the sum, so no pair divides by rho_a; FP64
-// takes 1/|r|^2 from a division and |r| from a square root, FP32 takes 1/|r|
-// from one reciprocal square root.
+// -wcv / (h^(dim+1) rho_a) after the sum, so no pair divides by rho_a; both
+// take 1/|r| from one reciprocal square root (FP64: the hardware estimate
+// refined by two Newton-Raphson steps).
 template <int DIM, int KT, typename PT, typename V4>
 __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
   int s = gtid();
@@ -1255,9 +1256,12 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
       } else {
         const double mb_rhob = pb.w, trb = d.tr;
         const double r2 = rx * rx + ry * ry + rz * rz;
-        const double inv_r2 = 1.0 / r2;
-        const double dr = sqrt(r2);
-        const double inv_dr = dr * inv_r2;
+        // 1/|r| refined from the hardware estimate; |r| = 0 at r = 0, as the
+        // square root gives (1/|r| is then NaN, and so was the pair term
+        // with a division), and NaN stays NaN
+        const double inv_dr = rsq_nr(r2);
+        const double inv_r2 = inv_dr * inv_dr;
+        const double dr = r2 == 0.0 ? 0.0 : r2 * inv_dr;
         const double qa = dr * inv_ha;
         const double dfa = K::df(qa);
         qbad |= !(qa >= 0.0);
