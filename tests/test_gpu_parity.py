@@ -111,6 +111,19 @@ def test_diffusion3d_operators_vs_oracle(gpu, ddw):
     _oracle_compare(state, st, tensor_mode=1, kappa_perp=0.1)
 
 
+@pytest.mark.parametrize("ddw,kernel", [("fab", 1), ("fab", 0), ("fab", 2), ("2nw+-", 1)])
+def test_fp64_pair_math_at_roundoff(gpu, ddw, kernel):
+    # the FP64 pair loops take 1/|r| from the hardware reciprocal square root
+    # refined by two Newton-Raphson steps: their fields (density, h, omega,
+    # grad T of loop 1, dT/dt of loop 2) must stay at summation roundoff of
+    # the oracle's IEEE square root and division: within 1e-13, ten times
+    # tighter than the other FP64 parity cases (measured: about 3e-15)
+    state, st = P.diffusion3d(16, ddw=ddw, b=(1.0, 1.0, 0.0), kernel=kernel)
+    print("case", ddw, "kernel", kernel, flush=True)
+    _oracle_compare(state, st, kernel=kernel, tol=1e-13, tensor_mode=1, kappa_perp=1e-3,
+                    fields=("es_den", "es_h", "es_om", "es_dtdx", "es_dtdy", "es_dtdz", "es_ddt"))
+
+
 def _diffusion1d(n=96, kernel=1):
     # 1D diffusion on periodic [0, 1]: T = sin(2 pi x), isotropic conduction
     box = (0.0, 1.0, 0.0, 0.0, 0.0, 0.0)
