@@ -1026,9 +1026,19 @@ void k_loop2(Loop2Args<V4> a) {
         inv_hb = (PT)B0.x; rhob = (PT)B0.y; omb = (PT)B0.z; tb = (PT)B0.w;
       }
       const PT r2 = rx * rx + ry * ry + rz * rz;
-      const PT inv_r2 = rcp_pair(r2);
-      const PT dr = sqrt_pair(r2);
-      const PT inv_dr = dr * inv_r2;
+      PT inv_r2, dr, inv_dr;
+      if constexpr (std::is_same_v<PT, double>) {
+        // 1/|r| refined from the hardware estimate; |r| = 0 at r = 0, as the
+        // square root gives (the pair terms are then NaN, as with the
+        // division), and NaN stays NaN
+        inv_dr = rsq_nr(r2);
+        inv_r2 = inv_dr * inv_dr;
+        dr = r2 == 0.0 ? 0.0 : r2 * inv_dr;
+      } else {
+        inv_r2 = rcp_pair(r2);
+        dr = sqrt_pair(r2);
+        inv_dr = dr * inv_r2;
+      }
       const PT qa = dr * inv_ha;
       PT dfa, ddfa = 0, fa = 0;
       if (hess && P.ddw == 401) K::df_ddf(qa, dfa, ddfa);
