@@ -657,10 +657,10 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
       PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
-      // with grad T: 1/|r| from one reciprocal square root (FP64: refined),
-      // |r| from it (FP64: 0 at r = 0, as the square root gives; NaN stays
-      // NaN)
-      constexpr bool RSQ = GRAD != 0;
+      // FP64, and FP32 with grad T: 1/|r| from one reciprocal square root
+      // (FP64: refined), |r| from it (FP64: 0 at r = 0, as the square root
+      // gives; NaN stays NaN)
+      constexpr bool RSQ = GRAD != 0 || std::is_same_v<PT, double>;
       PT inv_dr_rsq = 0;
       if constexpr (RSQ && std::is_same_v<PT, float>) inv_dr_rsq = __builtin_amdgcn_rsqf(rsq);
       if constexpr (RSQ && std::is_same_v<PT, double>) inv_dr_rsq = rsq_nr(rsq);
