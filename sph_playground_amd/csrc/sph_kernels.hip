@@ -956,7 +956,10 @@ void k_loop2(Loop2Args<V4> a) {
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
     const PT inv_rhoa = rcp_pair(rhoa);      // FP32 path: 1/rho_a once per particle
-    const PT rho2oma = rhoa * rhoa * oma;
+    // i-side reciprocals the pair terms multiply by: 1/(om_a rho_a) and
+    // 1/(rho_a^2 om_a), once per particle
+    const PT i_oa_ra = PT(1) / oma / rhoa;
+    const PT i_ra2_oa = i_oa_ra / rhoa;
     const PT ub2 = bax * bax + bay * bay + baz * baz;
     Sym3<PT> kta = {0, 0, 0, 0, 0, 0}, ktb_fixed = {0, 0, 0, 0, 0, 0};
     const PT mu0inv = magneto ? (PT)(1.0 / P.muzero) : PT(0);
@@ -1062,6 +1065,11 @@ void k_loop2(Loop2Args<V4> a) {
       }
       const PT vnwa = vabx * nwax + vaby * nway + vabz * nwaz;
       if (P.adden) drho += mb * vnwa;
+      // j-side 1/(rho_b^2 om_b), once per pair where a term divides by it
+      // (FP64: a division, so rho_b om_b = 0 still gives inf)
+      PT i_rb2_ob = 0;
+      if (MODE == MODE_GENERIC && (artts || hydro || magneto || (diff && P.ddw == 403)))
+        i_rb2_ob = rcp_pair(rhob * rhob * omb);
       PT cb = 0, ub = 0, kappab = 0;
       if (need_q2) { const V4 t = a.q2[d.j]; cb = (PT)t.x; ub = (PT)t.y; kappab = (PT)t.z; }
       if (artts) {
@@ -1074,23 +1082,23 @@ void k_loop2(Loop2Args<V4> a) {
           qva = PT(-0.5) * rhoa * (ca + PT(2) * fabs(dvr)) * dvr;
           qvb = PT(-0.5) * rhob * (cb + PT(2) * fabs(dvr)) * dvr;
         }
-        const PT ta_ = qva / (rhoa * rhoa) / oma, tb_ = qvb / (rhob * rhob) / omb;
+        const PT ta_ = qva * i_ra2_oa, tb_ = qvb * i_rb2_ob;
         dvx -= mb * (ta_ * nwax + tb_ * nwbx);
         dvy -= mb * (ta_ * nway + tb_ * nwby);
         dvz -= mb * (ta_ * nwaz + tb_ * nwbz);
         PT qc = 0;
         if (P.au != -1.0) {  // art_termcond, circuit2.f90:520-529
           PT vs = sqrt(fabs(pa_ - pbb) / (PT(0.5) * (rhoa + rhob)));
-          qc = (PT)P.au * (vs * (ua - ub) * PT(0.5) * (faba / oma / rhoa + fabb / omb / rhob));
+          qc = (PT)P.au * (vs * (ua - ub) * PT(0.5) * (faba * i_oa_ra + fabb * (rhob * i_rb2_ob)));
         }
-        du += mb * (dvr * qva * faba / oma / rhoa + qc);
+        du += mb * (dvr * qva * faba * i_oa_ra + qc);
       }
       if (hydro) {
-        const PT ta_ = pa_ / rho2oma, tb_ = pbb / (rhob * rhob * omb);
+        const PT ta_ = pa_ * i_ra2_oa, tb_ = pbb * i_rb2_ob;
         dvx -= mb * (ta_ * nwax + tb_ * nwbx);
         dvy -= mb * (ta_ * nway + tb_ * nwby);
         dvz -= mb * (ta_ * nwaz + tb_ * nwbz);
-        du += mb * pa_ * vnwa / (rhoa * rhoa) / oma;
+        du += mb * pa_ * vnwa * i_ra2_oa;
       }
       PT bbx = 0, bby = 0, bbz = 0;
       if (need_q4) { const V4 t = a.q4[d.j]; bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z; }
@@ -1141,11 +1149,11 @@ void k_loop2(Loop2Args<V4> a) {
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
           PT kdbz = ktb.xz * gbx + ktb.yz * gby + ktb.zz * gbz;
           if (P.ddw == 403) {
-            PT odda = PT(1) / oma / rhoa / rhoa, oddb = PT(1) / omb / rhob / rhob;
+            const PT odda = i_ra2_oa, oddb = i_rb2_ob;
             ddt += mb * ((kdax * nwax + kday * nway + kdaz * nwaz) * odda +
                          (kdbx * nwbx + kdby * nwby + kdbz * nwbz) * oddb);
           } else {
-            ddt += mb / oma / rhoa / rhoa * ((kdbx - kdax) * nwax + (kdby - kday) * nway + (kdbz - kdaz) * nwaz);
+            ddt += mb * i_ra2_oa * ((kdbx - kdax) * nwax + (kdby - kday) * nway + (kdbz - kdaz) * nwaz);
           }
         }
       }
@@ -1158,12 +1166,11 @@ void k_loop2(Loop2Args<V4> a) {
         PT mb_x = ((PT(0.5) * b2b - bbx * bbx) * nwbx - bbx * bby * nwby - bbx * bbz * nwbz) * mu0inv;
         PT mb_y = (-bby * bbx * nwbx + (PT(0.5) * b2b - bby * bby) * nwby - bby * bbz * nwbz) * mu0inv;
         PT mb_z = (-bbz * bbx * nwbx - bbz * bby * nwby + (PT(0.5) * b2b - bbz * bbz) * nwbz) * mu0inv;
-        PT ib = PT(1) / (rhob * rhob * omb);
-        dvx -= mb * (ma_x / rho2oma + mb_x * ib);
-        dvy -= mb * (ma_y / rho2oma + mb_y * ib);
-        dvz -= mb * (ma_z / rho2oma + mb_z * ib);
+        dvx -= mb * (ma_x * i_ra2_oa + mb_x * i_rb2_ob);
+        dvy -= mb * (ma_y * i_ra2_oa + mb_y * i_rb2_ob);
+        dvz -= mb * (ma_z * i_ra2_oa + mb_z * i_rb2_ob);
         PT bnwa = bax * nwax + bay * nway + baz * nwaz;
-        PT f = PT(1) / (rhoa * oma) * mb;
+        PT f = i_oa_ra * mb;
         dbx -= f * (vabx * bnwa - bbx * vnwa);
         dby -= f * (vaby * bnwa - bby * vnwa);
         dbz -= f * (vabz * bnwa - bbz * vnwa);
