@@ -955,7 +955,7 @@ void k_loop2(Loop2Args<V4> a) {
     // i-side constants
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
-    const PT inv_rhoa = rcp_pair(rhoa);      // FP32 path: 1/rho_a once per particle
+    const PT inv_rhoa = rcp_pair(rhoa);      // 1/rho_a once per particle
     // i-side reciprocals the pair terms multiply by: 1/(om_a rho_a) and
     // 1/(rho_a^2 om_a), once per particle
     const PT i_oa_ra = PT(1) / oma / rhoa;
@@ -1131,7 +1131,8 @@ void k_loop2(Loop2Args<V4> a) {
           if (P.ddw == 401) {
             // hessian_ddw_cart, kernel.f90:377-417
             PT cst = wcv * ihd1 * inv_ha;
-            PT dfq = dfa / qa;
+            // df / q with 1/q = h_a / |r| (q = 0 gives NaN, as the division)
+            PT dfq = dfa * inv_dr * (PT)ha_d;
             contr = cst * ((ddfa - dfq) * S + dfq * Tr);
           } else {
             // hessian_fab_cart / hessian_fw_cart, kernel.f90:460-539
@@ -1140,10 +1141,9 @@ void k_loop2(Loop2Args<V4> a) {
             contr = half * (PT(DIM + 2) * S - Tr);
           }
           if constexpr (MODE == MODE_DIFF) {
-            if constexpr (std::is_same_v<PT, float>) ddt += mb_rhob * inv_rhoa * (trb - ta * rhoa) * contr;
-            else ddt += mb_rhob / rhoa * (trb - ta * rhoa) * contr;
+            ddt += mb_rhob * inv_rhoa * (trb - ta * rhoa) * contr;
           }
-          else ddt += mb / rhob / rhoa * (tb * rhob - ta * rhoa) * contr;
+          else ddt += mb / rhob * inv_rhoa * (tb * rhob - ta * rhoa) * contr;
         } else if (MODE == MODE_GENERIC) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
