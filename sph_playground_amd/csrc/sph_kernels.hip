@@ -686,12 +686,20 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
           PT w = mb * (trb - (PT)ta * (PT)rhoa) * df * inv_dr;
           g0 += w * r0; g1 += w * r1; g2 += w * r2;
         } else {
-          PT inv_hb = (PT)(1.0 / sb.x);
+          PT inv_hb, i_ob_rb2;  // 1/h_b and 1/(om_b rho_b^2)
+          if constexpr (std::is_same_v<PT, double>) {
+            // FP64: both from one division of their product
+            const double r = 1.0 / (sb.x * (sb.z * sb.y * sb.y));
+            inv_hb = (sb.z * sb.y * sb.y) * r;
+            i_ob_rb2 = sb.x * r;
+          } else {
+            inv_hb = (PT)(1.0 / sb.x);
+            i_ob_rb2 = PT(1) / ((PT)sb.z * (PT)sb.y * (PT)sb.y);
+          }
           PT qb = dr * inv_hb;
           PT dfb = K::df(qb);
           PT wa = mb * df * inv_dr;
-          PT wb = mb * (PT)sb.w * (PT)rhoa / ((PT)sb.z * (PT)sb.y * (PT)sb.y) * dfb * inv_dr *
-                  ipow<DIM + 1>(inv_hb);
+          PT wb = mb * (PT)sb.w * (PT)rhoa * i_ob_rb2 * dfb * inv_dr * ipow<DIM + 1>(inv_hb);
           g0 += wa * r0; g1 += wa * r1; g2 += wa * r2;
           h0x += wb * r0; h0y += wb * r1; h0z += wb * r2;
         }
