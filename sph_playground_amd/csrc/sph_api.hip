@@ -184,6 +184,7 @@ struct sph_ctx {
   // cell keys kept from an earlier step: valid while the positions they were
   // computed from are unchanged (pos_version) and the grid is the same
   uint64_t pos_version = 1;       // bumped by every write of positions (load, leapfrog, wrap)
+  bool keep_q0 = false;           // the step writes the FP64 q0 pack for sph_diff_prepare
   uint64_t vb_version = 1;        // bumped by every write of v or B (load, leapfrog)
   uint64_t flags_version = 0;     // vb_version step_flags were computed from
   bool flags_pending = false;     // this step's k_prep computes them (read with the search)
@@ -844,8 +845,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   // loop 2 reads v only where it is non-zero and B only where it varies
   const long long fl = c->step_flags;
   // q0 feeds the generic loop, and in FP64 also the adjoint and batched
-  // operators built after the step; the FP32 diffusion loops read pd alone
-  V4 *q0 = std::is_same_v<V4, float4> && !generic ? nullptr : (V4 *)c->q[0];
+  // operators sph_diff_prepare builds after its step; the diffusion loops
+  // read pd alone
+  V4 *q0 = !generic && (std::is_same_v<V4, float4> || !c->keep_q0) ? nullptr : (V4 *)c->q[0];
   V4 *q1 = want_v && (generic || (fl & 1)) ? (V4 *)c->q[1] : nullptr;
   V4 *q2 = want_q2 ? (V4 *)c->q[2] : nullptr;
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
@@ -1534,7 +1536,10 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
   if (c->opt.precision != 0 || c->n_fixed != 0) return SPH_ERR_UNSUPPORTED;
   Phys P = call_phys(c, state);
   if (!P.diff || P.fld || !(P.ddw == 400 || P.ddw == 401 || P.ddw == 402)) return SPH_ERR_UNSUPPORTED;
-  RCHK(run_step(c, state));  // neighbours, Newton h, EOS, loop 2: h, rho, q0 now frozen
+  c->keep_q0 = true;
+  const int step_rc = run_step(c, state);  // neighbours, Newton h, EOS, loop 2: h, rho, q0 now frozen
+  c->keep_q0 = false;
+  RCHK(step_rc);
   c->diff_phys = P;
   const int n = (int)c->n;
   unsigned G = grid_for(n);
