@@ -42,8 +42,9 @@ extern "C" {
 typedef struct sph_ctx sph_ctx;
 
 typedef struct {
-  int precision;      /* 0 = FP64 (reference arithmetic); 1 = mixed: FP64 positions,
-                         separations, Newton update and energy sum, FP32 pair math */
+  int precision;      /* 0 = FP64 throughout (the pair loops take 1/|r| from the hardware
+                         reciprocal square root refined to a few ulp); 1 = mixed: FP64
+                         positions, separations, Newton update and energy sum, FP32 pair math */
   int tensor_mode;    /* 0 = reference diffusion tensor (difcond*I or difcond*diag(b^2),
                          circuit2.f90:194-206); 1 = kappa_par*bb + kappa_perp*(I-bb) */
   double kappa_perp;  /* perpendicular conductivity for tensor_mode 1 */

@@ -28,7 +28,7 @@ class Engine:
 
     Parameters mirror the reference's knobs: ``state`` (49 statevars),
     ``kernel`` ('m4' | 'm6' | 'gauss'), plus the device-path options:
-    ``precision`` ('fp64' reference arithmetic | 'mixed'), ``tensor``
+    ``precision`` ('fp64' throughout | 'mixed'), ``tensor``
     ('reference' diffusion tensor | 'full' kappa_par bb + kappa_perp (I-bb)),
     ``kappa_perp``, ``max_sweeps`` (100), ``newton_tol`` (1e-8 fp64, 1e-6
     mixed), ``neigh_cap`` (reference maxneibnum).
