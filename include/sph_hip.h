@@ -103,9 +103,12 @@ int sph_step(sph_ctx *ctx, const double state[49], double *maxconsenrg);
 int sph_neighbours(sph_ctx *ctx, int64_t *off, int32_t *j, int8_t *code, int64_t cap_pairs,
                    int64_t *n_pairs);
 
-/* One leapfrog step around iterate (main.f90:216-362, no STS): initial
- * iterate, dt = min(timestep.calc dt, dtcap) (timestep.f90:40-166),
- * predictor, iterate, corrector, all on the resident state. */
+/* One pass of the leapfrog loop around iterate (main.f90:216-362, no STS):
+ * dt = min(timestep.calc dt, dtcap) (timestep.f90:40-166), predictor,
+ * iterate, corrector, all on the resident state.  The initial iterate of
+ * main.f90:219 runs first only when the resident store does not hold the
+ * derivatives of its current state (after sph_load, or after a failed step);
+ * so N calls after a load are the reference's initial iterate plus N passes. */
 int sph_leapfrog(sph_ctx *ctx, const double state[49], double dtcap, double *dt_used);
 
 /* One RKL2 super-time-step of T (sts_integrator.f90:87-124) with s stages on

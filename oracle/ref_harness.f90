@@ -5,13 +5,18 @@
 ! placement, CLI parsing) with a direct fill of `statevars` and `store`, then
 ! calls exactly the reference entry points:
 !   mode 0: iterate(n, store, dedt)                        iterator.f90:44
-!   mode 1: one leapfrog step as main.f90:216-362 (no STS): iterate, calc,
-!           predictor, iterate, corrector; dt = min(calc dt, dtcap)
+!   mode 1: leapfrog steps as main.f90:216-362 (no STS): iterate once, then
+!           max(1, sts_s) times calc, predictor, iterate, corrector;
+!           dt = min(calc dt, dtcap) (the last step's dt is written)
 !   mode 2: sts_integrate(n, s, store, dt)                 sts_integrator.f90:87
 !   mode 3: iterate twice, timing only the second call (steady state: the
 !           first call also runs initIterate's extra neighbour search)
 !   mode 4: mode 1, then update_radenergy(n, store, dt) as main.f90:367-369
 !   mode 5: update_radenergy(n, store, dtcap) alone        rad_exchange.f90:9
+!   mode 6: createFixedBorders(store, sts_s) on the real particles (BC.f90:477,
+!           sts_s = the ebc_* side), as the mti / fld_gauss set-ups call it
+!           (IC/IC.f90:170, IC/setups/place_fld_gauss.f90:22); writes the
+!           store with the new fixed particles appended, nothing else runs
 !   mode + 100: same, without writing the neighbour lists
 !
 ! Input  (stream, little endian): int32 mode, n_real, n_fixed, sts_s;
@@ -22,9 +27,9 @@
 !        (sx+1) + 3*(sy+1) + 9*(sz+1) with s = sign(x_ghost - x_source) per axis.
 program ref_harness
   use const
-  use state,          only: setState, getdim
+  use state,          only: setState, getdim, getPartNumber
   use kernel,         only: initkernel
-  use BC,             only: initBorders, getCrossRef
+  use BC,             only: initBorders, getCrossRef, createFixedBorders
   use iterator,       only: iterate
   use timestep,       only: calc
   use sts_integrator, only: sts_integrate, sts_init => init
@@ -35,7 +40,7 @@ program ref_harness
 
   character(len=1024) :: fin, fout
   integer :: mode, nreal, nfixed, sts_s, ntot, dim, i, k, j, rj, code, cnt, ax
-  integer :: sgn(3), dumpnb
+  integer :: sgn(3), dumpnb, step, nsteps, rpn, fpn
   integer(8) :: npairs
   real :: dtcap, dedt, dt, dthydro, dtdiff, dtfld, t0, t1
   real :: st(ec_total)
@@ -71,31 +76,38 @@ program ref_harness
   case (0, 3)
     call iterate(ntot, store, dedt)
   case (1, 4)
-    ! main.f90:219 initial accelerations, then one pass of the loop body
+    ! main.f90:219 initial accelerations, then passes of the loop body
     call iterate(ntot, store, dedt)
-    call calc(store, dthydro, dtdiff, dtfld, dt)
-    dt = min(dt, dtcap)
+    nsteps = max(1, sts_s)
     allocate(tr(3,ntot), tv(3,ntot), ta(3,ntot), tdu(ntot), tdh(ntot), tddt(ntot), tdb(3,ntot))
-    tr(:,:)  = store(es_rx:es_rz,1:ntot)
-    tv(:,:)  = store(es_vx:es_vz,1:ntot)
-    ta(:,:)  = store(es_ax:es_az,1:ntot)
-    tdu(:)   = store(es_du,1:ntot)
-    tdh(:)   = store(es_dh,1:ntot)
-    tddt(:)  = store(es_ddt,1:ntot)
-    tdb(:,:) = store(es_dbx:es_dbz,1:ntot)
-    store(es_rx:es_rz,1:ntot) = store(es_rx:es_rz,1:ntot) + dt*tv(:,:) + 0.5*dt*dt*ta(:,:)
-    store(es_vx:es_vz,1:ntot) = tv(:,:) + dt*ta(:,:)
-    store(es_u,1:ntot) = store(es_u,1:ntot) + dt*tdu(:)
-    store(es_h,1:ntot) = store(es_h,1:ntot) + dt*tdh(:)
-    store(es_t,1:ntot) = store(es_t,1:ntot) + dt*tddt(:)
-    store(es_bx:es_bz,1:ntot) = store(es_bx:es_bz,1:ntot) + dt*tdb(:,:)
-    call iterate(ntot, store, dedt)
-    store(es_vx:es_vz,1:ntot) = store(es_vx:es_vz,1:ntot) + 0.5*dt*(store(es_ax:es_az,1:ntot) - ta(:,:))
-    store(es_u,1:ntot) = store(es_u,1:ntot) + 0.5*dt*(store(es_du,1:ntot) - tdu(:))
-    store(es_h,1:ntot) = store(es_h,1:ntot) + 0.5*dt*(store(es_dh,1:ntot) - tdh(:))
-    store(es_bx:es_bz,1:ntot) = store(es_bx:es_bz,1:ntot) + 0.5*dt*(store(es_dbx:es_dbz,1:ntot) - tdb(:,:))
-    store(es_t,1:ntot) = store(es_t,1:ntot) + 0.5*dt*(store(es_ddt,1:ntot) - tddt(:))
-    if (mode == 4) call update_radenergy(ntot, store, dt)
+    do step = 1, nsteps
+      call calc(store, dthydro, dtdiff, dtfld, dt)
+      dt = min(dt, dtcap)
+      tr(:,:)  = store(es_rx:es_rz,1:ntot)
+      tv(:,:)  = store(es_vx:es_vz,1:ntot)
+      ta(:,:)  = store(es_ax:es_az,1:ntot)
+      tdu(:)   = store(es_du,1:ntot)
+      tdh(:)   = store(es_dh,1:ntot)
+      tddt(:)  = store(es_ddt,1:ntot)
+      tdb(:,:) = store(es_dbx:es_dbz,1:ntot)
+      store(es_rx:es_rz,1:ntot) = store(es_rx:es_rz,1:ntot) + dt*tv(:,:) + 0.5*dt*dt*ta(:,:)
+      store(es_vx:es_vz,1:ntot) = tv(:,:) + dt*ta(:,:)
+      store(es_u,1:ntot) = store(es_u,1:ntot) + dt*tdu(:)
+      store(es_h,1:ntot) = store(es_h,1:ntot) + dt*tdh(:)
+      store(es_t,1:ntot) = store(es_t,1:ntot) + dt*tddt(:)
+      store(es_bx:es_bz,1:ntot) = store(es_bx:es_bz,1:ntot) + dt*tdb(:,:)
+      call iterate(ntot, store, dedt)
+      store(es_vx:es_vz,1:ntot) = store(es_vx:es_vz,1:ntot) + 0.5*dt*(store(es_ax:es_az,1:ntot) - ta(:,:))
+      store(es_u,1:ntot) = store(es_u,1:ntot) + 0.5*dt*(store(es_du,1:ntot) - tdu(:))
+      store(es_h,1:ntot) = store(es_h,1:ntot) + 0.5*dt*(store(es_dh,1:ntot) - tdh(:))
+      store(es_bx:es_bz,1:ntot) = store(es_bx:es_bz,1:ntot) + 0.5*dt*(store(es_dbx:es_dbz,1:ntot) - tdb(:,:))
+      store(es_t,1:ntot) = store(es_t,1:ntot) + 0.5*dt*(store(es_ddt,1:ntot) - tddt(:))
+      if (mode == 4) call update_radenergy(ntot, store, dt)
+    end do
+  case (6)
+    call createFixedBorders(store, sts_s)
+    call getPartNumber(r=rpn, f=fpn)
+    ntot = rpn + fpn
   case (5)
     dt = dtcap
     call update_radenergy(ntot, store, dt)
@@ -113,10 +125,10 @@ program ref_harness
   write(11) ntot
   write(11) store(:, 1:ntot)
   write(11) dt
-  if (dumpnb == 0 .or. mode == 5) then
-    ! only the pair count P = sum_i |N_i| (none: mode 5 searches nothing)
+  if (dumpnb == 0 .or. mode == 5 .or. mode == 6) then
+    ! only the pair count P = sum_i |N_i| (none: modes 5 and 6 search nothing)
     npairs = 0
-    if (mode /= 5) then
+    if (mode /= 5 .and. mode /= 6) then
       do i = 1, nreal
         npairs = npairs + xgetneighnumber(i)
       end do

@@ -129,6 +129,12 @@ ebc_all = 200
 ebc_x = 201
 ebc_y = 202
 ebc_z = 203
+ebc_x1 = 204
+ebc_y1 = 205
+ebc_z1 = 206
+ebc_x2 = 207
+ebc_y2 = 208
+ebc_z2 = 209
 
 # equation sets, const.f90:143-151
 eeq_hydro = 300

@@ -198,6 +198,10 @@ struct sph_ctx {
   int64_t keys_n = -1, keys_nkeys = -1;
   sph_stats stats{};
   bool have_step = false;
+  // the resident store holds the derivatives of the last successful iterate
+  // of this state (nothing loaded since): a leapfrog pass then starts at its
+  // predictor, as every pass after main.f90:219's initial iterate does
+  bool derivs_ready = false;
   hipEvent_t ev[9]{};             // 8: the short waits of read_ints
   // differentiable diffusion (sph_diff_*)
   bool diff_ready = false;
@@ -921,6 +925,9 @@ static int run_step(sph_ctx *c, const double *state) {
   // a new step changes the geometry the differentiable operator was prepared on
   c->diff_ready = false;
   c->w_valid = false;
+  c->fw_steps = 0;
+  c->fw_tail = false;
+  c->derivs_ready = false;
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -963,6 +970,7 @@ static int run_step(sph_ctx *c, const double *state) {
   c->stats.ms_loop2 = t23;
   c->stats.ms_total = t01 + t12 + t23;
   c->have_step = true;
+  c->derivs_ready = rc == SPH_OK;
   return rc;
 }
 
@@ -988,10 +996,17 @@ static int load_impl(sph_ctx *c, int64_t n, const double *store, int64_t ld, int
   c->pos_version++;
   c->vb_version++;
   c->h_version++;
+  // the particles (and their order) are new: lists, the prepared operator,
+  // its weights and the recorded forward belong to the old ones
+  c->have_step = false;
+  c->derivs_ready = false;
+  c->diff_ready = false;
+  c->w_valid = false;
+  c->fw_steps = 0;
+  c->fw_tail = false;
   hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, ld, (int)n, c->fields(),
                      c->orig[0]);
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->have_step = false;
   return SPH_OK;
 }
 
@@ -1258,7 +1273,9 @@ int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_us
   if (!c || !state || c->n <= 0) return SPH_ERR_BAD_ARG;
   (void)hipSetDevice(c->device);
   Phys P = call_phys(c, state);
-  RCHK(run_step(c, state));  // main.f90:219
+  // main.f90:219: the initial iterate, once; later passes take the
+  // derivatives the previous pass's iterate left (main.f90:323-329)
+  if (!c->derivs_ready) RCHK(run_step(c, state));
   double dt;
   RCHK(calc_dt(c, P, &dt));
   dt = std::min(dt, dtcap);

@@ -48,8 +48,13 @@ class Engine:
         self._h = h
         self.kernel = kid
         self.precision = precision
+        self.device = int(device)
         self.n_real = int(self._state[C.ec_realpn - 1])
         self.n = 0
+        # counts read by the autograd wrapper: calls that replace the resident
+        # particles or their geometry, and recorded sph_diff_forward runs
+        self._geometry_gen = 0
+        self._diff_forward_gen = 0
 
     # -- the reference's entry point ------------------------------------
     def iterate(self, store, state=None):
@@ -58,6 +63,7 @@ class Engine:
         st = self._check_store(store)
         s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
         e = ct.c_double(0.0)
+        self._geometry_gen += 1
         check(lib().sph_iterate(self._h, st.shape[0], _dptr(st), st.shape[1], _dptr(s), ct.byref(e)),
               "sph_iterate")
         self.n = st.shape[0]
@@ -68,10 +74,12 @@ class Engine:
     # -- resident use ---------------------------------------------------
     def load(self, store):
         st = self._check_store(store)
+        self._geometry_gen += 1
         check(lib().sph_load(self._h, st.shape[0], st.ctypes.data, st.shape[1], 0), "sph_load")
         self.n = st.shape[0]
 
     def load_device(self, ptr, n, ld=C.es_total):
+        self._geometry_gen += 1
         check(lib().sph_load(self._h, n, ct.c_void_p(ptr), ld, 1), "sph_load")
         self.n = n
 
@@ -85,18 +93,24 @@ class Engine:
 
     def step(self, state=None):
         s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        self._geometry_gen += 1
         check(lib().sph_step(self._h, _dptr(s), None), "sph_step")
 
     def leapfrog(self, dtcap, state=None):
-        """One leapfrog step (main.f90:216-362) on the resident state; returns dt."""
+        """One pass of the leapfrog loop (main.f90:241-362) on the resident
+        state; returns dt.  The initial iterate (main.f90:219) runs first when
+        the state was loaded since the last step, so N calls after ``load``
+        are the reference's run of N steps."""
         s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
         dt = ct.c_double(0.0)
+        self._geometry_gen += 1
         check(lib().sph_leapfrog(self._h, _dptr(s), float(dtcap), ct.byref(dt)), "sph_leapfrog")
         return dt.value
 
     def sts(self, s_stages, dt, state=None):
         """One RKL2 super-step of T (sts_integrator.f90:87-124) on the resident state."""
         s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        self._geometry_gen += 1
         check(lib().sph_sts(self._h, _dptr(s), int(s_stages), float(dt)), "sph_sts")
 
     def rad_exchange(self, dt, state=None):

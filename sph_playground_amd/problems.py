@@ -273,14 +273,22 @@ def batch2d_problem(spec):
 def random_slab3d(n, seed=3, box=(-1.0, 1.0, 0.0, 0.125, 0.0, 0.125), hfac=1.2,
                   equations="magnetohydro", b=(0.75, 1.0, 0.0), kernel=C.KERNEL_M6,
                   fld=False, adden="yes", artts="yes", au=1.0, process="fullyperiodic",
-                  muzero=1.0, velocity_seed=None):
+                  muzero=1.0, velocity_seed=None, lattice=None):
     """Uniform-random particles in a slab with a left/right shock-tube state
     (rho 1 | 0.125, P 1 | 0.1, E_rad 1 | 0.1, equal number density so m ~ rho),
     uniform B.  ``fld=True`` turns on diffusion+FLD (manual equation set, the
-    only way the reference enables FLD in loop 2, state.f90:140-145)."""
+    only way the reference enables FLD in loop 2, state.f90:140-145).
+    ``lattice=(nx, ny, nz)`` places the n = nx ny nz particles on a cell-centred
+    lattice jittered by 0.05 of its spacing (Gaussian, ``seed``) instead."""
     lo = np.array(box[0::2])
     hi = np.array(box[1::2])
-    u = uniform(seed, 3 * n).reshape(n, 3)
+    if lattice is None:
+        u = uniform(seed, 3 * n).reshape(n, 3)
+    else:
+        nl = np.array(lattice)
+        assert int(np.prod(nl)) == n
+        idx = np.indices(tuple(nl[::-1])).reshape(3, -1)[::-1].T.astype(np.float64)
+        u = (idx + 0.5 + 0.05 * gaussian(seed, 3 * n).reshape(n, 3)) / nl
     pos = lo + (hi - lo) * u
     vol = float(np.prod(hi - lo))
     left = pos[:, 0] < 0.5 * (lo[0] + hi[0])
@@ -310,3 +318,144 @@ def random_slab3d(n, seed=3, box=(-1.0, 1.0, 0.0, 0.125, 0.0, 0.125), hfac=1.2,
                        kernel=kernel, process=process, eqon=eqon, muzero=muzero,
                        bordersize=(int(C.KRAD[kernel] * hfac) + 1) * dxm * 2.0)
     return state, st
+
+
+def fixed_walls(state, st, side=C.ebc_all, return_source=False):
+    """Append fixed wall particles to ``st`` as ``createFixedBorders(store,
+    side)`` does (BC.f90:477-640, after findInsideBorderParticles,
+    BC.f90:196-306): every real or fixed particle strictly inside the band
+    (emin, emin + bordersize) of a walled axis is mirrored to
+    emin + (emin - x) (and likewise at emax), x walls first, then y, then z,
+    with a mirrored particle that lies in the y or z band mirrored again by
+    the later axis (the corner walls).  Only positions and the type are set
+    (the reference leaves every other field to the set-up); the new rows copy
+    nothing else and are zero.  Returns (state, store) with ec_fixedpn
+    updated, and with ``return_source`` also the index of the particle each
+    new row mirrors (traced back to a particle of ``st``)."""
+    state = np.array(state, dtype=np.float64, copy=True)
+    dim = int(state[C.ec_dim - 1])
+    bs = state[C.ec_bordsize - 1]
+    emin = np.array([state[C.ec_xmin - 1], state[C.ec_ymin - 1], state[C.ec_zmin - 1]])
+    emax = np.array([state[C.ec_xmax - 1], state[C.ec_ymax - 1], state[C.ec_zmax - 1]])
+    bmin, bmax = emin + bs, emax - bs
+    pos = [np.array(p) for p in st[:, 0:3]]
+    types = st[:, C.es_type - 1].astype(int)
+    lists = {k: [] for k in ("x1", "x2", "y1", "y2", "z1", "z2")}
+    for i, p in enumerate(pos):
+        if types[i] not in (C.ept_real, C.ept_fixed, C.ept_fixedreal):
+            continue
+        for a, ax in enumerate("xyz"[:dim]):
+            if p[a] < bmin[a]:
+                if p[a] > emin[a]:
+                    lists[ax + "1"].append(i)
+            elif p[a] > bmax[a]:
+                if p[a] < emax[a]:
+                    lists[ax + "2"].append(i)
+    new = []
+    origin = list(range(len(pos)))
+
+    def mirror(src, a, upper):
+        q = pos[src].copy()
+        q[a] = emax[a] + (emax[a] - q[a]) if upper else emin[a] + (emin[a] - q[a])
+        pos.append(q)
+        new.append(q)
+        origin.append(origin[src])
+        k = len(pos) - 1
+        for b in range(a + 1, dim):  # later axes see the new particle in their bands
+            if q[b] < bmin[b]:
+                lists["xyz"[b] + "1"].append(k)
+            elif q[b] > bmax[b]:
+                lists["xyz"[b] + "2"].append(k)
+
+    def walls(ax, a, ends):
+        for e in ends:
+            for src in list(lists[ax + e]):
+                mirror(src, a, e == "2")
+
+    if side in (C.ebc_all, C.ebc_x):
+        walls("x", 0, "12")
+    if dim > 1:
+        if side in (C.ebc_all, C.ebc_y, C.ebc_y1):
+            walls("y", 1, "1")
+        if side in (C.ebc_all, C.ebc_y, C.ebc_y2):
+            walls("y", 1, "2")
+        if dim == 3 and side in (C.ebc_all, C.ebc_z):
+            walls("z", 2, "12")
+    extra = np.zeros((len(new), st.shape[1]))
+    if new:
+        extra[:, 0:3] = np.array(new)
+    extra[:, C.es_type - 1] = C.ept_fixed
+    out = np.concatenate([st, extra])
+    state[C.ec_fixedpn - 1] = (out.shape[0] - int(state[C.ec_realpn - 1]))
+    if return_source:
+        return state, out, np.array(origin[st.shape[0]:], dtype=np.int64)
+    return state, out
+
+
+def hydro2d_mti_walls(n_side=16, seed=1, jitter=0.01, hfac=1.0, kernel=C.KERNEL_M6, au=1.0,
+                      velocity_seed=None):
+    """The 2D hydro lattice of ``hydro2d`` run as the reference's legacy mti
+    set-up (IC/IC.f90:149-174, iterator.f90:134-143): periodic along x,
+    fixed walls below and above y (createFixedBorders(store, ebc_y)), gravity
+    a_y -= 1.  The wall particles carry the real particles' mass, density,
+    h, u, p and c (the set-up initialises every real and fixed particle) and
+    no velocity."""
+    state, st = hydro2d(n_side, seed=seed, jitter=jitter, hfac=hfac, kernel=kernel, au=au)
+    if velocity_seed is not None:
+        n = st.shape[0]
+        st[:, C.es_vx - 1:C.es_vy] = 0.05 * gaussian(velocity_seed, 2 * n).reshape(n, 2)
+    state[C.ec_process - 1] = C.epc_backcompatibility
+    state[C.ec_ics - 1] = C.ett_mti
+    state, full, src = fixed_walls(state, st, C.ebc_y, return_source=True)
+    nr = st.shape[0]
+    for f in (C.es_m, C.es_den, C.es_h, C.es_u, C.es_p, C.es_c, C.es_om):
+        full[nr:, f - 1] = st[src, f - 1]
+    return state, full
+
+
+def fld2d_gauss_walls(n_side=20, hfac=1.2, kernel=C.KERNEL_M6, ddw="fab"):
+    """The reference's fld_gauss set-up in 2D (IC/setups/place_fld_gauss.f90,
+    init_fld_gauss.f90): a uniform lattice in [-10, 10]^2 with fixed walls on
+    every side (createFixedBorders(store, ebc_all)), no periodic images and no
+    EOS (iterator.f90:88-91); rho = 1, P = 1, gamma = 2, opacity 0.1 and
+    xi = T a Gaussian pulse (width 5/3 on this box, plus a floor of 1e-3),
+    with diffusion + flux-limited radiation on (manual equation set).  Every
+    particle, real or fixed, gets the set-up's fields."""
+    box = (-10.0, 10.0, -10.0, 10.0, 0.0, 0.0)
+    pos, dx = lattice_positions(2, n_side, box)
+    n = pos.shape[0]
+    st = empty_store(n)
+    st[:, 0:3] = pos
+    gamma = 2.0
+    state = make_state(dim=2, equations="manual", ddw=ddw, adden="yes", artts="no", hfac=hfac, box=box,
+                       spacing=dx, n_real=n, gamma=gamma, kernel=kernel, process="backcompatibility",
+                       ics=C.ett_fld_gauss, eqon=dict(diff=True, fld=True))
+    state, full = fixed_walls(state, st, C.ebc_all)
+    p = full[:, 0:3]
+    full[:, C.es_h - 1] = hfac * dx
+    full[:, C.es_m - 1] = dx ** 2
+    full[:, C.es_den - 1] = 1.0
+    full[:, C.es_p - 1] = 1.0
+    r2 = (p[:, 0] ** 2 + p[:, 1] ** 2) / 25.0
+    full[:, C.es_t - 1] = np.exp(-0.5 * r2 / (1.0 / 3.0) ** 2) + 1e-3
+    full[:, C.es_u - 1] = 1.0 / (gamma - 1.0)
+    full[:, C.es_kappa - 1] = 0.1
+    full[:, C.es_om - 1] = 1.0
+    return state, full
+
+
+def shock_slab_walls(n, seed=3, **kw):
+    """``random_slab3d`` run as the reference's hydroshock set-up
+    (IC/IC.f90:227-253, iterator.f90:102-112): periodic in y and z, fixed
+    walls beyond both x ends (createFixedBorders(store, ebc_x)).  A wall
+    particle carries the fields of the particle it mirrors (the left or right
+    state of its side) and no velocity."""
+    kw.setdefault("process", "backcompatibility")
+    state, st = random_slab3d(n, seed=seed, **kw)
+    state[C.ec_ics - 1] = C.ett_hydroshock
+    state, full, src = fixed_walls(state, st, C.ebc_x, return_source=True)
+    for f in range(4, C.es_total + 1):
+        if f in (C.es_vx, C.es_vy, C.es_vz, C.es_type):
+            continue
+        full[n:, f - 1] = st[src, f - 1]
+    return state, full

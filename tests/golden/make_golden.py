@@ -10,8 +10,14 @@ inputs and stores inputs + outputs as compressed .npz fixtures:
   store_out  (n,32)  store after the call
   nb_off/nb_j/nb_code  neighbour lists of the last iterate: real index of j
                      (0-based) and image code (sx+1)+3(sy+1)+9(sz+1)
-  meta       json: kernel, mode (0 iterate, 1 leapfrog step, 2 STS), sts_s,
-             dtcap, dt_used, generator call
+  meta       json: kernel, mode (0 iterate, 1 leapfrog steps, 2 STS), sts_s,
+             dtcap, dt_used (of the last step), steps (leapfrog passes),
+             walls (the ebc_* side of the fixed walls, 0 = none), generator call
+
+Stores with fixed walls are built by problems.fixed_walls; before such a case
+is written, the wall positions and types are checked bit for bit against the
+reference's own createFixedBorders (BC.f90:477-640, harness mode 6) run on the
+case's real particles.
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -38,7 +44,8 @@ def _slab(**kw):
 
 
 def cases():
-    """(name, kernel, mode, sts_s, dtcap, builder, state overrides)."""
+    """(name, kernel, mode, sts_s, dtcap, builder, state overrides).  For
+    mode 1, sts_s is the number of leapfrog passes (0 = 1)."""
     c = []
     c.append(("hydro2d_m6_32", 1, 0, 0, 0.0, lambda: P.hydro2d(32), {}))
     c.append(("hydro2d_m6_32_leapfrog", 1, 1, 0, 1e-3, lambda: P.hydro2d(32), {}))
@@ -79,7 +86,43 @@ def cases():
               lambda: P.random_slab3d(3000, seed=7, hfac=0.85, equations="hyrad"), {}))
     c.append(("hyrad3d_3000_leapfrog_radexch", 1, 4, 0, 1.0,
               lambda: P.random_slab3d(3000, seed=7, hfac=0.85, velocity_seed=9, equations="hyrad"), {}))
+    # several leapfrog passes: the initial iterate runs once (main.f90:219),
+    # then each pass is calc, predictor, iterate, corrector (main.f90:241-362)
+    c.append(("hydro2d_m6_32_leapfrog3", 1, 1, 3, 1e-3, lambda: P.hydro2d(32), {}))
+    # fixed walls (createFixedBorders): the mti set-up's y walls (IC/IC.f90:170),
+    # the fld_gauss set-up's walls on every side (place_fld_gauss.f90:22), the
+    # hydroshock set-up's x walls (IC/IC.f90:253); fixed particles are
+    # neighbours and are never updated (eos.f90:27-32, circuit1/2 real loops)
+    c.append(("hydro2d_mti_walls_16", 1, 0, 0, 0.0, lambda: P.hydro2d_mti_walls(16), {}))
+    c.append(("hydro2d_mti_walls_16_leapfrog3", 1, 1, 3, 1e-3,
+              lambda: P.hydro2d_mti_walls(16, velocity_seed=9), {}))
+    c.append(("fld2d_gauss_walls_20", 1, 0, 0, 0.0, lambda: P.fld2d_gauss_walls(20), {}))
+    c.append(("mhd3d_hydroshock_walls_2000", 1, 0, 0, 0.0,
+              lambda: P.shock_slab_walls(2000, seed=3, hfac=1.0, velocity_seed=9), {}))
+    c.append(("fld3d_hydroshock_walls_2000", 1, 0, 0, 0.0,
+              lambda: P.shock_slab_walls(2000, seed=5, hfac=1.0, velocity_seed=9, fld=True), {}))
+    c.append(("mhd3d_hydroshock_walls_2000_leapfrog2", 1, 1, 2, 1e-3,
+              lambda: P.shock_slab_walls(2000, seed=3, hfac=0.9, velocity_seed=9, lattice=(80, 5, 5)), {}))
     return c
+
+
+WALL_SIDE = {C.ett_mti: C.ebc_y, C.ett_fld_gauss: C.ebc_all, C.ett_hydroshock: C.ebc_x}
+
+
+def check_walls(state, st, kernel):
+    """The store's fixed particles are the reference createFixedBorders' walls
+    of its real particles (positions and types bit for bit)."""
+    nr = int(state[C.ec_realpn - 1])
+    if st.shape[0] == nr:
+        return 0
+    side = WALL_SIDE[int(state[C.ec_ics - 1])]
+    s0 = state.copy()
+    s0[C.ec_fixedpn - 1] = 0
+    r = refrun.run(s0, st[:nr], kernel=kernel, mode=6, sts_s=side)
+    assert r["store"].shape[0] == st.shape[0], (r["store"].shape, st.shape)
+    assert np.array_equal(r["store"][:, 0:3], st[:, 0:3])
+    assert np.array_equal(r["store"][:, C.es_type - 1], st[:, C.es_type - 1])
+    return side
 
 
 def main():
@@ -92,6 +135,7 @@ def main():
         state, st = build()
         for k, v in over.items():
             state[k - 1] = v
+        walls = check_walls(state, st, kernel)
         r = refrun.run(state, st, kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, threads=1)
         nr = int(state[C.ec_realpn - 1])
         off = np.zeros(nr + 1, dtype=np.int64)
@@ -100,6 +144,7 @@ def main():
             off[1:] = np.cumsum([len(a) for a in neigh])
         nb = np.concatenate(neigh) if off[-1] else np.zeros((0, 2), np.int32)
         meta = dict(kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, dt_used=r["dt"],
+                    steps=max(1, sts_s) if mode in (1, 4) else 0, walls=walls,
                     reference_build="amdflang -fdefault-real-8 -fdefault-double-8 -O2 -fopenmp "
                                     "-ffp-contract=off, OMP_NUM_THREADS=1")
         np.savez_compressed(os.path.join(OUT, name + ".npz"), state=state, store_in=st,

@@ -87,3 +87,34 @@ def compare_neighbours(g, off, j, code):
             if not is_exact_tie(g["state"], g["store_in"], g["store_out"], g["meta"]["kernel"], i, jj, cc):
                 bad.append((i, (jj, cc)))
     return bad
+
+
+# per-field groups of the error report: density-like fields from loop 1 and
+# the EOS, and the derivatives loop 2 writes
+LOOP1_FIELDS = ["es_den", "es_h", "es_om", "es_p", "es_c"]
+LOOP2_FIELDS = ["es_ax", "es_ay", "es_az", "es_du", "es_dh", "es_ddt", "es_dtdx", "es_dtdy", "es_dtdz",
+                "es_dbx", "es_dby", "es_dbz", "es_vfld"]
+
+
+def rel_error_stats(ref, got, floor=1e-3):
+    """Max and RMS of |got - ref| / max(|ref|, floor * rms(ref)), the
+    per-particle relative error with a floor at a fraction of the field's RMS
+    (so particles where the field crosses zero do not dominate).  Returns
+    (max, rms); (0, 0) for a field that is zero in both."""
+    ref = np.asarray(ref, dtype=np.float64)
+    got = np.asarray(got, dtype=np.float64)
+    rms = float(np.sqrt(np.mean(ref * ref))) if ref.size else 0.0
+    if rms == 0.0:
+        d = np.abs(got - ref)
+        return float(d.max()) if d.size else 0.0, float(np.sqrt(np.mean(d * d))) if d.size else 0.0
+    d = np.abs(got - ref) / np.maximum(np.abs(ref), floor * rms)
+    return float(d.max()), float(np.sqrt(np.mean(d * d)))
+
+
+def field_report(ref, got, nr, fields=None, floor=1e-3):
+    """{field: (max, rms)} of rel_error_stats over the real particles."""
+    out = {}
+    for f in fields or (LOOP1_FIELDS + LOOP2_FIELDS):
+        k = getattr(C, f) - 1
+        out[f] = rel_error_stats(ref[:nr, k], got[:nr, k], floor)
+    return out

@@ -153,3 +153,91 @@ def test_sts_gradient_matches_dense_fp64_reference(gpu):
         ref = float(kp[k].grad)
         assert abs(g[k] - ref) <= 1e-9 * max(1.0, abs(ref)), (k, g[k], ref)
     assert np.abs(gT0.cpu().numpy() - T0r.grad.numpy()).max() <= 1e-11 * np.abs(T0r.grad.numpy()).max()
+
+
+def _dense_ops(AG, eng, n):
+    import torch
+    A = np.zeros((6, n, n))
+    for k in range(6):
+        ek = np.zeros(6)
+        ek[k] = 1.0
+        for c in range(n):
+            e = torch.zeros(n, dtype=torch.float64, device="cuda")
+            e[c] = 1.0
+            A[k][:, c] = AG.apply(eng, ek, e).cpu().numpy()
+    return torch.from_numpy(A)
+
+
+def _dense_sts(Am, T, s, dt, nsteps):
+    from sph_playground_amd.engine import sts_coeffs
+    for _ in range(nsteps):
+        Y0 = T
+        DY0 = Am @ Y0
+        Ym2, Ym1 = Y0, Y0 + sts_coeffs(1, s)[2] * dt * DY0
+        for j in range(2, s + 1):
+            mu, nu, mh, gh = sts_coeffs(j, s)
+            Yj = mu * Ym1 + nu * Ym2 + (1 - mu - nu) * Y0 + mh * dt * (Am @ Ym1) + gh * dt * DY0
+            Ym2, Ym1 = Ym1, Yj
+        T = Ym1
+    return T
+
+
+def test_two_sts_calls_in_one_graph_match_dense_reference(gpu):
+    # two forwards on one engine before one backward: sts(sts(T0, K1), K2)
+    # plus a second branch sts(T0, K1) summed into the loss.  The library
+    # keeps only the latest forward's trajectory, so the earlier calls'
+    # backwards must recompute theirs; the gradient must still equal the
+    # dense FP64 torch reference
+    import torch
+    state, st, eng, AG = _setup(8)
+    n = st.shape[0]
+    At = _dense_ops(AG, eng, n)
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy())
+    target = torch.from_numpy(np.random.default_rng(7).standard_normal(n) * 0.1)
+    p1 = [1.0, 1e-3, 1.1, 0.3]
+    p2 = [0.7, 2e-2, 0.4, 1.2]
+    s, nsteps, every, dt = 3, 3, 2, 2e-5
+
+    def run(dev, dense):
+        kp = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in p1 + p2]
+        T0v = T0.clone().to(dev).requires_grad_(True)
+        K1 = AG.conduction_tensor(*kp[:4])
+        K2 = AG.conduction_tensor(*kp[4:])
+        if dense:
+            A1 = torch.einsum("k,kij->ij", K1, At)
+            A2 = torch.einsum("k,kij->ij", K2, At)
+            Ta = _dense_sts(A2, _dense_sts(A1, T0v, s, dt, nsteps), s, dt, nsteps)
+            Tb = _dense_sts(A1, T0v, s, dt, nsteps)
+        else:
+            Ta = AG.sts(AG.sts(T0v, K1, eng, s, dt, nsteps, every), K2, eng, s, dt, nsteps, every)
+            Tb = AG.sts(T0v, K1, eng, s, dt, nsteps, every)
+        loss = 0.5 * torch.sum((Ta - target.to(dev)) ** 2) + 0.25 * torch.sum((Tb - target.to(dev)) ** 2)
+        loss.backward()
+        return float(loss), [float(x.grad) for x in kp], T0v.grad.cpu().numpy()
+
+    loss_ref, g_ref, gT_ref = run("cpu", True)
+    loss, g, gT = run("cuda", False)
+    assert abs(loss - loss_ref) <= 1e-12 * abs(loss_ref)
+    for k in range(8):
+        assert abs(g[k] - g_ref[k]) <= 1e-9 * max(1.0, abs(g_ref[k])), (k, g[k], g_ref[k])
+    assert np.abs(gT - gT_ref).max() <= 1e-11 * np.abs(gT_ref).max()
+
+
+def test_load_invalidates_prepared_operator(gpu):
+    # after a new load the prepared operator, its weights and the recorded
+    # forward belong to particles that are gone: the entry points refuse
+    # (before any kernel launch) and a pending backward raises
+    import torch
+    from sph_playground_amd._lib import SphError
+    state, st, eng, AG = _setup(6)
+    n = st.shape[0]
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda().requires_grad_(True)
+    K = torch.tensor(_K_of_b((1.0, 1.0, 0.0), 1.0, 1e-3), dtype=torch.float64, requires_grad=True)
+    T = AG.sts(T0, K, eng, 3, 2e-5, 2, 1)
+    big = P.diffusion3d(8, ddw="fab", b=(1.0, 1.0, 0.0), h_scale=1.03)[1]
+    eng.load(big)   # more particles: the old lists' storage is reallocated
+    with pytest.raises(SphError) as e:
+        AG.apply(eng, _K_of_b((1.0, 1.0, 0.0), 1.0, 1e-3), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    assert e.value.name == "BAD_ARG"
+    with pytest.raises(RuntimeError):
+        torch.sum(T).backward()

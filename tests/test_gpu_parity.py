@@ -36,23 +36,35 @@ def test_iterate_matches_reference_golden(gpu, name):
     assert not bad, bad[:5]
     errs = G.field_errors(g["store_out"], st, nr)
     assert max(errs.values()) <= FP64_TOL, errs
+    # fixed particles are neighbours only: iterate writes none of their rows
+    # (the reference loops over real particles, circuit1/circuit2, eos.f90:27-32)
+    assert np.array_equal(st[nr:], g["store_in"][nr:])
+    assert np.array_equal(g["store_out"][nr:], g["store_in"][nr:])
     eng.close()
 
 
 @pytest.mark.parametrize("name", G.names(mode=1))
 def test_leapfrog_matches_reference_golden(gpu, name):
+    # N passes of the leapfrog loop after one initial iterate (main.f90:219,
+    # 241-362): N calls of sph_leapfrog after a load
     g = G.load(name)
-    print("case", name, flush=True)
+    steps = g["meta"].get("steps", 1)
+    print("case", name, "steps", steps, flush=True)
     nr = int(g["state"][C.ec_realpn - 1])
     eng = _engine(g["state"], 1)
     eng.load(g["store_in"])
-    dt = eng.leapfrog(g["meta"]["dtcap"])
+    for _ in range(steps):
+        dt = eng.leapfrog(g["meta"]["dtcap"])
     # dt is a min/max over loop outputs (h, c, rho, kappa, grad T) that agree
     # with the reference to roundoff (neighbour sums run in another order)
     assert abs(dt - g["meta"]["dt_used"]) <= 1e-14 * abs(g["meta"]["dt_used"])
     out = eng.save()
     errs = G.field_errors(g["store_out"], out, nr)
     assert max(errs.values()) <= FP64_TOL, errs
+    # fixed particles have v = a = du = dh = ddt = dB = 0 and are never
+    # written by iterate: predictor and corrector leave them as they were
+    assert np.array_equal(out[nr:], g["store_in"][nr:])
+    assert np.array_equal(g["store_out"][nr:], g["store_in"][nr:])
     off, j, code = eng.neighbours()
     assert not G.compare_neighbours(g, off, j, code)
 

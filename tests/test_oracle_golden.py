@@ -36,8 +36,28 @@ def test_golden_set_is_complete():
     for n in ("hydro2d_m4_16", "hydro2d_gauss_16", "hydro1d_m6_64", "mhd3d_hydroshock_2000",
               "fld3d_fab_2000", "hyrad3d_2000", "diff2d_fab_sts3_16", "hydro2d_m6_32_leapfrog",
               "hydro2d_mti_16", "fld3d_fab_3000_leapfrog", "radexch3d_3000",
-              "hyrad3d_3000_leapfrog_radexch"):
+              "hyrad3d_3000_leapfrog_radexch", "hydro2d_m6_32_leapfrog3", "hydro2d_mti_walls_16",
+              "hydro2d_mti_walls_16_leapfrog3", "fld2d_gauss_walls_20", "mhd3d_hydroshock_walls_2000",
+              "fld3d_hydroshock_walls_2000", "mhd3d_hydroshock_walls_2000_leapfrog2"):
         assert n in names
+
+
+@pytest.mark.parametrize("name", [n for n in G.names() if G.load(n)["meta"].get("walls")])
+def test_fixed_walls_match_reference_create_fixed_borders(name):
+    """problems.fixed_walls restates createFixedBorders (BC.f90:477-640): on the
+    golden's real particles it must give the reference-built walls (positions
+    and types of every fixed row) bit for bit, and ec_fixedpn."""
+    from sph_playground_amd import problems as P
+    g = G.load(name)
+    nr = int(g["state"][C.ec_realpn - 1])
+    s0 = g["state"].copy()
+    s0[C.ec_fixedpn - 1] = 0
+    s1, st = P.fixed_walls(s0, g["store_in"][:nr], g["meta"]["walls"])
+    assert st.shape[0] == g["store_in"].shape[0] > nr
+    assert np.array_equal(st[:, 0:3], g["store_in"][:, 0:3])
+    assert np.array_equal(st[:, C.es_type - 1], g["store_in"][:, C.es_type - 1])
+    assert s1[C.ec_fixedpn - 1] == g["state"][C.ec_fixedpn - 1]
+    assert (g["store_in"][nr:, C.es_type - 1] == C.ept_fixed).all()
 
 
 def test_tie_rule_detects_real_mismatch():
