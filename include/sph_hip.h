@@ -103,6 +103,21 @@ int sph_step(sph_ctx *ctx, const double state[49], double *maxconsenrg);
 int sph_neighbours(sph_ctx *ctx, int64_t *off, int32_t *j, int8_t *code, int64_t cap_pairs,
                    int64_t *n_pairs);
 
+/* No counterpart in the reference: sph_neighbours for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * The same lists, written to DEVICE memory the caller owns: d_off (n_real + 1
+ * entries), d_j and d_code (cap_pairs entries; may be NULL with cap_pairs 0
+ * for the offsets alone).  The call allocates nothing: its scratch (list
+ * lengths and the prefix sum's storage) is a fixed device array of the
+ * library, one per GPU, held until the process ends, which bounds n_real to
+ * at most 4,194,304 (larger inputs return SPH_ERR_BAD_ARG).  Calls on one GPU
+ * share that array: calls that overlap in time on one GPU (other host
+ * threads, other contexts) race and must be serialised by the caller.  When
+ * the lists hold more than cap_pairs entries, the first cap_pairs are written,
+ * *n_pairs gets the total and SPH_ERR_BAD_ARG is returned.  Synchronous. */
+int sph_neighbours_device_noalloc(sph_ctx *ctx, int64_t *d_off, int32_t *d_j, int8_t *d_code, int64_t cap_pairs,
+                                  int64_t *n_pairs);
+
 /* One pass of the leapfrog loop around iterate (main.f90:216-362, no STS):
  * dt = min(timestep.calc dt, dtcap) (timestep.f90:40-166), predictor,
  * iterate, corrector, all on the resident state.  The initial iterate of

@@ -14,7 +14,7 @@ EXPORTS = (
     "sph_load", "sph_save", "sph_step", "sph_neighbours", "sph_leapfrog", "sph_sts", "sph_sts_coeffs",
     "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
     "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange", "sph_batch_create", "sph_batch_destroy",
-    "sph_batch_sts", "sph_batch_get_info", "sph_batch_offsets",
+    "sph_batch_sts", "sph_batch_get_info", "sph_batch_offsets", "sph_neighbours_device_noalloc",
 )
 
 SPH_OK = 0
@@ -92,6 +92,8 @@ def lib():
     L.sph_diff_apply_adjoint.argtypes = [P, D, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, D, D]
     L.sph_diff_forward.argtypes = [P, D, ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
     L.sph_diff_backward.argtypes = [P, ct.c_void_p, ct.c_void_p, D]
+    L.sph_neighbours_device_noalloc.argtypes = [P, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64,
+                                                ct.POINTER(ct.c_int64)]
     L.sph_error_string.argtypes = [ct.c_int]
     L.sph_error_string.restype = ct.c_char_p
     for name in EXPORTS:

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/sph_hip.h"
@@ -49,7 +50,8 @@ __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
 __global__ void k_step_close(const double *, int, const int *, const long long *, long long *);
 __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
 __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
-                               const int64_t *, int32_t *, int8_t *);
+                               const int64_t *, int32_t *, int8_t *, int64_t);
+__global__ void k_set_i64(int64_t *, int64_t);
 __global__ void k_counts_by_orig(const int *, const int64_t *, const uint8_t *, int, int64_t *);
 __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
 __global__ void k_predict(Fields, int, double, int, double *, int);
@@ -1228,7 +1230,7 @@ int sph_neighbours(sph_ctx *c, int64_t *off, int32_t *j, int8_t *code, int64_t c
       else {
         HIPCHK(hipMemcpyAsync(d_cnt, off, sizeof(int64_t) * nr, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_neigh_export, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->nb, c->cnt,
-                           c->orig[c->cur], c->real, n, (int)c->npad, d_cnt, dj, dc);
+                           c->orig[c->cur], c->real, n, (int)c->npad, d_cnt, dj, dc, total);
         HIPCHK(hipMemcpyAsync(j, dj, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(code, dc, total, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1239,6 +1241,83 @@ int sph_neighbours(sph_ctx *c, int64_t *off, int32_t *j, int8_t *code, int64_t c
   }
   (void)hipFree(d_cnt);
   return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- allocation-free list export
+// Scratch of sph_neighbours_device_noalloc, in device memory the library
+// never allocates: the list lengths in original order (int64, at most
+// kNoAllocMaxReal + 1 of them) and the prefix sum's temporary storage.  Each
+// GPU the library runs on holds its own copy (about 33 MiB) until the process
+// ends, and every call on that GPU shares it.
+constexpr int64_t kNoAllocMaxReal = (int64_t)1 << 22;
+constexpr size_t kNoAllocCountBytes = ((size_t)(kNoAllocMaxReal + 1) * sizeof(int64_t) + 255) / 256 * 256;
+constexpr size_t kNoAllocTempBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocCountBytes + kNoAllocTempBytes];
+constexpr int kExportLL = 1600;  // h_ll slot of the pair-count read-back
+
+// The device address of this device's `scratch`, looked up once per device
+// (the device must be current).  Host code uses only this address: the
+// array's own name is a host-side stand-in.
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[64] = {};
+static int scratch_on_device(int device, unsigned char **out) {
+  if (device < 0 || device >= 64) return SPH_ERR_BAD_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "sph_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+              hipGetErrorString(e));
+      return SPH_ERR_HIP;
+    }
+    g_scratch[device] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch[device];
+  return SPH_OK;
+}
+
+extern "C" {
+
+int sph_neighbours_device_noalloc(sph_ctx *c, int64_t *d_off, int32_t *d_j, int8_t *d_code, int64_t cap_pairs,
+                                  int64_t *n_pairs) {
+  if (!c || !d_off || !c->have_step || cap_pairs < 0 || (cap_pairs > 0 && (!d_j || !d_code)))
+    return SPH_ERR_BAD_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t nr = c->n_real;
+  if (nr > kNoAllocMaxReal) {
+    fprintf(stderr, "sph_neighbours_device_noalloc: %lld real particles, at most %lld\n", (long long)nr,
+            (long long)kNoAllocMaxReal);
+    return SPH_ERR_BAD_ARG;
+  }
+  unsigned char *base = nullptr;
+  RCHK(scratch_on_device(c->device, &base));
+  int64_t *counts = reinterpret_cast<int64_t *>(base);
+  void *temp = base + kNoAllocCountBytes;
+  size_t temp_bytes = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, counts, d_off, (int)(nr + 1), c->stream));
+  if (temp_bytes > kNoAllocTempBytes) {
+    fprintf(stderr, "sph_neighbours_device_noalloc: the prefix sum needs %zu bytes of scratch, %zu held\n",
+            temp_bytes, kNoAllocTempBytes);
+    return SPH_ERR_BAD_ARG;
+  }
+  const int n = (int)c->n;
+  // every real particle writes its own slot; the slot after them (the
+  // total) is the only one the last call's contents could leave stale
+  hipLaunchKernelGGL(k_counts_by_orig, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->cnt, c->orig[c->cur],
+                     c->real, n, counts);
+  hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(64), 0, c->stream, counts + nr, (int64_t)0);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, d_off, (int)(nr + 1), c->stream));
+  if (cap_pairs > 0)
+    hipLaunchKernelGGL(k_neigh_export, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->nb, c->cnt,
+                       c->orig[c->cur], c->real, n, (int)c->npad, d_off, d_j, d_code, cap_pairs);
+  HIPCHK(hipMemcpyAsync(c->h_ll + kExportLL, d_off + nr, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t total = (int64_t)c->h_ll[kExportLL];
+  if (n_pairs) *n_pairs = total;
+  return (cap_pairs > 0 && total > cap_pairs) ? SPH_ERR_BAD_ARG : SPH_OK;
 }
 
 // timestep.calc, timestep.f90:40-166 (hydro, MHD, diffusion and FLD limits)

@@ -1382,17 +1382,26 @@ __global__ void k_soa_to_aos(Fields F, const int64_t *orig, int n, double *aos, 
   for (int f = 0; f < F_TOTAL; f++) row[f] = F.f[f][s];
 }
 
-// Neighbour lists back in original numbering (for sph_neighbours).
+// Neighbour lists back in original numbering (for sph_neighbours); entries
+// at or past `cap` (the output's length) are not written.
 __global__ void k_neigh_export(const uint32_t *nb, const int *cnt, const int64_t *orig, const uint8_t *real,
-                               int n, int npad, const int64_t *row_off_by_orig, int32_t *jout, int8_t *cout) {
+                               int n, int npad, const int64_t *row_off_by_orig, int32_t *jout, int8_t *cout,
+                               int64_t cap) {
   int s = gtid();
   if (s >= n || !real[s]) return;
   int64_t o = row_off_by_orig[orig[s]];
-  for (int k = 0; k < cnt[s]; k++) {
+  const int m = (int)max<int64_t>(0, min<int64_t>(cnt[s], cap - o));
+  for (int k = 0; k < m; k++) {
     uint32_t e = nb[nb_slot(k, s, npad)];
     jout[o + k] = (int32_t)orig[nb_index(e)];
     cout[o + k] = (int8_t)nb_code(e);
   }
+}
+
+// One int64 set on the device (a kernel, not a memset, so a captured graph
+// replays it).
+__global__ void k_set_i64(int64_t *p, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __global__ void k_counts_by_orig(const int *cnt, const int64_t *orig, const uint8_t *real, int n, int64_t *out) {
