@@ -26,20 +26,72 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic work per pair / per particle (fixed figures used for the roofline)
-FLOP_PER_PAIR_LOOP2_FAB = 84      # loop 2, fab, per-particle kappa
+# algorithmic work per pair / per particle (add/sub/mul/div/sqrt = 1 flop,
+# fma = 2; per-particle constants hoisted)
+FLOP_PER_PAIR_LOOP2_FAB = 72      # loop 2, fab, one tensor for every pair (84 with a per-particle one)
 FLOP_PER_PAIR_LOOP1_GRAD = 50     # loop 1 sweep with grad T
+# compulsory HBM bytes per particle of one step of this workload: search
+# (read x, y, z, h; write key and index: 40), loop 1 sweep with grad T (read
+# x, y, z, m, h, T, rho, Omega; write rho, Omega, h, eps: 92), loop 2 fab with
+# one tensor (read x, y, z, m, rho, h, T; write dT/dt: 64)
+COMPULSORY_BYTES_PER_PARTICLE = {"search": 40, "loop1_per_sweep": 92, "loop2": 64}
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector and dense matrix)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the 128^3 workload's kernels, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the counters read
-# for a plain 1 GiB copy (x2 and x1): profiles/hbm_traffic_diffusion3d_128cube.md
-MEASURED_TRAFFIC_128 = {"neigh": 1.204e9, "loop1_mixed": 1.525e9, "loop2_mixed": 1.632e9}
+GPU_CLOCK_HZ = 2.4e9              # MI355X peak engine clock
+N_SIMD = 1024                     # 256 CUs x 4 SIMDs
+N_TA = 256                        # one texture address unit per CU
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = "hbm_traffic_diffusion3d_128cube.md"
+COUNTERS_FILE = "counters_diffusion3d_128cube.md"
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def profile_table(name):
+    """Rows of the first markdown table in profiles/<name>: {kernel: [cells]},
+    the kernel name with its template arguments as rocprofv3 prints it."""
+    rows = {}
+    try:
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                if line.startswith("| `"):
+                    cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                    rows[cells[0].strip("`")] = cells[1:]
+    except OSError:
+        pass
+    return rows
+
+
+def kernel_roofs(kname, ms, alg_bytes, alg_flops, peak_tflops):
+    """Both roofs of one kernel at its measured launch time: algorithmic and
+    measured HBM bytes (FETCH_SIZE/WRITE_SIZE per launch, copy-calibrated, from
+    the traffic profile) against 8 TB/s; algorithmic flops against the VALU
+    peak; VALU and address-unit busy from the counter profile (a wave64 VALU
+    instruction occupies its SIMD 4 cycles)."""
+    t = ms * 1e-3
+    out = {"ms_per_launch": ms, "algorithmic_bytes": alg_bytes,
+           "algorithmic_hbm_frac": alg_bytes / t / (PEAK_HBM_GBS * 1e9)}
+    tr = profile_table(TRAFFIC_FILE).get(kname)
+    if tr:
+        out["measured_bytes"] = float(tr[-1])
+        out["measured_hbm_frac"] = out["measured_bytes"] / t / (PEAK_HBM_GBS * 1e9)
+    if alg_flops:
+        out["algorithmic_flops"] = alg_flops
+        out["algorithmic_tflops"] = alg_flops / t / 1e12
+        out["valu_peak_tflops"] = peak_tflops
+        out["algorithmic_valu_frac"] = alg_flops / t / 1e12 / peak_tflops
+    ct = profile_table(COUNTERS_FILE).get(kname)
+    if ct:
+        cyc = t * GPU_CLOCK_HZ
+        out["valu_busy"] = float(ct[1]) * 4.0 / (N_SIMD * cyc)
+        out["ta_busy"] = float(ct[6]) / (N_TA * cyc)
+    busy = {"hbm": out.get("measured_hbm_frac", out["algorithmic_hbm_frac"]), "valu": out.get("valu_busy", 0.0),
+            "address_unit": out.get("ta_busy", 0.0)}
+    out["measured_bound"] = max(busy, key=busy.get)
+    return out
 
 
 def run_path(state, st, precision, steps, warmup, torch):
@@ -120,22 +172,23 @@ def batched_cost(nprob=256, s=10, nsteps=5, reps=3):
     from sph_playground_amd._lib import check, lib
     from sph_playground_amd.batch import Batch
     from sph_playground_amd.engine import Engine
+    from sph_playground_amd import batch as B
     specs = P.batch2d_specs(nprob)
-    engines, Ks, T0s, dts = [], [], [], []
+    built = [P.batch2d_problem(sp) for sp in specs]
+    triples = [(state, st, sp["ratio"]) for (state, st, _), sp in zip(built, specs)]
+    Ks = [K for _, _, K in built]
+    T0s = [st[:, C.es_t - 1].copy() for _, st, _ in built]
+    dts = [0.05 * (1.0 / sp["n_side"]) ** 2 * (s * s + s - 2) / 4.0 for sp in specs]
     t0 = time.perf_counter()
-    for k, sp in enumerate(specs):
-        state, st, K = P.batch2d_problem(sp)
-        eng = Engine(state, kernel="m6", tensor="full", kappa_perp=sp["ratio"])
-        eng.load(st)
-        AG.prepare(eng)
-        engines.append(eng)
-        Ks.append(K)
-        T0s.append(st[:, C.es_t - 1].copy())
-        dx = 1.0 / sp["n_side"]
-        dts.append(0.05 * dx * dx * (s * s + s - 2) / 4.0)
-        if (k + 1) % 64 == 0:
-            log("batch: %d problems prepared" % (k + 1))
-    t_prep = time.perf_counter() - t0
+    cells, cell_table = B.tune_cells(triples)
+    t_tune = time.perf_counter() - t0
+    log("batch: search cells per size bucket %s (tuned in %.2f s)" % (cells, t_tune))
+    engines, t_prep = B.prepare_problems(triples, cells)
+    log("batch: %d problems prepared in %.2f s" % (nprob, t_prep))
+    # the same preparation one problem after another on one host thread
+    eng1, t_prep_serial = B.prepare_problems(triples, cells, threads=1)
+    for e in eng1:
+        e.close()
     batch = Batch(engines, np.stack(Ks))
     info = batch.info()
     T0 = torch.from_numpy(np.concatenate(T0s)).cuda()
@@ -203,12 +256,122 @@ def batched_cost(nprob=256, s=10, nsteps=5, reps=3):
             "stage_ms": stage_ms, "stage_algorithmic_bytes": byts,
             "stage_hbm_gbs": byts / (stage_ms * 1e-3) / 1e9,
             "autotune_block": info["block"], "autotune_ms_per_4_stages": info["tune"],
-            "prepare_s": t_prep,
+            "prepare_s": t_prep, "prepare_s_one_thread": t_prep_serial,
+            "cell_autotune": {"cells_r_over": {str(4 ** b * 1024): d for b, d in cells.items()},
+                              "step_ms": {str(4 ** b * 1024): {str(d): ms for d, ms in t.items()}
+                                          for b, t in cell_table.items()},
+                              "tune_s": t_tune},
             "fp32_weights": {"ms_graph": t32 * 1e3, "pair_interactions_per_s": inter / t32,
                              "stage_ms": t32 * 1e3 / (s * nsteps),
                              "stage_algorithmic_bytes": 8 * P_ + 80 * N_,
                              "stage_hbm_gbs": (8 * P_ + 80 * N_) / (t32 / (s * nsteps)) / 1e9,
                              "max_dev_rel_to_change_vs_fp64": dev32, "autotune_block": info32["block"]}}
+
+
+def rel_error_stats(ref, got, floor=1e-3):
+    """Max and RMS over particles of |got - ref| / max(|ref|, floor rms(ref))."""
+    import numpy as np
+    rms = float(np.sqrt(np.mean(ref * ref)))
+    if rms == 0.0:
+        d = np.abs(got - ref)
+        return {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d)))}
+    d = np.abs(got - ref) / np.maximum(np.abs(ref), floor * rms)
+    return {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d)))}
+
+
+def field_errors(ref, got, nr, names):
+    from sph_playground_amd import consts as C
+    out = {}
+    for f in names:
+        k = getattr(C, "es_" + f) - 1
+        if float(abs(ref[:nr, k]).max()) > 0.0:
+            out[f] = rel_error_stats(ref[:nr, k], got[:nr, k])
+    return out
+
+
+def moving_cost(n=1048576, steps=20, warmup=3):
+    """Steps in which particles move: leapfrog passes (main.f90:241-362:
+    time step, predictor, iterate, corrector) of the 3D shock tube, 1,048,576
+    uniform-random particles in [-1,1] x [0,0.125]^2, periodic in y and z,
+    fixed x walls, B = (0.75, 1, 0), opacity 1; its MHD half and its FLD half,
+    both precisions.  Every pass wraps, re-keys and checks the cell order of
+    the moved particles (and re-sorts them when the order broke) before the
+    search.  Pairs per pass are the iterate's P."""
+    import numpy as np
+    import torch
+    from sph_playground_amd import problems as P
+    from sph_playground_amd.engine import Engine
+    out = {"workload": "3D shock tube, %d uniform-random particles in [-1,1]x[0,0.125]^2, periodic y/z, "
+                       "fixed x walls, B=(0.75,1,0), opacity 1, LP limiter; %d leapfrog passes after %d"
+                       % (n, steps, warmup)}
+    for half in ("mhd", "fld"):
+        state, st = P.shock_tube3d(n, half=half, velocity_seed=2)
+        res = {}
+        saved = {}
+        for prec in ("fp64", "mixed"):
+            eng = Engine(state, kernel="m6", precision=prec, neigh_cap=400)
+            eng.load(st)
+            for _ in range(warmup):
+                eng.leapfrog(1.0)
+            pairs, sorted_, sweeps = [], 0, []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                eng.leapfrog(1.0)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            s = eng.stats()
+            saved[prec] = eng.save()
+            res[prec] = {"ms_per_step": t * 1e3 / steps, "pairs_per_step": s["n_pairs"],
+                         "pair_interactions_per_s": s["n_pairs"] * steps / t,
+                         "newton_sweeps_last_step": s["sweeps"]}
+            eng.close()
+        nr = int(state[24])
+        res["mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+        res["mixed_field_errors_vs_fp64_after_%d_steps" % (warmup + steps)] = field_errors(
+            saved["fp64"], saved["mixed"], nr, ("den", "h", "om", "ax", "ay", "az", "du", "dh", "dbx", "dby",
+                                                 "dbz", "ddt", "dtdx", "vfld"))
+        out[half] = res
+    return out
+
+
+def sts_cost(size=128, s=5, nsteps=10):
+    """The anisotropic-diffusion workload's 10 RKL2 super-steps
+    (sts_integrator.f90:87-124, every stage a full iterate) at its size, both
+    precisions, against the closed-form decay of its two eigenmodes."""
+    import numpy as np
+    import torch
+    from sph_playground_amd import consts as C
+    from sph_playground_amd import problems as P
+    from sph_playground_amd.engine import Engine
+    state, st = P.diffusion3d(size, ddw="fab", b=(1.0, 1.0, 0.0), disotropic="no")
+    h = float(st[0, C.es_h - 1])
+    dt = (s * s + s - 2) / 4.0 * 0.2 * h * h
+    T0 = st[:, C.es_t - 1]
+    exact = P.anisotropic_decay_exact(st[:, 0:3], nsteps * dt)
+    change = float(np.abs(exact - T0).max())
+    res = {"workload": "%d^3 anisotropic diffusion, %d RKL2 super-steps x %d stages, dt = %.3e "
+                       "((s^2+s-2)/4 x 0.2 h^2 / kappa_par)" % (size, nsteps, s, dt),
+           "t_final": nsteps * dt, "change_of_T": change}
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        eng = Engine(state, kernel="m6", precision=prec, tensor="full", kappa_perp=1e-3)
+        eng.load(st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            eng.sts(s, dt)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        T = eng.save()[:, C.es_t - 1]
+        eng.close()
+        outs[prec] = T
+        d = T - exact
+        res[prec] = {"ms_per_super_step": t * 1e3 / nsteps,
+                     "err_vs_closed_form_max_over_change": float(np.abs(d).max()) / change,
+                     "err_vs_closed_form_rms_over_change": float(np.sqrt(np.mean(d * d))) / change}
+    res["mixed_vs_fp64_max_over_change"] = float(np.abs(outs["mixed"] - outs["fp64"]).max()) / change
+    return res
 
 
 def cpu_baseline(size):
@@ -244,6 +407,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-moving", action="store_true")
+    ap.add_argument("--no-sts", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the step runs on one GPU; runs on more than one GPU are not supported "
@@ -279,37 +444,70 @@ def main():
     kmx = {k: avg(permx, k) for k in ("ms_k_neigh", "ms_k_loop1", "ms_k_loop2", "ms_total")}
     sweeps = avg(permx, "sweeps")
 
-    def traffic(key):  # measured HBM bytes per launch (only for the default 128^3 workload)
-        return MEASURED_TRAFFIC_128[key] if args.size == 128 else None
-
-    # dominant kernel of the headline (mixed) path
+    # both roofs of the three main kernels of each path; the profiles hold
+    # the counters of the 128^3 workload, so smaller sizes report no
+    # measured bytes or busy fractions
+    sweeps64 = avg(per64, "sweeps")
+    p1 = avg(permx, "loop1_pairs") / max(sweeps, 1)
+    p164 = avg(per64, "loop1_pairs") / max(sweeps64, 1)
+    names = {"mixed": ("k_neigh<3>", "k_loop1<3, 1, 1, float>", "k_loop2_fab<3, 1, float, HIP_vector_type<float, 4u> >"),
+             "fp64": ("k_neigh<3>", "k_loop1<3, 1, 1, double>",
+                      "k_loop2_fab<3, 1, double, HIP_vector_type<double, 4u> >")}
+    roofs = {}
+    for path, km, sw, pp, peak in (("mixed", kmx, sweeps, p1, PEAK_FP32_TFLOPS),
+                                   ("fp64", k64, sweeps64, p164, PEAK_FP64_TFLOPS)):
+        nk, n1, n2 = names[path] if args.size == 128 else ("", "", "")
+        roofs[path] = {
+            "k_neigh": kernel_roofs(nk, km["ms_k_neigh"], n * 44 + pairs * 4, 0, peak),
+            "k_loop1": kernel_roofs(n1, km["ms_k_loop1"] / max(sw, 1),
+                                    n * COMPULSORY_BYTES_PER_PARTICLE["loop1_per_sweep"],
+                                    pp * FLOP_PER_PAIR_LOOP1_GRAD, peak),
+            "k_loop2_fab": kernel_roofs(n2, km["ms_k_loop2"], n * COMPULSORY_BYTES_PER_PARTICLE["loop2"],
+                                        pairs * FLOP_PER_PAIR_LOOP2_FAB, peak)}
+    # traffic of a whole steady mixed step: every kernel of it in the profile
+    # (search prep, search, Newton sweep, post, loop 2, close) against the
+    # compulsory bytes
+    step_kernels = ("k_prep", "k_neigh<3>", "k_loop1<3, 1, 1, float>", "k_post1<HIP_vector_type<float, 4u> >",
+                    "k_loop2_fab<3, 1, float, HIP_vector_type<float, 4u> >", "k_step_close")
+    tr = profile_table(TRAFFIC_FILE)
+    step_measured = sum(float(tr[k][-1]) for k in step_kernels if k in tr) if args.size == 128 else None
+    cb = COMPULSORY_BYTES_PER_PARTICLE
+    step_compulsory = n * (cb["search"] + cb["loop1_per_sweep"] * sweeps + cb["loop2"])
+    # the contract's block: the dominant kernel of the headline (mixed) path
     dom = max(("ms_k_neigh", "ms_k_loop1", "ms_k_loop2"), key=lambda k: kmx[k])
-    if dom == "ms_k_loop2":
-        flops = pairs * FLOP_PER_PAIR_LOOP2_FAB
-        ach = flops / (kmx[dom] * 1e-3) / 1e12
-        roof = dict(kernel="k_loop2_fab<3,M6,float> (loop 2, fab, full tensor)", bound="valu", achieved=ach,
-                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=traffic("loop2_mixed"),
-                    flop_per_pair=FLOP_PER_PAIR_LOOP2_FAB, pairs_per_launch=pairs, ms_per_launch=kmx[dom])
-    elif dom == "ms_k_loop1":
-        p1 = avg(permx, "loop1_pairs") / max(sweeps, 1)
-        flops = p1 * FLOP_PER_PAIR_LOOP1_GRAD
-        ms = kmx[dom] / max(sweeps, 1)
-        ach = flops / (ms * 1e-3) / 1e12
-        roof = dict(kernel="k_loop1<3,M6,grad,float> (one Newton sweep)", bound="valu", achieved=ach,
-                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS, traffic=traffic("loop1_mixed"),
-                    flop_per_pair=FLOP_PER_PAIR_LOOP1_GRAD, pairs_per_launch=p1, ms_per_launch=ms)
+    if dom == "ms_k_neigh":
+        r = roofs["mixed"]["k_neigh"]
+        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm",
+                    achieved=r["algorithmic_bytes"] / (r["ms_per_launch"] * 1e-3) / 1e9, peak=PEAK_HBM_GBS,
+                    unit="GB/s", frac=r["algorithmic_hbm_frac"], traffic=r.get("measured_bytes"),
+                    bytes_per_launch=r["algorithmic_bytes"], ms_per_launch=r["ms_per_launch"],
+                    measured_bound=r["measured_bound"], valu_busy=r.get("valu_busy"), ta_busy=r.get("ta_busy"))
     else:
-        # neighbour build: algorithmic bytes = read x,y,z,m,h (40 B) + write the list (4 B/pair) + count
-        byts = n * 44 + pairs * 4
-        ach = byts / (kmx[dom] * 1e-3) / 1e9
-        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm", achieved=ach,
-                    peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS, traffic=traffic("neigh"),
-                    bytes_per_launch=byts, ms_per_launch=kmx[dom])
+        key = "k_loop1" if dom == "ms_k_loop1" else "k_loop2_fab"
+        r = roofs["mixed"][key]
+        roof = dict(kernel=key + " (FP32 pair loop)", bound="valu", achieved=r["algorithmic_tflops"],
+                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=r["algorithmic_valu_frac"],
+                    traffic=r.get("measured_bytes"), ms_per_launch=r["ms_per_launch"],
+                    measured_bound=r["measured_bound"], valu_busy=r.get("valu_busy"), ta_busy=r.get("ta_busy"))
+    roof["kernels"] = roofs
+    roof["step_measured_bytes"] = step_measured
+    roof["step_compulsory_bytes"] = step_compulsory
+    roof["profiles"] = [TRAFFIC_FILE, COUNTERS_FILE]
 
     adj = None
     if not args.no_adjoint:
         log("adjoint cost")
         adj = adjoint_cost()
+
+    sts = None
+    if not args.no_sts:
+        log("STS super-steps at full size")
+        sts = sts_cost(args.size)
+
+    mov = None
+    if not args.no_moving:
+        log("moving particles (shock tube)")
+        mov = moving_cost()
 
     bat = None
     if not args.no_batch:
@@ -335,17 +533,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic (f64 positions, separations, Newton h, sums)",
+        "dtype": "f32 pair arithmetic and per-particle sums (f64 positions, separations, Newton update, "
+                 "energy sum)",
         "data": "synthetic (seeded lattice, T = sin sin sin)",
         "config": {"workload": "3D anisotropic diffusion density+force step, %d^3 = %d particles, M6, fab, "
                                "kappa_par=1 kappa_perp=1e-3 b=(1,1,0)/sqrt2, periodic unit cube" % (args.size, n),
                    "pairs_per_step": pairs, "newton_sweeps_per_step": sweeps},
         "fp64": {"value": v64, "ms_per_step": t64 * 1e3 / args.steps, "dtype": "f64", "kernels_ms": k64},
         "mixed": {"value": vmx, "kernels_ms": kmx, "density_max_rel_err_vs_fp64": den_err,
-                  "ddt_max_err_rel_to_max": ddt_err},
+                  "ddt_max_err_rel_to_max": ddt_err,
+                  "field_errors_vs_fp64": field_errors(out64, outmx, n, ("den", "h", "om", "ddt", "dtdx", "dtdy",
+                                                                         "dtdz"))},
         "mixed_vs_fp64": vmx / v64,
         "adjoint": adj,
         "batched": bat,
+        "moving": mov,
+        "sts": sts,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -52,6 +52,8 @@ typedef struct {
   double newton_tol;  /* relative h tolerance; 0 -> 1e-8 (circuit1.f90:104) FP64, 1e-6 mixed */
   int neigh_cap;      /* neighbour cap incl. self; 0 -> reference maxneibnum(dim) */
   int device;         /* HIP device ordinal */
+  int cell_div;       /* search cells are r_max / cell_div wide (1..4); 0 -> 3.  The
+                         neighbour sets do not depend on it, only the search's speed */
 } sph_options;
 
 typedef struct {
@@ -118,10 +120,18 @@ int sph_neighbours(sph_ctx *ctx, int64_t *off, int32_t *j, int8_t *code, int64_t
 int sph_neighbours_device_noalloc(sph_ctx *ctx, int64_t *d_off, int32_t *d_j, int8_t *d_code, int64_t cap_pairs,
                                   int64_t *n_pairs);
 
+/* checkVarsReady (preruncheck.f90:13-60, called by main.f90:217 before the
+ * initial iterate) on the resident store: SPH_ERR_PRECONDITION when mu0 <= 0
+ * with MHD (*what = 32), the isotropic conductivity is <= 0 (64), or any
+ * loaded particle of non-empty type has density (1), sound speed (2), h (4),
+ * pressure (8) or internal energy (16) <= 0; the first failing test in the
+ * reference's order.  sph_leapfrog runs it before its initial iterate. */
+int sph_prerun_check(sph_ctx *ctx, const double state[49], int *what);
+
 /* One pass of the leapfrog loop around iterate (main.f90:216-362, no STS):
  * dt = min(timestep.calc dt, dtcap) (timestep.f90:40-166), predictor,
- * iterate, corrector, all on the resident state.  The initial iterate of
- * main.f90:219 runs first only when the resident store does not hold the
+ * iterate, corrector, all on the resident state.  The pre-run checks and
+ * the initial iterate of main.f90:216-219 run first only when the resident store does not hold the
  * derivatives of its current state (after sph_load, or after a failed step);
  * so N calls after a load are the reference's initial iterate plus N passes. */
 int sph_leapfrog(sph_ctx *ctx, const double state[49], double dtcap, double *dt_used);

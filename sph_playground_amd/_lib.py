@@ -15,6 +15,7 @@ EXPORTS = (
     "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
     "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange", "sph_batch_create", "sph_batch_destroy",
     "sph_batch_sts", "sph_batch_get_info", "sph_batch_offsets", "sph_neighbours_device_noalloc",
+    "sph_prerun_check",
 )
 
 SPH_OK = 0
@@ -27,7 +28,7 @@ ERRORS = {
 class Options(ct.Structure):
     _fields_ = [("precision", ct.c_int), ("tensor_mode", ct.c_int), ("kappa_perp", ct.c_double),
                 ("max_sweeps", ct.c_int), ("newton_tol", ct.c_double), ("neigh_cap", ct.c_int),
-                ("device", ct.c_int)]
+                ("device", ct.c_int), ("cell_div", ct.c_int)]
 
 
 class Stats(ct.Structure):
@@ -94,6 +95,7 @@ def lib():
     L.sph_diff_backward.argtypes = [P, ct.c_void_p, ct.c_void_p, D]
     L.sph_neighbours_device_noalloc.argtypes = [P, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64,
                                                 ct.POINTER(ct.c_int64)]
+    L.sph_prerun_check.argtypes = [P, D, ct.POINTER(ct.c_int)]
     L.sph_error_string.argtypes = [ct.c_int]
     L.sph_error_string.restype = ct.c_char_p
     for name in EXPORTS:

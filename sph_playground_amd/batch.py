@@ -9,6 +9,9 @@ prepared operators are merged into one: every RKL2 stage
 super-step is one replayed hipGraph (``sph_batch_*`` in libsph_hip.so).
 """
 import ctypes as ct
+import math
+import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -18,6 +21,72 @@ _D = ct.POINTER(ct.c_double)
 
 
 _PREC = {"fp64": 0, "mixed": 1}
+
+# problem-size buckets of the cell-size autotuner: N in [1024 4^k, 1024 4^(k+1))
+BUCKET_BASE = 1024
+CELL_DIVS = (1, 2, 3, 4)
+
+
+def bucket_of(n):
+    """Size bucket of an N-particle problem: floor(log4(N / 1024)), >= 0."""
+    return max(0, int(math.floor(math.log(max(n, 1) / BUCKET_BASE, 4) + 1e-12)))
+
+
+def tune_cells(problems, reps=3, cell_divs=CELL_DIVS, device=0):
+    """The autotuner of the search cells: for every size bucket, the bucket's
+    largest problem is stepped with search cells r_max / d wide for each d
+    in ``cell_divs`` (device time of the whole step: search, Newton sweeps,
+    loop 2, since the cell size also orders the particles the loops gather)
+    and the fastest d is kept.  ``problems``: (state, store, kappa_perp)
+    triples.  Returns ({bucket: d}, {bucket: {d: ms}})."""
+    from .engine import Engine
+    rep = {}
+    for k, (state, st, kperp) in enumerate(problems):
+        b = bucket_of(st.shape[0])
+        if b not in rep or st.shape[0] > problems[rep[b]][1].shape[0]:
+            rep[b] = k
+    best, table = {}, {}
+    for b, k in sorted(rep.items()):
+        state, st, kperp = problems[k]
+        table[b] = {}
+        for d in cell_divs:
+            eng = Engine(state, kernel="m6", tensor="full", kappa_perp=kperp, cell_div=d, device=device)
+            eng.load(st)
+            eng.step()
+            ms = []
+            for _ in range(reps):
+                eng.step()
+                ms.append(eng.stats()["ms_total"])
+            eng.close()
+            table[b][d] = float(np.median(ms))
+        best[b] = min(table[b], key=table[b].get)
+    return best, table
+
+
+def prepare_problems(problems, cells=None, threads=8, device=0):
+    """Engines for ``problems`` ((state, store, kappa_perp) triples), each
+    loaded and prepared (sph_diff_prepare: search, Newton h, EOS, reverse
+    lists), with search cells from ``cells`` ({bucket: d}, see tune_cells;
+    missing buckets use the default).  Contexts are independent, so
+    ``threads`` host threads prepare them concurrently, each on its own
+    stream: the small problems' launches and host round trips overlap instead
+    of queueing one problem after another.  Returns (engines, seconds)."""
+    from . import autograd as AG
+    from .engine import Engine
+    cells = cells or {}
+
+    def one(pr):
+        state, st, kperp = pr
+        eng = Engine(state, kernel="m6", tensor="full", kappa_perp=kperp,
+                     cell_div=cells.get(bucket_of(st.shape[0]), 0), device=device)
+        eng.load(st)
+        AG.prepare(eng)
+        return eng
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        engines = list(ex.map(one, problems))
+    return engines, time.perf_counter() - t0
 
 
 class Batch:

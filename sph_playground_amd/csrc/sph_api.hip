@@ -52,6 +52,7 @@ __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
 __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
                                const int64_t *, int32_t *, int8_t *, int64_t);
 __global__ void k_set_i64(int64_t *, int64_t);
+__global__ void k_prerun_check(Fields, int, int *);
 __global__ void k_counts_by_orig(const int *, const int64_t *, const uint8_t *, int, int64_t *);
 __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
 __global__ void k_predict(Fields, int, double, int, double *, int);
@@ -525,7 +526,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   const int axis_cap = dim == 3 ? 1024 : (dim == 2 ? 32768 : (1 << 30));
   Grid g{};
   double cs[3];
-  int RS = kNeighRS;
+  int RS = c->opt.cell_div > 0 ? c->opt.cell_div : kNeighRS;
   for (int attempt = 0; attempt < 2; attempt++) {
     g = Grid{};
     g.mdim = dim;
@@ -1044,6 +1045,7 @@ int sph_default_options(sph_options *o) {
   o->newton_tol = 0.0;
   o->neigh_cap = 0;
   o->device = 0;
+  o->cell_div = 0;
   return SPH_OK;
 }
 
@@ -1056,7 +1058,8 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   sph_ctx *c = new sph_ctx();
   if (opt) c->opt = *opt;
   else sph_default_options(&c->opt);
-  if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1) {
+  if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1 ||
+      c->opt.cell_div < 0 || c->opt.cell_div > 4) {
     delete c;
     return SPH_ERR_BAD_ARG;
   }
@@ -1348,13 +1351,48 @@ static int calc_dt(sph_ctx *c, const Phys &P, double *dt) {
   return SPH_OK;
 }
 
+// checkVarsReady (preruncheck.f90:13-60) on the resident store; *what gets
+// the failed test: 32 mu0 <= 0 with MHD, 64 isotropic conductivity <= 0,
+// else the per-particle bits of k_prerun_check.
+static int prerun_check(sph_ctx *c, const double *state, int *what) {
+  *what = 0;
+  const int eqs = (int)state[S_EQS];
+  if (state[S_MUZERO] <= 0.0 && (eqs == 302 || eqs == 303)) *what = 32;
+  else if ((int)state[S_DISO] > 0 && state[S_DCOND] <= 0.0) *what = 64;
+  if (*what) return SPH_ERR_PRECONDITION;
+  if (c->n <= 0) return SPH_ERR_BAD_ARG;
+  HIPCHK(hipMemsetAsync(c->d_int + 5, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_prerun_check, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), (int)c->n,
+                     c->d_int + 5);
+  RCHK(read_ints(c, 5, 1));
+  const int b = c->h_int[5];
+  // the reference stops at the first failing test, in this order
+  for (int bit : {1, 2, 4, 8, 16})
+    if (b & bit) { *what = bit; return SPH_ERR_PRECONDITION; }
+  return SPH_OK;
+}
+
+int sph_prerun_check(sph_ctx *c, const double state[49], int *what) {
+  if (!c || !state) return SPH_ERR_BAD_ARG;
+  (void)hipSetDevice(c->device);
+  int w = 0;
+  const int rc = prerun_check(c, state, &w);
+  if (what) *what = w;
+  return rc;
+}
+
 int sph_leapfrog(sph_ctx *c, const double state[49], double dtcap, double *dt_used) {
   if (!c || !state || c->n <= 0) return SPH_ERR_BAD_ARG;
   (void)hipSetDevice(c->device);
   Phys P = call_phys(c, state);
-  // main.f90:219: the initial iterate, once; later passes take the
-  // derivatives the previous pass's iterate left (main.f90:323-329)
-  if (!c->derivs_ready) RCHK(run_step(c, state));
+  // main.f90:216-219: the pre-run checks and the initial iterate, once;
+  // later passes take the derivatives the previous pass's iterate left
+  // (main.f90:323-329)
+  if (!c->derivs_ready) {
+    int what = 0;
+    RCHK(prerun_check(c, state, &what));
+    RCHK(run_step(c, state));
+  }
   double dt;
   RCHK(calc_dt(c, P, &dt));
   dt = std::min(dt, dtcap);

@@ -1398,6 +1398,21 @@ __global__ void k_neigh_export(const uint32_t *nb, const int *cnt, const int64_t
   }
 }
 
+// checkVarsReady's per-particle tests (preruncheck.f90:37-58) on every loaded
+// particle whose type is not empty: bits 1 density, 2 sound speed, 4 h,
+// 8 pressure, 16 internal energy <= 0 (a NaN passes, as in the reference).
+__global__ void k_prerun_check(Fields F, int n, int *bits) {
+  int s = gtid();
+  if (s >= n || (int)F.f[F_TYPE][s] == 0) return;
+  int b = 0;
+  if (F.f[F_DEN][s] <= 0.0) b |= 1;
+  if (F.f[F_C][s] <= 0.0) b |= 2;
+  if (F.f[F_H][s] <= 0.0) b |= 4;
+  if (F.f[F_P][s] <= 0.0) b |= 8;
+  if (F.f[F_U][s] <= 0.0) b |= 16;
+  if (b) atomicOr(bits, b);
+}
+
 // One int64 set on the device (a kernel, not a memset, so a captured graph
 // replays it).
 __global__ void k_set_i64(int64_t *p, int64_t v) {

@@ -31,18 +31,19 @@ class Engine:
     ``precision`` ('fp64' throughout | 'mixed'), ``tensor``
     ('reference' diffusion tensor | 'full' kappa_par bb + kappa_perp (I-bb)),
     ``kappa_perp``, ``max_sweeps`` (100), ``newton_tol`` (1e-8 fp64, 1e-6
-    mixed), ``neigh_cap`` (reference maxneibnum).
+    mixed), ``neigh_cap`` (reference maxneibnum), ``cell_div`` (search cells
+    r_max / cell_div wide, 0 = 3; the lists do not depend on it).
     """
 
     def __init__(self, state, kernel="m6", precision="fp64", tensor="reference", kappa_perp=0.0,
-                 max_sweeps=100, newton_tol=0.0, neigh_cap=0, device=0):
+                 max_sweeps=100, newton_tol=0.0, neigh_cap=0, device=0, cell_div=0):
         L = lib()
         self._state = np.ascontiguousarray(np.asarray(state, dtype=np.float64))
         if self._state.shape != (C.ec_total,):
             raise ValueError("state must have %d entries" % C.ec_total)
         kid = C.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
         opt = Options(_PREC[precision], _TENSOR[tensor], float(kappa_perp), int(max_sweeps),
-                      float(newton_tol), int(neigh_cap), int(device))
+                      float(newton_tol), int(neigh_cap), int(device), int(cell_div))
         h = ct.c_void_p()
         check(L.sph_ctx_create(_dptr(self._state), kid, ct.byref(opt), ct.byref(h)), "sph_ctx_create")
         self._h = h
@@ -106,6 +107,18 @@ class Engine:
         self._geometry_gen += 1
         check(lib().sph_leapfrog(self._h, _dptr(s), float(dtcap), ct.byref(dt)), "sph_leapfrog")
         return dt.value
+
+    PRERUN_MESSAGES = {1: "density <= 0", 2: "csound <= 0", 4: "smoothing length <= 0", 8: "pressure <= 0",
+                       16: "internal energy <= 0", 32: "mhd_magneticconstant <= 0",
+                       64: "diff_conductivity <= 0"}
+
+    def prerun_check(self, state=None):
+        """checkVarsReady (preruncheck.f90:13-60) on the resident store: raises
+        SphError (PRECONDITION) naming the reference's message."""
+        s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        what = ct.c_int(0)
+        rc = lib().sph_prerun_check(self._h, _dptr(s), ct.byref(what))
+        check(rc, "sph_prerun_check: %s" % self.PRERUN_MESSAGES.get(what.value, "") if rc else "")
 
     def sts(self, s_stages, dt, state=None):
         """One RKL2 super-step of T (sts_integrator.f90:87-124) on the resident state."""

@@ -15,6 +15,7 @@ module sph_hip_capi
     real(c_double) :: newton_tol = 0.0_c_double
     integer(c_int) :: neigh_cap = 0
     integer(c_int) :: device = 0
+    integer(c_int) :: cell_div = 0
   end type
 
   interface

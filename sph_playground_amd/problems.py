@@ -338,57 +338,60 @@ def fixed_walls(state, st, side=C.ebc_all, return_source=False):
     emin = np.array([state[C.ec_xmin - 1], state[C.ec_ymin - 1], state[C.ec_zmin - 1]])
     emax = np.array([state[C.ec_xmax - 1], state[C.ec_ymax - 1], state[C.ec_zmax - 1]])
     bmin, bmax = emin + bs, emax - bs
-    pos = [np.array(p) for p in st[:, 0:3]]
+    n0 = st.shape[0]
     types = st[:, C.es_type - 1].astype(int)
-    lists = {k: [] for k in ("x1", "x2", "y1", "y2", "z1", "z2")}
-    for i, p in enumerate(pos):
-        if types[i] not in (C.ept_real, C.ept_fixed, C.ept_fixedreal):
-            continue
-        for a, ax in enumerate("xyz"[:dim]):
-            if p[a] < bmin[a]:
-                if p[a] > emin[a]:
-                    lists[ax + "1"].append(i)
-            elif p[a] > bmax[a]:
-                if p[a] < emax[a]:
-                    lists[ax + "2"].append(i)
-    new = []
-    origin = list(range(len(pos)))
+    live = np.isin(types, (C.ept_real, C.ept_fixed, C.ept_fixedreal))
+    pos = st[:, 0:3].copy()
+    # findInsideBorderParticles: per axis and end, the sources in index order
+    lists = {}
+    for a in range(3):
+        lo = live & (pos[:, a] < bmin[a]) & (pos[:, a] > emin[a])
+        hi = live & ~(pos[:, a] < bmin[a]) & (pos[:, a] > bmax[a]) & (pos[:, a] < emax[a])
+        lists[(a, 1)] = list(np.nonzero(lo)[0]) if a < dim else []
+        lists[(a, 2)] = list(np.nonzero(hi)[0]) if a < dim else []
+    blocks, origin_blocks = [pos], [np.arange(n0)]
+    origin = np.arange(n0)
+    total = n0
 
-    def mirror(src, a, upper):
-        q = pos[src].copy()
-        q[a] = emax[a] + (emax[a] - q[a]) if upper else emin[a] + (emin[a] - q[a])
-        pos.append(q)
-        new.append(q)
-        origin.append(origin[src])
-        k = len(pos) - 1
-        for b in range(a + 1, dim):  # later axes see the new particle in their bands
-            if q[b] < bmin[b]:
-                lists["xyz"[b] + "1"].append(k)
-            elif q[b] > bmax[b]:
-                lists["xyz"[b] + "2"].append(k)
-
-    def walls(ax, a, ends):
-        for e in ends:
-            for src in list(lists[ax + e]):
-                mirror(src, a, e == "2")
+    def walls(a, end):
+        nonlocal total, origin
+        src = np.array(lists[(a, end)], dtype=np.int64)
+        if src.size == 0:
+            return
+        allpos = np.concatenate(blocks)
+        q = allpos[src].copy()
+        q[:, a] = emax[a] + (emax[a] - q[:, a]) if end == 2 else emin[a] + (emin[a] - q[:, a])
+        idx = np.arange(total, total + src.size)
+        blocks.append(q)
+        org = np.concatenate(origin_blocks)[src]
+        origin_blocks.append(org)
+        total += src.size
+        # a new particle in a later axis' band joins that axis' list (corner walls)
+        for b in range(a + 1, dim):
+            lo = q[:, b] < bmin[b]
+            hi = ~lo & (q[:, b] > bmax[b])
+            lists[(b, 1)].extend(idx[lo].tolist())
+            lists[(b, 2)].extend(idx[hi].tolist())
 
     if side in (C.ebc_all, C.ebc_x):
-        walls("x", 0, "12")
+        walls(0, 1)
+        walls(0, 2)
     if dim > 1:
         if side in (C.ebc_all, C.ebc_y, C.ebc_y1):
-            walls("y", 1, "1")
+            walls(1, 1)
         if side in (C.ebc_all, C.ebc_y, C.ebc_y2):
-            walls("y", 1, "2")
+            walls(1, 2)
         if dim == 3 and side in (C.ebc_all, C.ebc_z):
-            walls("z", 2, "12")
-    extra = np.zeros((len(new), st.shape[1]))
-    if new:
-        extra[:, 0:3] = np.array(new)
+            walls(2, 1)
+            walls(2, 2)
+    new = np.concatenate(blocks[1:]) if len(blocks) > 1 else np.zeros((0, 3))
+    extra = np.zeros((new.shape[0], st.shape[1]))
+    extra[:, 0:3] = new
     extra[:, C.es_type - 1] = C.ept_fixed
     out = np.concatenate([st, extra])
     state[C.ec_fixedpn - 1] = (out.shape[0] - int(state[C.ec_realpn - 1]))
     if return_source:
-        return state, out, np.array(origin[st.shape[0]:], dtype=np.int64)
+        return state, out, np.concatenate(origin_blocks[1:]) if len(blocks) > 1 else np.zeros(0, np.int64)
     return state, out
 
 
@@ -459,3 +462,57 @@ def shock_slab_walls(n, seed=3, **kw):
             continue
         full[n:, f - 1] = st[src, f - 1]
     return state, full
+
+
+def shock_tube3d(n=1048576, half="mhd", seed=3, hfac=1.0, velocity_seed=None):
+    """The 3D MHD + flux-limited-radiation shock tube: n uniform-random
+    particles (``seed``) in [-1,1] x [0,0.125]^2, periodic in y and z, left
+    rho = 1, P = 1, E_rad = 1, right rho = 0.125, P = 0.1, E_rad = 0.1,
+    B = (0.75, 1, 0), opacity 1, Levermore-Pomraning limiter, run as the
+    reference's hydroshock set-up (fixed walls beyond both x ends,
+    IC/IC.f90:227-253, iterator.f90:102-112).  No reference equation set has
+    MHD and FLD together (state.f90:117-149), so the workload runs as two
+    halves: ``half='mhd'`` (magnetohydro: hydro + AV + MHD) and ``half='fld'``
+    (manual: hydro + AV + diffusion with the FLD conductivity)."""
+    if half == "mhd":
+        return shock_slab_walls(n, seed=seed, hfac=hfac, equations="magnetohydro", velocity_seed=velocity_seed)
+    if half == "fld":
+        return shock_slab_walls(n, seed=seed, hfac=hfac, fld=True, velocity_seed=velocity_seed)
+    raise ValueError("half must be 'mhd' or 'fld'")
+
+
+def energy_budget(state, store):
+    """Energies of the real particles (code units): kinetic sum m |v|^2 / 2,
+    thermal sum m u, magnetic sum m |B|^2 / (2 rho mu0) when the equation set
+    has MHD, radiation sum m xi (xi = E_rad / rho, the T row) when it has FLD;
+    ``total`` is their sum.  The quantities whose rates loop 2's energy sum
+    adds up (circuit2.f90:490-492)."""
+    nr = int(state[C.ec_realpn - 1])
+    s = store[:nr]
+    m = s[:, C.es_m - 1]
+    v = s[:, C.es_vx - 1:C.es_vz]
+    out = {"kinetic": float(np.sum(0.5 * m * np.sum(v * v, axis=1))),
+           "thermal": float(np.sum(m * s[:, C.es_u - 1]))}
+    eq = int(state[C.ec_eqs - 1])
+    if eq in (C.eeq_magnetohydro, C.eeq_magnetohydrodiffusion):
+        b = s[:, C.es_bx - 1:C.es_bz]
+        out["magnetic"] = float(np.sum(0.5 * m * np.sum(b * b, axis=1) / (s[:, C.es_den - 1]
+                                                                          * state[C.ec_muzero - 1])))
+    if eq == C.eeq_manual and int(state[C.ec_eqonfluxlim - 1]) == 1:
+        out["radiation"] = float(np.sum(m * s[:, C.es_t - 1]))
+    out["total"] = float(sum(out.values()))
+    return out
+
+
+def anisotropic_decay_exact(pos, t, kpar=1.0, kperp=1e-3):
+    """Exact solution of dT/dt = div(K grad T), K = kpar b b + kperp (I - b b),
+    b = (1, 1, 0)/sqrt(2), from T0 = sin(2 pi x) sin(2 pi y) sin(2 pi z) in
+    the periodic unit cube.  T0 = (cos 2pi(x-y) - cos 2pi(x+y)) sin(2 pi z) / 2,
+    and each term is an eigenmode: k = 2pi(1, -1, 1) has k.b = 0, rate
+    3 kperp (2 pi)^2; k = 2pi(1, 1, 1) has (k.b)^2 = 2 (2 pi)^2, rate
+    (2 kpar + kperp) (2 pi)^2."""
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    w2 = (2.0 * math.pi) ** 2
+    return 0.5 * (np.cos(2 * math.pi * (x - y)) * math.exp(-3.0 * kperp * w2 * t)
+                  - np.cos(2 * math.pi * (x + y)) * math.exp(-(2.0 * kpar + kperp) * w2 * t)) \
+        * np.sin(2 * math.pi * z)

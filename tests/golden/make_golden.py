@@ -103,6 +103,16 @@ def cases():
               lambda: P.shock_slab_walls(2000, seed=5, hfac=1.0, velocity_seed=9, fld=True), {}))
     c.append(("mhd3d_hydroshock_walls_2000_leapfrog2", 1, 1, 2, 1e-3,
               lambda: P.shock_slab_walls(2000, seed=3, hfac=0.9, velocity_seed=9, lattice=(80, 5, 5)), {}))
+    # many leapfrog passes: 50 of the 2D mti set-up with its walls; 10 of the
+    # MHD shock tube with its x walls on a jittered lattice (2000 particles,
+    # dt capped at 5e-4; in longer runs of this small set the flow passes the
+    # reference's neighbour cap of 150 near the walls).  Their lists are not
+    # stored, only the states
+    c.append(("hydro2d_mti_walls_16_leapfrog50", 1, 101, 50, 1e-3,
+              lambda: P.hydro2d_mti_walls(16, velocity_seed=9), {}))
+    c.append(("mhd3d_shocktube_2000_leapfrog10", 1, 101, 10, 5e-4,
+              lambda: P.shock_slab_walls(2000, seed=3, hfac=0.9, lattice=(80, 5, 5), equations="magnetohydro",
+                                         velocity_seed=9), {}))
     return c
 
 
@@ -136,6 +146,7 @@ def main():
         for k, v in over.items():
             state[k - 1] = v
         walls = check_walls(state, st, kernel)
+        lists = mode < 100
         r = refrun.run(state, st, kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, threads=1)
         nr = int(state[C.ec_realpn - 1])
         off = np.zeros(nr + 1, dtype=np.int64)
@@ -143,8 +154,8 @@ def main():
         if neigh:
             off[1:] = np.cumsum([len(a) for a in neigh])
         nb = np.concatenate(neigh) if off[-1] else np.zeros((0, 2), np.int32)
-        meta = dict(kernel=kernel, mode=mode, sts_s=sts_s, dtcap=dtcap, dt_used=r["dt"],
-                    steps=max(1, sts_s) if mode in (1, 4) else 0, walls=walls,
+        meta = dict(kernel=kernel, mode=mode % 100, sts_s=sts_s, dtcap=dtcap, dt_used=r["dt"],
+                    steps=max(1, sts_s) if mode % 100 in (1, 4) else 0, walls=walls, lists=lists,
                     reference_build="amdflang -fdefault-real-8 -fdefault-double-8 -O2 -fopenmp "
                                     "-ffp-contract=off, OMP_NUM_THREADS=1")
         np.savez_compressed(os.path.join(OUT, name + ".npz"), state=state, store_in=st,

@@ -118,3 +118,56 @@ def field_report(ref, got, nr, fields=None, floor=1e-3):
         k = getattr(C, f) - 1
         out[f] = rel_error_stats(ref[:nr, k], got[:nr, k], floor)
     return out
+
+
+def rotation(a=1.4, b=2.0, c=2.1):
+    """A generic rotation matrix (z-y-z Euler angles): every entry non-zero."""
+    def rz(t):
+        return np.array([[np.cos(t), -np.sin(t), 0.0], [np.sin(t), np.cos(t), 0.0], [0.0, 0.0, 1.0]])
+
+    def ry(t):
+        return np.array([[np.cos(t), 0.0, np.sin(t)], [0.0, 1.0, 0.0], [-np.sin(t), 0.0, np.cos(t)]])
+    return rz(a) @ ry(b) @ rz(c)
+
+
+def diffusion_ball(R=None, n_side=12, ddw="fab", b=(1.0, 0.3, 0.2), seed=4):
+    """A 3D diffusion set with no periodic images (borderless): the jittered
+    lattice points of a ball of radius 0.45 about the origin, T a smooth
+    non-polynomial field attached to each particle, one field direction b.
+    With ``R`` the positions and b are rotated together (T stays with its
+    particle), so every scalar the loops write must not change and grad T must
+    rotate with R."""
+    from sph_playground_amd import problems as P
+    pos, dx = P.lattice_positions(3, n_side, (-0.5, 0.5, -0.5, 0.5, -0.5, 0.5))
+    pos = pos + 0.05 * dx * P.gaussian(seed, 3 * pos.shape[0]).reshape(-1, 3)
+    pos = pos[np.sum(pos * pos, axis=1) < 0.45 ** 2]
+    n = pos.shape[0]
+    T = np.sin(2.0 * pos[:, 0] + 3.0 * pos[:, 1]) * np.cos(2.5 * pos[:, 2]) + pos[:, 0] ** 2
+    bv = np.asarray(b, dtype=np.float64) / np.linalg.norm(b)
+    if R is not None:
+        pos = pos @ R.T
+        bv = R @ bv
+    st = P.empty_store(n)
+    st[:, 0:3] = pos
+    st[:, C.es_m - 1] = 1.0 / n
+    st[:, C.es_den - 1] = 1.0
+    st[:, C.es_h - 1] = 1.2 * dx
+    st[:, C.es_t - 1] = T
+    st[:, C.es_kappa - 1] = 1.0
+    st[:, C.es_bx - 1:C.es_bz] = bv
+    state = P.make_state(dim=3, equations="diffusion", ddw=ddw, artts="no", disotropic="no", hfac=1.2,
+                         box=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0), spacing=dx, n_real=n, process="borderless")
+    return state, st
+
+
+def rotation_errors(out0, out1, R, n):
+    """(scalar errors, grad T error) between an unrotated and a rotated run:
+    normwise max |d| / max |ref| of density, h, Omega, dT/dt, and of grad T
+    against R grad T."""
+    sc = {}
+    for f in ("es_den", "es_h", "es_om", "es_ddt"):
+        k = getattr(C, f) - 1
+        sc[f] = np.abs(out1[:n, k] - out0[:n, k]).max() / np.abs(out0[:n, k]).max()
+    g0 = out0[:n, C.es_dtdx - 1:C.es_dtdz] @ R.T
+    g1 = out1[:n, C.es_dtdx - 1:C.es_dtdz]
+    return sc, np.abs(g1 - g0).max() / np.abs(g0).max()

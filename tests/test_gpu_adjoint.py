@@ -241,3 +241,40 @@ def test_load_invalidates_prepared_operator(gpu):
     assert e.value.name == "BAD_ARG"
     with pytest.raises(RuntimeError):
         torch.sum(T).backward()
+
+
+def test_frozen_geometry_forward_vs_reference_sts_golden(gpu):
+    # sph_diff_forward freezes h, rho and Omega after one iterate; the
+    # reference's sts_integrate re-runs iterate at every stage
+    # (sts_integrator.f90:100-108), whose Newton iteration moves h again by
+    # up to its tolerance (1e-8 relative) before the stage's dT/dt.  On the
+    # reference's own STS step (diff2d_fab_sts3_16: 3 stages, reference
+    # tensor difcond diag(b^2)) the frozen operator's T must stay within
+    # 1e-7 of the change of T (measured below); eng.sts, which re-iterates
+    # like the reference, matches it to 1e-12 in test_sts_matches_reference_golden
+    import torch
+    from tests import goldens as G
+    from sph_playground_amd import autograd as AG
+    from sph_playground_amd.engine import Engine
+    g = G.load("diff2d_fab_sts3_16")
+    state, st = g["state"], g["store_in"]
+    eng = Engine(state, kernel="m6")
+    eng.load(st)
+    AG.prepare(eng)
+    b = st[0, C.es_bx - 1:C.es_bz] / np.linalg.norm(st[0, C.es_bx - 1:C.es_bz])
+    dc = state[C.ec_dcondconst - 1]
+    K = np.array([dc * b[0] ** 2, dc * b[1] ** 2, dc * b[2] ** 2, 0.0, 0.0, 0.0])
+    T0 = torch.from_numpy(st[:, C.es_t - 1].copy()).cuda()
+    out = torch.empty_like(T0)
+    import ctypes as ct
+    from sph_playground_amd._lib import check, lib
+    torch.cuda.synchronize()
+    check(lib().sph_diff_forward(eng._h, K.ctypes.data_as(ct.POINTER(ct.c_double)), int(g["meta"]["sts_s"]),
+                                 float(g["meta"]["dtcap"]), 1, 1, ct.c_void_p(T0.data_ptr()),
+                                 ct.c_void_p(out.data_ptr())), "sph_diff_forward")
+    T = out.cpu().numpy()
+    ref = g["store_out"][:, C.es_t - 1]
+    change = np.abs(ref - st[:, C.es_t - 1]).max()
+    dev = np.abs(T - ref).max() / change
+    print("frozen-geometry forward vs reference STS: %.2e of the change of T" % dev, flush=True)
+    assert change > 0 and dev <= 1e-7

@@ -228,3 +228,52 @@ def test_batch2d_specs_follow_the_study_distribution():
     Km = r * np.eye(3) + (1 - r) * np.outer(b, b)
     assert np.allclose(K, [Km[0, 0], Km[1, 1], Km[2, 2], Km[0, 1], Km[0, 2], Km[1, 2]], rtol=0, atol=1e-15)
     assert st.shape[0] == a[0]["n_side"] ** 2
+
+
+@pytest.mark.parametrize("ddw", ["fab", "n2w", "fw", "2nw-+", "2nw+-"])
+def test_full_tensor_rotation_invariance(ddw):
+    # kappa_par b b + kappa_perp (I - b b) with every entry of K non-zero
+    # (xz and yz included): rotating the positions and b together leaves
+    # every scalar the loops write unchanged and rotates grad T.  A wrong sign
+    # or a transposed index in the tensor contraction or in a Hessian breaks
+    # it; so does an operator that reads K's diagonal alone
+    from tests import goldens as G
+    R = G.rotation()
+    s0, st0 = G.diffusion_ball(ddw=ddw)
+    s1, st1 = G.diffusion_ball(R, ddw=ddw)
+    # the ball's surface particles grow h past the reference's list cap (150)
+    o0 = oracle.iterate(s0, st0, tensor_mode=1, kappa_perp=0.1, neigh_cap=400)
+    o1 = oracle.iterate(s1, st1, tensor_mode=1, kappa_perp=0.1, neigh_cap=400)
+    assert np.array_equal(np.diff(o0["off"]), np.diff(o1["off"]))
+    sc, gerr = G.rotation_errors(o0["store"], o1["store"], R, st0.shape[0])
+    assert max(sc.values()) <= 1e-12 and gerr <= 1e-12, (sc, gerr)
+    # the rotated tensor really has off-diagonal entries in every plane
+    K = 0.1 * np.eye(3) + 0.9 * np.outer(st1[0, C.es_bx - 1:C.es_bz], st1[0, C.es_bx - 1:C.es_bz])
+    assert min(abs(K[0, 1]), abs(K[0, 2]), abs(K[1, 2])) > 0.05
+    # a planted mistake: the oracle run on the rotated set with the unrotated
+    # b (a tensor that did not rotate) is caught
+    st_bad = st1.copy()
+    st_bad[:, C.es_bx - 1:C.es_bz] = st0[:, C.es_bx - 1:C.es_bz]
+    ob = oracle.iterate(s1, st_bad, tensor_mode=1, kappa_perp=0.1, neigh_cap=400)
+    assert G.rotation_errors(o0["store"], ob["store"], R, st0.shape[0])[0]["es_ddt"] > 1e-3
+
+
+def test_anisotropic_decay_closed_form():
+    # the closed form is the initial field at t = 0 and solves
+    # dT/dt = K : grad grad T (checked by finite differences in t and space)
+    from sph_playground_amd import problems as P
+    rng = np.random.default_rng(2)
+    pos = rng.random((50, 3))
+    T0 = np.sin(2 * np.pi * pos[:, 0]) * np.sin(2 * np.pi * pos[:, 1]) * np.sin(2 * np.pi * pos[:, 2])
+    assert np.allclose(P.anisotropic_decay_exact(pos, 0.0), T0, atol=1e-14)
+    kpar, kperp, t, e = 1.0, 1e-3, 3e-3, 1e-4
+    b = np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    K = kperp * np.eye(3) + (kpar - kperp) * np.outer(b, b)
+    dTdt = (P.anisotropic_decay_exact(pos, t + 1e-7) - P.anisotropic_decay_exact(pos, t - 1e-7)) / 2e-7
+    lap = np.zeros(pos.shape[0])
+    for a in range(3):
+        for c in range(3):
+            da, dc = np.eye(3)[a] * e, np.eye(3)[c] * e
+            f = lambda p: P.anisotropic_decay_exact(p, t)  # noqa: E731
+            lap += K[a, c] * (f(pos + da + dc) - f(pos + da - dc) - f(pos - da + dc) + f(pos - da - dc)) / (4 * e * e)
+    assert np.abs(dTdt - lap).max() <= 1e-4 * np.abs(dTdt).max()
