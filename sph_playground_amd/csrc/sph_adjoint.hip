@@ -507,6 +507,310 @@ __global__ void k_lincomb(int n, double *y, double c0, const double *x0, double 
   y[s] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Operators that read grad T: the 2nw- + / 2nw+ - second derivatives
+// (circuit2.f90:398-420) and flux-limited diffusion (circuit2.f90:177-190,
+// 356-397), with the geometry (h, rho, Omega of the prepared step) frozen.
+//
+//   U = G(Y): loop 1's grad T (circuit1.f90:172-181) of a field Y,
+//     mode 1  U_a = 1/(rho_a Om_a) sum_b m_b (Y_b rho_b - Y_a rho_a) nw(r_ab, h_a)
+//     mode 2  U_a = sum_b m_b (Y_a/(Om_a rho_a) nw(r_ab, h_a) + Y_b rho_a/(Om_b rho_b^2) nw(r_ab, h_b))
+//   2nw-+ : f_a = sum_b m_b ((K U_a).nw_a /(Om_a rho_a^2) + (K U_b).nw_b /(Om_b rho_b^2))   (mode 1)
+//   2nw+- : f_a = 1/(Om_a rho_a^2) sum_b m_b (K U_b - K U_a).nw_a                          (mode 2)
+//   FLD   : f_a = 1/rho_a sum_b (m_b/rho_b) tr H_ab(h_a) (D_a + D_b)/2 (Y_b rho_b - Y_a rho_a),
+//           D_a = c lambda(R_a)/(kappa_a rho_a), R_a = |U_a|/(kappa_a rho_a Y_a rho_a),
+//           lambda(R) = (2 + R)/(6 + 3R + R^2)                                            (mode 1)
+// with nw(r, h) = wcv df(q) r / (h^(dim+2) q) (kernel.f90:189-197).  The 2nw
+// operators are linear in Y and in K; FLD is not linear in Y, and its
+// adjoint is the transposed Jacobian at a state.  Every adjoint sum is a
+// gather over the reverse lists: no atomics, a fixed order.
+
+// nw(r, h) / r-vector: wcv df(q) / (h^(dim+2) q)
+template <int DIM, int KT>
+__device__ __forceinline__ double nw_coef(const DiffOpArgs &a, double dr, double inv_h) {
+  const double q = dr * inv_h;
+  return a.wcv * KernelFn<KT, double>::df(q) * ipow<DIM + 2>(inv_h) / q;
+}
+
+__device__ __forceinline__ double3 kmul(const double K[6], double x, double y, double z) {
+  return make_double3(K[0] * x + K[3] * y + K[4] * z, K[3] * x + K[1] * y + K[5] * z,
+                      K[4] * x + K[5] * y + K[2] * z);
+}
+
+// Pair loop over s's list: fn(j, rx, ry, rz, dr) with r = x_s - x_j(image).
+template <typename Fn>
+__device__ __forceinline__ void over_list(const DiffOpArgs &a, int s, const double4 &ps, Fn fn) {
+  const int nn = a.cnt[s];
+  for (int k = 0; k < nn; k++) {
+    const uint32_t e = a.nb[nb_slot(k, s, a.npad)];
+    const uint32_t j = nb_index(e);
+    const double4 pb = a.pm[j];
+    double xb = pb.x, yb = pb.y, zb = pb.z;
+    if (nb_code(e) != kNoImage) image_shift(xb, yb, zb, nb_code(e), a.b);
+    const double rx = ps.x - xb, ry = ps.y - yb, rz = ps.z - zb;
+    fn(j, rx, ry, rz, sqrt(rx * rx + ry * ry + rz * rz));
+  }
+}
+
+// Pair loop over s's reverse list: fn(i, rx, ry, rz, dr) with i the owner
+// (s in i's list) and r = x_i - x_s(image as i sees it).
+template <typename Fn>
+__device__ __forceinline__ void over_rlist(const DiffOpArgs &a, int s, const double4 &ps, Fn fn) {
+  const int nr = a.rcnt[s];
+  for (int k = 0; k < nr; k++) {
+    const uint32_t e = a.rent[nb_slot(k, s, a.npad)];
+    const uint32_t i = nb_index(e);
+    const double4 pi = a.pm[i];
+    double xb = ps.x, yb = ps.y, zb = ps.z;
+    if (nb_code(e) != kNoImage) image_shift(xb, yb, zb, nb_code(e), a.b);
+    const double rx = pi.x - xb, ry = pi.y - yb, rz = pi.z - zb;
+    fn(i, rx, ry, rz, sqrt(rx * rx + ry * ry + rz * rz));
+  }
+}
+
+// U = G(Y), and with FSUM the geometry sum F_s = sum_b m_b nw(r_sb, h_s).
+template <int DIM, int KT, int MODE>
+__global__ __launch_bounds__(256) void k_nl_grad(DiffOpArgs a, const double *Y, double4 *U, double4 *F) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const double4 qa = a.q0[s];
+  const double inv_ha = qa.x, rhoa = qa.y, oma = qa.z;
+  const double ya = Y ? Y[s] : 0.0;
+  double ux = 0, uy = 0, uz = 0, fx = 0, fy = 0, fz = 0;
+  over_list(a, s, ps, [&](uint32_t j, double rx, double ry, double rz, double dr) {
+    const double4 qb = a.q0[j];
+    const double mb = a.pm[j].w;
+    const double wa = nw_coef<DIM, KT>(a, dr, inv_ha);
+    if (F) { fx += mb * wa * rx; fy += mb * wa * ry; fz += mb * wa * rz; }
+    if (!Y) return;
+    if constexpr (MODE == 1) {
+      const double t = mb * (Y[j] * qb.y - ya * rhoa) * wa;
+      ux += t * rx; uy += t * ry; uz += t * rz;
+    } else {
+      const double wb = nw_coef<DIM, KT>(a, dr, qb.x);
+      const double ca = mb * ya / (oma * rhoa) * wa, cb = mb * Y[j] * rhoa / (qb.z * qb.y * qb.y) * wb;
+      ux += (ca + cb) * rx; uy += (ca + cb) * ry; uz += (ca + cb) * rz;
+    }
+  });
+  if (Y) {
+    const double c = MODE == 1 ? 1.0 / (rhoa * oma) : 1.0;
+    U[s] = make_double4(c * ux, c * uy, c * uz, 0.0);
+  }
+  if (F) F[s] = make_double4(fx, fy, fz, 0.0);
+}
+
+// FLD conductivity at state Y from U = G(Y): D, dD/dR and R per particle.
+__global__ void k_fld_coef(DiffOpArgs a, const double *Y, const double4 *U, double *D, double *dDdR, double *R) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 u = U[s];
+  const double rho = a.q0[s].y, kap = a.kap[s];
+  const double un = sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+  const double r = un / (kap * rho * Y[s] * rho);
+  const double den = 6. + 3. * r + r * r;
+  const double lam = (2. + r) / den;
+  D[s] = kLightspeed * lam / kap / rho;
+  dDdR[s] = kLightspeed * (-r * (4. + r) / (den * den)) / kap / rho;
+  R[s] = r;
+}
+
+// f = A Y for the three operators (OP 3 = 2nw-+, 4 = 2nw+-, 5 = FLD).
+template <int DIM, int KT, int OP>
+__global__ __launch_bounds__(256) void k_nl_apply(DiffOpArgs a, const double *Y, const double4 *U, const double *D,
+                                                  double *out) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const double4 qa = a.q0[s];
+  const double inv_ha = qa.x, rhoa = qa.y, oma = qa.z;
+  double acc = 0.0;
+  if constexpr (OP == 5) {
+    const double za = Y[s] * rhoa, da = D[s];
+    const double I6[6] = {1, 1, 1, 0, 0, 0};
+    over_list(a, s, ps, [&](uint32_t j, double rx, double ry, double rz, double) {
+      const double4 qb = a.q0[j];
+      double G[6];
+      hess_basis<DIM, KT>(a, rx, ry, rz, inv_ha, G);
+      acc += a.pm[j].w / qb.y * kdot(I6, G) * (0.5 * (da + D[j])) * (Y[j] * qb.y - za);
+    });
+    out[s] = acc / rhoa;
+  } else {
+    const double4 ua = U[s];
+    const double3 ka = kmul(a.K, ua.x, ua.y, ua.z);
+    const double odda = 1.0 / (oma * rhoa * rhoa);
+    over_list(a, s, ps, [&](uint32_t j, double rx, double ry, double rz, double dr) {
+      const double4 qb = a.q0[j];
+      const double4 ub = U[j];
+      const double3 kb = kmul(a.K, ub.x, ub.y, ub.z);
+      const double mb = a.pm[j].w;
+      const double wa = nw_coef<DIM, KT>(a, dr, inv_ha);
+      if constexpr (OP == 3) {
+        const double wb = nw_coef<DIM, KT>(a, dr, qb.x);
+        const double oddb = 1.0 / (qb.z * qb.y * qb.y);
+        acc += mb * ((ka.x * rx + ka.y * ry + ka.z * rz) * wa * odda + (kb.x * rx + kb.y * ry + kb.z * rz) * wb * oddb);
+      } else {
+        acc += mb * ((kb.x - ka.x) * rx + (kb.y - ka.y) * ry + (kb.z - ka.z) * rz) * wa;
+      }
+    });
+    out[s] = OP == 3 ? acc : acc * odda;
+  }
+}
+
+// 2nw adjoint, the vector nu with <lam, A Y> = sum_s (K U_s) . nu_s:
+//   2nw-+ : nu_s = (lam_s F_s + m_s sum_{a in R(s)} lam_a nw(r_as, h_s)) / (Om_s rho_s^2)
+//   2nw+- : nu_s = m_s sum_{a in R(s)} lam_a nw(r_as, h_a)/(Om_a rho_a^2) - lam_s F_s/(Om_s rho_s^2)
+template <int DIM, int KT, int OP>
+__global__ __launch_bounds__(256) void k_nl_nu(DiffOpArgs a, const double *lam, const double4 *F, double4 *nu) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const double4 qs = a.q0[s];
+  const double odds = 1.0 / (qs.z * qs.y * qs.y);
+  double rx_ = 0, ry_ = 0, rz_ = 0;
+  over_rlist(a, s, ps, [&](uint32_t i, double rx, double ry, double rz, double dr) {
+    const double4 qi = a.q0[i];
+    double c;
+    if constexpr (OP == 3) c = lam[i] * nw_coef<DIM, KT>(a, dr, qs.x);
+    else c = lam[i] / (qi.z * qi.y * qi.y) * nw_coef<DIM, KT>(a, dr, qi.x);
+    rx_ += c * rx; ry_ += c * ry; rz_ += c * rz;
+  });
+  const double4 f = F[s];
+  const double ls = lam[s], ms = ps.w;
+  if constexpr (OP == 3)
+    nu[s] = make_double4((ls * f.x + ms * rx_) * odds, (ls * f.y + ms * ry_) * odds, (ls * f.z + ms * rz_) * odds, 0.0);
+  else
+    nu[s] = make_double4(ms * rx_ - ls * f.x * odds, ms * ry_ - ls * f.y * odds, ms * rz_ - ls * f.z * odds, 0.0);
+}
+
+// mu = K nu, and per block the tensor-gradient partials
+// sum_s sym(U_s (x) nu_s)_k for U = UA (slots 0..5) and UB (6..11).
+__global__ __launch_bounds__(256) void k_nl_mu(DiffOpArgs a, const double4 *nu, const double4 *UA, const double4 *UB,
+                                               double4 *mu, double *gpart) {
+  __shared__ double red[4][12];
+  int s = gtid_a();
+  double g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < a.n && a.real[s]) {
+    const double4 v = nu[s];
+    const double3 m = kmul(a.K, v.x, v.y, v.z);
+    mu[s] = make_double4(m.x, m.y, m.z, 0.0);
+    const double4 *Us[2] = {UA, UB};
+    for (int w = 0; w < 2; w++) {
+      if (!Us[w]) continue;
+      const double4 u = Us[w][s];
+      double *o = g + 6 * w;
+      o[0] = u.x * v.x; o[1] = u.y * v.y; o[2] = u.z * v.z;
+      o[3] = u.x * v.y + u.y * v.x; o[4] = u.x * v.z + u.z * v.x; o[5] = u.y * v.z + u.z * v.y;
+    }
+  }
+  for (int c = 0; c < 12; c++) {
+    double v = g[c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    double t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w][threadIdx.x];
+    gpart[(size_t)xcd_block_a(blockIdx.x, gridDim.x) * 12 + threadIdx.x] = t;
+  }
+}
+
+// out (+)= d<mu, G(Y)>/dY:
+//   mode 1: rho_s m_s sum_{a in R(s)} mu_a.nw(r_as, h_a)/(rho_a Om_a) - mu_s.F_s/Om_s
+//   mode 2: mu_s.F_s/(Om_s rho_s) + m_s/(Om_s rho_s^2) sum_{a in R(s)} rho_a mu_a.nw(r_as, h_s)
+template <int DIM, int KT, int MODE>
+__global__ __launch_bounds__(256) void k_nl_gT(DiffOpArgs a, const double4 *mu, const double4 *F, double *out, int add) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const double4 qs = a.q0[s];
+  double acc = 0.0;
+  over_rlist(a, s, ps, [&](uint32_t i, double rx, double ry, double rz, double dr) {
+    const double4 qi = a.q0[i];
+    const double4 m = mu[i];
+    const double dot = m.x * rx + m.y * ry + m.z * rz;
+    if constexpr (MODE == 1) acc += dot * nw_coef<DIM, KT>(a, dr, qi.x) / (qi.y * qi.z);
+    else acc += qi.y * dot * nw_coef<DIM, KT>(a, dr, qs.x);
+  });
+  const double4 m = mu[s], f = F[s];
+  const double mf = m.x * f.x + m.y * f.y + m.z * f.z;
+  double v;
+  if constexpr (MODE == 1) v = qs.y * ps.w * acc - mf / qs.z;
+  else v = mf / (qs.z * qs.y) + ps.w / (qs.z * qs.y * qs.y) * acc;
+  out[s] = add ? out[s] + v : v;
+}
+
+// FLD adjoint at state Y, the part with the conductivity held:
+//   out_s = m_s sum_{a in R(s)} (lam_a/rho_a) trH_as (D_a + D_s)/2 - lam_s sum_b (m_b/rho_b) trH_sb (D_s + D_b)/2
+// and sD_s = d<lam, f>/dD_s
+//   = ((lam_s/rho_s) sum_b (m_b/rho_b) trH_sb (Z_b - Z_s) + (m_s/rho_s) sum_{a in R(s)} (lam_a/rho_a) trH_as (Z_s - Z_a)) / 2
+// with Z = Y rho and trH at the list owner's h.
+template <int DIM, int KT>
+__global__ __launch_bounds__(256) void k_fld_adj(DiffOpArgs a, const double *lam, const double *Y, const double *D,
+                                                 double *out, double *sD) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double4 ps = a.pm[s];
+  const double4 qs = a.q0[s];
+  const double zs = Y[s] * qs.y, ds = D[s], ls = lam[s];
+  const double I6[6] = {1, 1, 1, 0, 0, 0};
+  double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+  over_list(a, s, ps, [&](uint32_t j, double rx, double ry, double rz, double) {
+    const double4 qb = a.q0[j];
+    double G[6];
+    hess_basis<DIM, KT>(a, rx, ry, rz, qs.x, G);
+    const double w = a.pm[j].w / qb.y * kdot(I6, G);
+    s1 += w * (0.5 * (ds + D[j]));
+    s2 += w * (Y[j] * qb.y - zs);
+  });
+  over_rlist(a, s, ps, [&](uint32_t i, double rx, double ry, double rz, double) {
+    const double4 qi = a.q0[i];
+    double G[6];
+    hess_basis<DIM, KT>(a, rx, ry, rz, qi.x, G);
+    const double w = lam[i] / qi.y * kdot(I6, G);
+    s3 += w * (0.5 * (D[i] + ds));
+    s4 += w * (zs - Y[i] * qi.y);
+  });
+  out[s] = ps.w * s3 - ls * s1;
+  sD[s] = 0.5 * (ls / qs.y * s2 + ps.w / qs.y * s4);
+}
+
+// FLD adjoint, the conductivity's dependence on the state: p = sD dD/dR,
+// v = p dR/dU = p U / (|U| kappa rho^2 Y) (0 where U = 0), and the direct
+// dR/dY = -R / Y into out.
+__global__ void k_fld_pv(DiffOpArgs a, const double *sD, const double *dDdR, const double *R, const double4 *U,
+                         const double *Y, double *out, double4 *v) {
+  int s = gtid_a();
+  if (s >= a.n || !a.real[s]) return;
+  const double p = sD[s] * dDdR[s];
+  const double4 u = U[s];
+  const double un = sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+  const double rho = a.q0[s].y;
+  const double c = un > 0.0 ? p / (un * a.kap[s] * rho * rho * Y[s]) : 0.0;
+  v[s] = make_double4(c * u.x, c * u.y, c * u.z, 0.0);
+  out[s] -= p * R[s] / Y[s];
+}
+
+#define SPH_INST_NL(DIM, KT)                                                                                    \
+  template __global__ void k_nl_grad<DIM, KT, 1>(DiffOpArgs, const double *, double4 *, double4 *);               \
+  template __global__ void k_nl_grad<DIM, KT, 2>(DiffOpArgs, const double *, double4 *, double4 *);               \
+  template __global__ void k_nl_apply<DIM, KT, 3>(DiffOpArgs, const double *, const double4 *, const double *,    \
+                                                  double *);                                                      \
+  template __global__ void k_nl_apply<DIM, KT, 4>(DiffOpArgs, const double *, const double4 *, const double *,    \
+                                                  double *);                                                      \
+  template __global__ void k_nl_apply<DIM, KT, 5>(DiffOpArgs, const double *, const double4 *, const double *,    \
+                                                  double *);                                                      \
+  template __global__ void k_nl_nu<DIM, KT, 3>(DiffOpArgs, const double *, const double4 *, double4 *);           \
+  template __global__ void k_nl_nu<DIM, KT, 4>(DiffOpArgs, const double *, const double4 *, double4 *);           \
+  template __global__ void k_nl_gT<DIM, KT, 1>(DiffOpArgs, const double4 *, const double4 *, double *, int);      \
+  template __global__ void k_nl_gT<DIM, KT, 2>(DiffOpArgs, const double4 *, const double4 *, double *, int);      \
+  template __global__ void k_fld_adj<DIM, KT>(DiffOpArgs, const double *, const double *, const double *, double *, \
+                                              double *);
+SPH_INST_NL(1, 0) SPH_INST_NL(1, 1) SPH_INST_NL(1, 2)
+SPH_INST_NL(2, 0) SPH_INST_NL(2, 1) SPH_INST_NL(2, 2)
+SPH_INST_NL(3, 0) SPH_INST_NL(3, 1) SPH_INST_NL(3, 2)
+
 template __global__ void k_batch_stage<true, double>(BatchStage);
 template __global__ void k_batch_stage<false, double>(BatchStage);
 template __global__ void k_batch_stage<true, float>(BatchStage);

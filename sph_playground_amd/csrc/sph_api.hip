@@ -42,7 +42,7 @@ __global__ void k_loop1(Loop1Args<double4>);
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1_simple(Loop1Args<double4>, Fields);
 template <typename V4>
-__global__ void k_post1(Fields, const double4 *, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
+__global__ void k_post1(Fields, PostSel, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
                         V4 *, V4 *, V4 *, double4 *, double *, long long *, long long *);
 template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ void k_loop2(Loop2Args<V4>);
@@ -52,6 +52,7 @@ __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
 __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
                                const int64_t *, int32_t *, int8_t *, int64_t);
 __global__ void k_set_i64(int64_t *, int64_t);
+__global__ void k_zero_ll(long long *, int);
 __global__ void k_prerun_check(Fields, int, int *);
 __global__ void k_counts_by_orig(const int *, const int64_t *, const uint8_t *, int, int64_t *);
 __global__ void k_find_orig(const int64_t *, int, int64_t, int *);
@@ -92,6 +93,20 @@ __global__ void k_batch_merge(int, int, int64_t, int, const int *, const uint32_
 template <bool FIRST, typename WT>
 __global__ void k_batch_stage(BatchStage);
 __global__ void k_batch_in(const double *, const int64_t *, const double *, int, double *, double *);
+template <int DIM, int KT, int MODE>
+__global__ void k_nl_grad(DiffOpArgs, const double *, double4 *, double4 *);
+__global__ void k_fld_coef(DiffOpArgs, const double *, const double4 *, double *, double *, double *);
+template <int DIM, int KT, int OP>
+__global__ void k_nl_apply(DiffOpArgs, const double *, const double4 *, const double *, double *);
+template <int DIM, int KT, int OP>
+__global__ void k_nl_nu(DiffOpArgs, const double *, const double4 *, double4 *);
+__global__ void k_nl_mu(DiffOpArgs, const double4 *, const double4 *, const double4 *, double4 *, double *);
+template <int DIM, int KT, int MODE>
+__global__ void k_nl_gT(DiffOpArgs, const double4 *, const double4 *, double *, int);
+template <int DIM, int KT>
+__global__ void k_fld_adj(DiffOpArgs, const double *, const double *, const double *, double *, double *);
+__global__ void k_fld_pv(DiffOpArgs, const double *, const double *, const double *, const double4 *, const double *,
+                         double *, double4 *);
 }  // namespace sph
 
 namespace {
@@ -199,13 +214,17 @@ struct sph_ctx {
   uint64_t keys_version = 0;      // pos_version the keys and cell starts belong to
   Grid keys_grid{};
   int64_t keys_n = -1, keys_nkeys = -1;
+  // per-sweep counters of loop 1 (still-active stripes, pair stripes, a
+  // ran flag), one region per sweep of a step, and their pinned mirror
+  long long *d_sweep = nullptr, *h_sweep = nullptr;
+  int sweep_hint = 4;             // sweeps the next step launches before it synchronises
   sph_stats stats{};
   bool have_step = false;
   // the resident store holds the derivatives of the last successful iterate
   // of this state (nothing loaded since): a leapfrog pass then starts at its
   // predictor, as every pass after main.f90:219's initial iterate does
   bool derivs_ready = false;
-  hipEvent_t ev[9]{};             // 8: the short waits of read_ints
+  hipEvent_t ev[11]{};            // 8: the short waits of read_ints; 9, 10: a batch of Newton sweeps
   // differentiable diffusion (sph_diff_*)
   bool diff_ready = false;
   Phys diff_phys{};
@@ -233,6 +252,11 @@ struct sph_ctx {
   int64_t ckpt_cap = 0;
   int fw_s = 0, fw_steps = 0, fw_every = 0;
   double fw_dt = 0.0, fw_K[6] = {0, 0, 0, 0, 0, 0};
+  // operators that read grad T (2nw-+ 3, 2nw+- 4, flux-limited diffusion 5;
+  // 0 = the Hessian operators of the assembled path)
+  int diff_op = 0;
+  double4 *nlF = nullptr, *nlU = nullptr, *nlUB = nullptr, *nlNu = nullptr, *nlMu = nullptr;
+  double *nlD = nullptr, *nlDD = nullptr, *nlR = nullptr, *nlSD = nullptr;
 
   Fields fields() const {
     Fields F;
@@ -444,6 +468,11 @@ static bool same_grid(const Grid &a, const Grid &b) {
 // error bits) can be left to a later synchronisation: copies to these pinned
 // slots, taken by finish_neighbours.
 constexpr int kNeighLL = 1024;   // h_ll slots of the search's read-back ([0, kFlagSlots + kStripes) of d_stripes)
+// loop 1's per-sweep counter regions: [0, kStripes) still active,
+// [kStripes, 2 kStripes) pairs, [2 kStripes] ran; region k for sweep k
+constexpr int kSweepRegion = 2 * kStripes + 8;
+constexpr int kSweepRegions = 242;  // sweeps 1..240 (the cap on max_sweeps) and slack
+constexpr int kWhichInt = 6;        // d_int slot: parity of the last sweep that ran
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
 
 // the error words of the step (ints in d_stripes), and the same words in a
@@ -468,6 +497,12 @@ static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
   if (!(bits & EB_OVERFLOW)) return SPH_OK;
   RCHK(ensure_nb(c, maxcnt + 8));
   return kRedoNeigh;
+}
+
+// The deferred read-back after a synchronisation the caller made.
+static int take_neighbours_now(sph_ctx *c) {
+  const long long *ll = c->h_ll + kNeighLL;
+  return finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
 }
 
 // A deferred read-back, once the stream has passed its copies.
@@ -713,85 +748,70 @@ static void launch_loop1_k(sph_ctx *c, Loop1Args<double4> &a, int grad, bool sim
   else launch_loop1<DIM, KT_GAUSS>(c, a, grad, simple, F);
 }
 
-static int run_loop1(sph_ctx *c, const Phys &P) {
+// c1simple (circuit1.f90:227-305): one pass, grad T only with diffusion and
+// the 2nw operators; the search is checked first (it writes h, rho, om in
+// place), so this waits for the pass.
+static int run_loop1_simple(sph_ctx *c, const Phys &P) {
   const int n = (int)c->n;
   Fields F = c->fields();
   Loop1Args<double4> a{};
   a.pm = c->pm; a.h0 = c->h0; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt; a.eps = c->eps;
-  // error word and the first sweep's counters: cleared by this step's k_prep
   a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = step_err(c) + kErrLoop1;
   a.b = c->box; a.p = P; a.n = n; a.npad = (int)c->npad;
-  double tol = c->opt.newton_tol > 0 ? c->opt.newton_tol : (c->opt.precision == 1 ? 1e-6 : 1e-8);
-  a.tol = tol;
-  int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
-  c->stats.loop1_pairs = 0;
-  int bits = 0;
-  if (!P.adden) {
-    // c1simple: grad T only with diffusion and the 2nw operators
-    int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
-    a.cur = c->st[0];
-    a.next = c->st[1];
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    if (P.dim == 1) launch_loop1_k<1>(c, a, grad, true, F);
-    else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, true, F);
-    else launch_loop1_k<3>(c, a, grad, true, F);
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    c->final_st = c->st[0];
-    c->stats.sweeps = 0;
-    HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
-    RCHK(take_neighbours(c));
-    c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
-    c->stats.loop1_pairs = c->stats.n_pairs;
-    return map_err(copied_err(c->h_ll, kErrLL, kErrLoop1));
-  }
-  int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
-  int sweeps = 0;
-  // while (maxval(eps, eps > 0) > tol .and. iter < 100), circuit1.f90:114;
-  // each sweep counts the particles it leaves active
-  for (int it = 1; c->n_real > 0 && it <= maxsw; it++) {
+  const int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
+  a.cur = c->st[0];
+  a.next = c->st[1];
+  HIPCHK(hipEventRecord(c->ev[5], c->stream));
+  if (P.dim == 1) launch_loop1_k<1>(c, a, grad, true, F);
+  else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, true, F);
+  else launch_loop1_k<3>(c, a, grad, true, F);
+  HIPCHK(hipEventRecord(c->ev[6], c->stream));
+  c->final_st = c->st[0];
+  c->stats.sweeps = 0;
+  HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipEventRecord(c->ev[8], c->stream));
+  HIPCHK(wait_event(c->ev[8]));
+  RCHK(take_neighbours(c));
+  c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
+  c->stats.loop1_pairs = c->stats.n_pairs;
+  return map_err(copied_err(c->h_ll, kErrLL, kErrLoop1));
+}
+
+// Sweeps first..last of c1advanced (circuit1.f90:104-211) on the stream with
+// no host round trip: each sweep runs only if the one before left a particle
+// active (while maxval(eps) > tol, circuit1.f90:114) and no error stopped the
+// search or loop 1 (k_loop1 decides on the device); their counters go to
+// the sweep regions, zeroed here first.
+static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
+  const int n = (int)c->n;
+  Fields F = c->fields();
+  Loop1Args<double4> a{};
+  a.pm = c->pm; a.h0 = c->h0; a.real = c->real; a.nb = c->nb; a.cnt = c->cnt; a.eps = c->eps;
+  a.dtdx = F.f[F_DTDX]; a.dtdy = F.f[F_DTDY]; a.dtdz = F.f[F_DTDZ]; a.err = step_err(c) + kErrLoop1;
+  a.b = c->box; a.p = P; a.n = n; a.npad = (int)c->npad;
+  a.tol = c->opt.newton_tol > 0 ? c->opt.newton_tol : (c->opt.precision == 1 ? 1e-6 : 1e-8);
+  a.serr = step_err(c) + kErrSearch;
+  a.which = c->d_int + kWhichInt;
+  const int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
+  const int nreg = (last - first + 1) * kSweepRegion;
+  hipLaunchKernelGGL(k_zero_ll, dim3((nreg + 255) / 256), dim3(256), 0, c->stream,
+                     c->d_sweep + (size_t)first * kSweepRegion, nreg);
+  (void)hipEventRecord(c->ev[9], c->stream);
+  for (int it = first; it <= last; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
     a.trc = c->tr[(it - 1) & 1];
     a.trn = c->tr[it & 1];
     a.jrc = c->jr[(it - 1) & 1];
     a.jrn = c->jr[it & 1];
-    // odd sweeps count into the region k_prep cleared, even ones into their own
-    const int region = (it & 1) ? kSweepOddLL : kSweepEvenLL;
-    if (it > 1) HIPCHK(hipMemsetAsync(c->d_stripes + region, 0, sizeof(long long) * 2 * kStripes, c->stream));
-    a.stripes = c->d_stripes + region;
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    a.stripes = c->d_sweep + (size_t)it * kSweepRegion;
+    a.prev = it > 1 ? c->d_sweep + (size_t)(it - 1) * kSweepRegion : nullptr;
+    a.sweep = it;
     if (P.dim == 1) launch_loop1_k<1>(c, a, grad, false, F);
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
     else launch_loop1_k<3>(c, a, grad, false, F);
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    sweeps = it;
-    // the sweep's counters to h_ll[0, 2 kStripes) and the error words as they
-    // lie after the odd region (one copy for odd sweeps, whose region runs on
-    // into them)
-    if (it & 1) {
-      HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kSweepOddLL, sizeof(long long) * (kZeroLL - kSweepOddLL),
-                            hipMemcpyDeviceToHost, c->stream));
-    } else {
-      HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kSweepEvenLL, sizeof(long long) * 2 * kStripes,
-                            hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(c->h_ll + (kErrLL - kSweepOddLL), c->d_stripes + kErrLL, sizeof(long long) * 4,
-                            hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
-    RCHK(take_neighbours(c));
-    c->stats.ms_k_loop1 += elapsed(c, c->ev[5], c->ev[6]);
-    c->stats.loop1_pairs += stripe_sum(c->h_ll, 1);
-    bits = copied_err(c->h_ll, kSweepOddLL, kErrLoop1);
-    if (bits) break;  // circuit1.f90:202-204 stops inside the sweep
-    if (stripe_sum(c->h_ll, 0) == 0) break;
   }
-  c->final_st = c->st[sweeps & 1];
-  c->stats.sweeps = sweeps;
-  return map_err(bits);
+  (void)hipEventRecord(c->ev[10], c->stream);
 }
 
 template <int DIM, int KT, int MODE>
@@ -860,7 +880,13 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   V4 *q3 = want_q3 ? (V4 *)c->q[3] : nullptr;
   V4 *q4 = want_q4 && (generic || (fl & 2)) ? (V4 *)c->q[4] : nullptr;
   long long *none = nullptr;
-  hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, c->final_st, c->real, n, P.adden ? 1 : 0,
+  PostSel sel{};
+  sel.fin0 = P.adden ? c->st[0] : c->final_st;
+  sel.fin1 = c->st[1];
+  sel.which = P.adden ? c->d_int + kWhichInt : nullptr;
+  sel.serr = step_err(c) + kErrSearch;
+  sel.err1 = step_err(c) + kErrLoop1;
+  hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, sel, c->real, n, P.adden ? 1 : 0,
                      c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
                      generic ? nullptr : c->trd, none, c->d_stripes + kHmaxLL);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
@@ -923,6 +949,100 @@ static Phys call_phys(const sph_ctx *c, const double *state) {
   return P;
 }
 
+// The tail of a step after loop 1: Newton bookkeeping, EOS and packs
+// (k_post1), loop 2, mti gravity and the closing record's copy.
+static int enqueue_tail(sph_ctx *c, const Phys &P) {
+  HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
+  if (rc == SPH_OK && c->gravity)
+    hipLaunchKernelGGL(k_gravity, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
+                       c->gravity);
+  HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  return rc;
+}
+
+constexpr int kErrCopyLL = 1700;  // h_ll slots of the error words' read-back
+
+// c1advanced steps: the search (its read-back left pending), a batch of
+// sweeps, the tail, then one synchronisation.  Only when k_prep's v / B
+// flags must come back before the tail picks loop 2's packs (after a load or
+// a leapfrog change of v or B) does the step wait after the sweeps.  Lists
+// that overflowed leave loop 1 and k_post1 idle (device-side checks), so the
+// step searches again with room and starts over; a batch that ends with
+// particles still active is followed by more sweeps and the tail again.
+static int run_advanced(sph_ctx *c, const Phys &P) {
+  const int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
+  int first = 1, sweeps = 0, bits = 0;
+  long long pairs = 0;
+  float ms1 = 0.f;
+  if (c->n_real <= 0) {  // no sweep, no active particle (circuit1.f90:114)
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->neigh_pending) { c->neigh_pending = false; RCHK(take_neighbours(c)); }
+    c->stats.sweeps = 0;
+    c->stats.loop1_pairs = 0;
+    const int rc = enqueue_tail(c, P);
+    HIPCHK(wait_event(c->ev[3]));
+    const int rc2 = finish_loop2(c);
+    return rc != SPH_OK ? rc : rc2;
+  }
+  for (;;) {
+    const int last = std::min(maxsw, first + std::max(1, c->sweep_hint) - 1);
+    launch_sweeps(c, P, first, last);
+    if (c->neigh_pending && c->flags_pending) {
+      HIPCHK(hipEventRecord(c->ev[8], c->stream));
+      HIPCHK(wait_event(c->ev[8]));
+      c->neigh_pending = false;
+      const int fin = take_neighbours_now(c);
+      if (fin == kRedoNeigh) {
+        RCHK(build_neighbours(c));
+        first = 1; sweeps = 0; pairs = 0; ms1 = 0.f;
+        continue;
+      }
+      RCHK(fin);
+    }
+    const int rc = enqueue_tail(c, P);
+    const size_t nreg = (size_t)(last - first + 1) * kSweepRegion;
+    HIPCHK(hipMemcpyAsync(c->h_sweep + (size_t)first * kSweepRegion, c->d_sweep + (size_t)first * kSweepRegion,
+                          sizeof(long long) * nreg, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_ll + kErrCopyLL, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
+    ms1 += elapsed(c, c->ev[9], c->ev[10]);
+    if (rc != SPH_OK) return rc;
+    if (c->neigh_pending) {
+      c->neigh_pending = false;
+      const int fin = take_neighbours_now(c);
+      if (fin == kRedoNeigh) {
+        RCHK(build_neighbours(c));
+        first = 1; sweeps = 0; pairs = 0; ms1 = 0.f;
+        continue;
+      }
+      RCHK(fin);
+    }
+    for (int it = first; it <= last; it++) {
+      const long long *r = c->h_sweep + (size_t)it * kSweepRegion;
+      if (!r[2 * kStripes]) break;
+      sweeps = it;
+      pairs += stripe_sum(r, 1);
+    }
+    bits = copied_err(c->h_ll + kErrCopyLL, kErrLL, kErrLoop1);
+    if (bits) break;  // circuit1.f90:202-204 stops inside the sweep
+    if (sweeps == last && last < maxsw && stripe_sum(c->h_sweep + (size_t)last * kSweepRegion, 0) > 0) {
+      first = last + 1;  // still active: more sweeps, then the tail again
+      continue;
+    }
+    break;
+  }
+  c->stats.sweeps = sweeps;
+  c->stats.loop1_pairs = pairs;
+  c->stats.ms_k_loop1 = ms1;
+  c->sweep_hint = std::min(maxsw, std::max(2, sweeps + 1));
+  const int rc2 = finish_loop2(c);
+  const int r1 = map_err(bits);
+  return r1 != SPH_OK ? r1 : rc2;
+}
+
 static int run_step(sph_ctx *c, const double *state) {
   if (c->n <= 0) return SPH_ERR_BAD_ARG;
   // a new step changes the geometry the differentiable operator was prepared on
@@ -934,36 +1054,25 @@ static int run_step(sph_ctx *c, const double *state) {
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
-  // the Newton loop 1 follows the search without a host round trip; its
-  // first synchronisation brings the search's read-back, and lists that
-  // overflowed send the step back to a full search with room for them (it
-  // writes nothing the search set-up reads).  c1simple writes h, rho and om
-  // in place, so its search is checked before it runs.
+  // the search's read-back is taken with the step's closing synchronisation
+  // (c1advanced); c1simple writes h, rho and om in place, so its search is
+  // checked before it runs
   RCHK(build_neighbours(c, P.adden != 0));
   c->hmax_version = 0;  // h changes from here; this step's k_post1 reports the new h_max
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
-  int r1 = run_loop1(c, P);
-  if (r1 == SPH_OK && c->neigh_pending) {  // loop 1 ran no sweep (no real particles)
-    HIPCHK(hipStreamSynchronize(c->stream));
-    r1 = take_neighbours(c);
-  }
-  if (r1 == kRedoNeigh) {
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    RCHK(build_neighbours(c));
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    r1 = run_loop1(c, P);
+  int rc;
+  if (P.adden) {
+    rc = run_advanced(c, P);
+  } else {
+    int r1 = run_loop1_simple(c, P);
+    c->neigh_pending = false;
+    RCHK(r1);
+    rc = enqueue_tail(c, P);
+    HIPCHK(wait_event(c->ev[3]));
+    const int rc2 = finish_loop2(c);
+    if (rc == SPH_OK) rc = rc2;
   }
   c->neigh_pending = false;
-  RCHK(r1);
-  HIPCHK(hipEventRecord(c->ev[2], c->stream));
-  int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
-  if (rc == SPH_OK && c->gravity)
-    hipLaunchKernelGGL(k_gravity, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
-                       c->gravity);
-  HIPCHK(hipEventRecord(c->ev[3], c->stream));
-  HIPCHK(wait_event(c->ev[3]));
-  const int rc2 = finish_loop2(c);
-  if (rc == SPH_OK) rc = rc2;
   float t01 = 0, t12 = 0, t23 = 0;
   (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
@@ -1129,7 +1238,9 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
       dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
       dev_alloc((void **)&c->d_stripes, kStepLL * sizeof(long long)) != SPH_OK ||
       hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_ll, 2048 * sizeof(long long)) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_ll, 2048 * sizeof(long long)) != hipSuccess ||
+      dev_alloc((void **)&c->d_sweep, kSweepRegions * kSweepRegion * sizeof(long long)) != SPH_OK ||
+      hipHostMalloc((void **)&c->h_sweep, kSweepRegions * kSweepRegion * sizeof(long long)) != hipSuccess) {
     sph_ctx_destroy(c);
     return SPH_ERR_NOMEM;
   }
@@ -1148,11 +1259,13 @@ int sph_ctx_destroy(sph_ctx *c) {
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
                   c->dvec, c->gpart, c->ckpt, c->rcnt, c->dpxh, c->dirho, c->dW, c->ddW, c->dR, c->dS, c->dZ, c->dLZ[0], c->dLZ[1], c->dLZ[2], c->dLZ[3],
-                  c->gkdev};
+                  c->gkdev, c->nlF, c->nlU, c->nlUB, c->nlNu, c->nlMu, c->nlD, c->nlDD, c->nlR, c->nlSD};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
   if (c->h_ll) (void)hipHostFree(c->h_ll);
+  if (c->h_sweep) (void)hipHostFree(c->h_sweep);
+  if (c->d_sweep) (void)hipFree(c->d_sweep);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1519,8 +1632,67 @@ static DiffOpArgs diff_args(sph_ctx *c, const double K[6]) {
   for (int k = 0; k < 6; k++) a.K[k] = K[k];
   a.wcv = c->diff_phys.wcv; a.fwc = c->diff_phys.fwc; a.ddw = c->diff_phys.ddw;
   a.n = (int)c->n; a.npad = (int)c->npad;
+  a.kap = c->fields().f[F_KAPPA];
   return a;
 }
+
+// ---- operators that read grad T (frozen geometry; see sph_adjoint.hip)
+template <int DIM, int KT>
+static void nl_grad_k(sph_ctx *c, const DiffOpArgs &a, const double *Y, double4 *U, double4 *F) {
+  const unsigned G = grid_for(c->n);
+  if (c->diff_op == 4) hipLaunchKernelGGL((k_nl_grad<DIM, KT, 2>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, U, F);
+  else hipLaunchKernelGGL((k_nl_grad<DIM, KT, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, U, F);
+}
+template <int DIM, int KT>
+static void nl_apply_k(sph_ctx *c, const DiffOpArgs &a, const double *Y, double *out) {
+  const unsigned G = grid_for(c->n);
+  nl_grad_k<DIM, KT>(c, a, Y, c->nlU, nullptr);
+  if (c->diff_op == 5) {
+    hipLaunchKernelGGL(k_fld_coef, dim3(G), dim3(kBlock), 0, c->stream, a, Y, c->nlU, c->nlD, c->nlDD, c->nlR);
+    hipLaunchKernelGGL((k_nl_apply<DIM, KT, 5>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, c->nlU, c->nlD, out);
+  } else if (c->diff_op == 3) {
+    hipLaunchKernelGGL((k_nl_apply<DIM, KT, 3>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, c->nlU, c->nlD, out);
+  } else {
+    hipLaunchKernelGGL((k_nl_apply<DIM, KT, 4>), dim3(G), dim3(kBlock), 0, c->stream, a, Y, c->nlU, c->nlD, out);
+  }
+}
+// out = A^T lam (2nw: A linear; FLD: the Jacobian of f at Ya, transposed);
+// for 2nw the tensor gradients lam^T dA/dK Ya -> gkdev[0..5], Yb -> [6..11]
+template <int DIM, int KT>
+static void nl_adjoint_k(sph_ctx *c, const DiffOpArgs &a, const double *lam, const double *Ya, const double *Yb,
+                         double *out) {
+  const unsigned G = grid_for(c->n);
+  if (c->diff_op == 5) {
+    nl_grad_k<DIM, KT>(c, a, Ya, c->nlU, nullptr);
+    hipLaunchKernelGGL(k_fld_coef, dim3(G), dim3(kBlock), 0, c->stream, a, Ya, c->nlU, c->nlD, c->nlDD, c->nlR);
+    hipLaunchKernelGGL((k_fld_adj<DIM, KT>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, Ya, c->nlD, out, c->nlSD);
+    hipLaunchKernelGGL(k_fld_pv, dim3(G), dim3(kBlock), 0, c->stream, a, c->nlSD, c->nlDD, c->nlR, c->nlU, Ya, out,
+                       c->nlMu);
+    hipLaunchKernelGGL((k_nl_gT<DIM, KT, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->nlMu, c->nlF, out, 1);
+    return;
+  }
+  nl_grad_k<DIM, KT>(c, a, Ya, c->nlU, nullptr);
+  if (Yb) nl_grad_k<DIM, KT>(c, a, Yb, c->nlUB, nullptr);
+  if (c->diff_op == 3)
+    hipLaunchKernelGGL((k_nl_nu<DIM, KT, 3>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, c->nlF, c->nlNu);
+  else
+    hipLaunchKernelGGL((k_nl_nu<DIM, KT, 4>), dim3(G), dim3(kBlock), 0, c->stream, a, lam, c->nlF, c->nlNu);
+  hipLaunchKernelGGL(k_nl_mu, dim3(G), dim3(kBlock), 0, c->stream, a, c->nlNu, c->nlU, Yb ? c->nlUB : nullptr,
+                     c->nlMu, c->gpart);
+  hipLaunchKernelGGL(k_gk_accum, dim3(1), dim3(256), 0, c->stream, c->gpart, (int)G, 1.0, 0.0, c->gkdev);
+  hipLaunchKernelGGL(k_gk_accum, dim3(1), dim3(256), 0, c->stream, c->gpart, (int)G, 0.0, 1.0, c->gkdev + 6);
+  if (c->diff_op == 4)
+    hipLaunchKernelGGL((k_nl_gT<DIM, KT, 2>), dim3(G), dim3(kBlock), 0, c->stream, a, c->nlMu, c->nlF, out, 0);
+  else
+    hipLaunchKernelGGL((k_nl_gT<DIM, KT, 1>), dim3(G), dim3(kBlock), 0, c->stream, a, c->nlMu, c->nlF, out, 0);
+}
+#define NL_DISPATCH(FN, ...)                                                                       \
+  do {                                                                                            \
+    const int d_ = c->diff_phys.dim, k_ = c->kt;                                                  \
+    if (d_ == 1) { if (k_ == KT_M4) FN<1, KT_M4>(__VA_ARGS__); else if (k_ == KT_M6) FN<1, KT_M6>(__VA_ARGS__); else FN<1, KT_GAUSS>(__VA_ARGS__); } \
+    else if (d_ == 2) { if (k_ == KT_M4) FN<2, KT_M4>(__VA_ARGS__); else if (k_ == KT_M6) FN<2, KT_M6>(__VA_ARGS__); else FN<2, KT_GAUSS>(__VA_ARGS__); } \
+    else { if (k_ == KT_M4) FN<3, KT_M4>(__VA_ARGS__); else if (k_ == KT_M6) FN<3, KT_M6>(__VA_ARGS__); else FN<3, KT_GAUSS>(__VA_ARGS__); } \
+  } while (0)
 
 template <int DIM>
 static void launch_weights_d(sph_ctx *c, const DiffOpArgs &a) {
@@ -1669,12 +1841,27 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
   (void)hipSetDevice(c->device);
   if (c->opt.precision != 0 || c->n_fixed != 0) return SPH_ERR_UNSUPPORTED;
   Phys P = call_phys(c, state);
-  if (!P.diff || P.fld || !(P.ddw == 400 || P.ddw == 401 || P.ddw == 402)) return SPH_ERR_UNSUPPORTED;
+  const bool hess = P.ddw == 400 || P.ddw == 401 || P.ddw == 402;
+  int op;
+  if (!P.diff) return SPH_ERR_UNSUPPORTED;
+  if (P.fld) {
+    // FLD: isotropic (circuit2.f90:189), one of the Hessian operators
+    if (!hess || P.diso != 1) return SPH_ERR_UNSUPPORTED;
+    op = 5;
+  } else if (P.ddw == 403 || P.ddw == 404) {
+    // the 2nw operators with one tensor K for every particle
+    op = P.ddw == 403 ? 3 : 4;
+  } else if (hess) {
+    op = 0;
+  } else {
+    return SPH_ERR_UNSUPPORTED;
+  }
   c->keep_q0 = true;
   const int step_rc = run_step(c, state);  // neighbours, Newton h, EOS, loop 2: h, rho, q0 now frozen
   c->keep_q0 = false;
   RCHK(step_rc);
   c->diff_phys = P;
+  c->diff_op = op;
   const int n = (int)c->n;
   unsigned G = grid_for(n);
   const int64_t npairs = c->stats.n_pairs;
@@ -1684,7 +1871,7 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
   RCHK(dalloc(&c->rcnt, c->npad));
   RCHK(dalloc(&c->dpxh, c->npad));
   RCHK(dalloc(&c->dirho, c->npad));
-  RCHK(dalloc(&c->dW, (size_t)c->npad * c->nb_cap));
+  if (op == 0) RCHK(dalloc(&c->dW, (size_t)c->npad * c->nb_cap));
   RCHK(dalloc(&c->ddW, c->npad));
   RCHK(dalloc(&c->dZ, c->npad));
   for (int q = 0; q < 4; q++) RCHK(dalloc(&c->dLZ[q], c->npad));
@@ -1734,8 +1921,11 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
     for (int k = 0; k < n; k++) rmax = std::max(rmax, hoff[k + 1] - hoff[k]);
     c->rcap = std::max(8, (rmax + 7) / 8 * 8);
     const size_t rslots = (size_t)c->npad * c->rcap;
-    if ((rc = dalloc(&c->rent, rslots)) || (rc = dalloc(&c->dR, rslots)) || (rc = dalloc(&c->dS, rslots))) break;
-    HIPCHK(hipMemsetAsync(c->dS, 0, sizeof(double) * rslots, c->stream));
+    if ((rc = dalloc(&c->rent, rslots))) break;
+    if (op == 0) {
+      if ((rc = dalloc(&c->dR, rslots)) || (rc = dalloc(&c->dS, rslots))) break;
+      HIPCHK(hipMemsetAsync(c->dS, 0, sizeof(double) * rslots, c->stream));
+    }
     // owner-major enumeration of the pairs, stable sort on the neighbour index
     bytes = c->scan_tmp_bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, bytes, c->cnt, foff, n, c->stream));
@@ -1756,6 +1946,20 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
   if (c->gpart) (void)hipFree(c->gpart);
   c->gpart = nullptr;
   RCHK(dalloc(&c->gpart, (size_t)G * 12));
+  if (op != 0) {
+    RCHK(dalloc(&c->nlF, c->npad));
+    RCHK(dalloc(&c->nlU, c->npad));
+    RCHK(dalloc(&c->nlUB, c->npad));
+    RCHK(dalloc(&c->nlNu, c->npad));
+    RCHK(dalloc(&c->nlMu, c->npad));
+    RCHK(dalloc(&c->nlD, c->npad));
+    RCHK(dalloc(&c->nlDD, c->npad));
+    RCHK(dalloc(&c->nlR, c->npad));
+    RCHK(dalloc(&c->nlSD, c->npad));
+    const double K0[6] = {0, 0, 0, 0, 0, 0};
+    DiffOpArgs a = diff_args(c, K0);
+    NL_DISPATCH(nl_grad_k, c, a, nullptr, nullptr, c->nlF);  // F_s = sum_b m_b nw(r_sb, h_s)
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   c->fw_tail = false;
   c->diff_ready = true;
@@ -1763,13 +1967,19 @@ int sph_diff_prepare(sph_ctx *c, const double state[49]) {
 }
 
 int sph_diff_apply(sph_ctx *c, const double K[6], const double *d_T, double *d_out) {
-  if (!c || !c->diff_ready || !K || !d_T || !d_out) return SPH_ERR_BAD_ARG;
+  if (!c || !c->diff_ready || !d_T || !d_out || (!K && c->diff_op != 5)) return SPH_ERR_BAD_ARG;
   (void)hipSetDevice(c->device);
   RCHK(ensure_dvec(c, 2));
   const int n = (int)c->n;
   unsigned G = grid_for(n);
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_T, c->orig[c->cur], n, dv(c, 0));
-  diff_apply(c, K, dv(c, 0), dv(c, 1));
+  if (c->diff_op != 0) {
+    const double K0[6] = {0, 0, 0, 0, 0, 0};
+    DiffOpArgs a = diff_args(c, K ? K : K0);
+    NL_DISPATCH(nl_apply_k, c, a, dv(c, 0), dv(c, 1));
+  } else {
+    diff_apply(c, K, dv(c, 0), dv(c, 1));
+  }
   hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, dv(c, 1), c->orig[c->cur], n, d_out);
   HIPCHK(hipStreamSynchronize(c->stream));
   return SPH_OK;
@@ -1777,7 +1987,8 @@ int sph_diff_apply(sph_ctx *c, const double K[6], const double *d_T, double *d_o
 
 int sph_diff_apply_adjoint(sph_ctx *c, const double K[6], const double *d_lam, const double *d_Ya,
                            const double *d_Yb, double *d_out, double gA[6], double gB[6]) {
-  if (!c || !c->diff_ready || !K || !d_lam || !d_Ya || !d_Yb || !d_out || !gA || !gB) return SPH_ERR_BAD_ARG;
+  if (!c || !c->diff_ready || !d_lam || !d_Ya || !d_out || !gA || !gB) return SPH_ERR_BAD_ARG;
+  if (c->diff_op != 5 && (!K || !d_Yb)) return SPH_ERR_BAD_ARG;
   (void)hipSetDevice(c->device);
   RCHK(ensure_dvec(c, 4));
   const int n = (int)c->n;
@@ -1785,12 +1996,18 @@ int sph_diff_apply_adjoint(sph_ctx *c, const double K[6], const double *d_lam, c
   const int64_t *o = c->orig[c->cur];
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_lam, o, n, dv(c, 0));
   hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Ya, o, n, dv(c, 1));
-  hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Yb, o, n, dv(c, 2));
+  if (d_Yb) hipLaunchKernelGGL(k_to_sorted, dim3(G), dim3(kBlock), 0, c->stream, d_Yb, o, n, dv(c, 2));
   HIPCHK(hipMemsetAsync(c->gkdev, 0, sizeof(double) * 12, c->stream));
-  diff_adj_stage(c, K, dv(c, 0), dv(c, 1), nullptr, 1.0, 0.0, dv(c, 3));
-  RCHK(diff_gcontract(c, K, c->gkdev));
-  diff_adj_stage(c, K, dv(c, 0), dv(c, 2), nullptr, 1.0, 0.0, dv(c, 3));
-  RCHK(diff_gcontract(c, K, c->gkdev + 6));
+  if (c->diff_op != 0) {
+    const double K0[6] = {0, 0, 0, 0, 0, 0};
+    DiffOpArgs a = diff_args(c, K ? K : K0);
+    NL_DISPATCH(nl_adjoint_k, c, a, dv(c, 0), dv(c, 1), c->diff_op == 5 ? nullptr : dv(c, 2), dv(c, 3));
+  } else {
+    diff_adj_stage(c, K, dv(c, 0), dv(c, 1), nullptr, 1.0, 0.0, dv(c, 3));
+    RCHK(diff_gcontract(c, K, c->gkdev));
+    diff_adj_stage(c, K, dv(c, 0), dv(c, 2), nullptr, 1.0, 0.0, dv(c, 3));
+    RCHK(diff_gcontract(c, K, c->gkdev + 6));
+  }
   hipLaunchKernelGGL(k_from_sorted, dim3(G), dim3(kBlock), 0, c->stream, dv(c, 3), o, n, d_out);
   double g12[12];
   HIPCHK(hipMemcpyAsync(g12, c->gkdev, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
@@ -1804,6 +2021,7 @@ int sph_diff_apply_adjoint(sph_ctx *c, const double K[6], const double *d_lam, c
 int sph_diff_forward(sph_ctx *c, const double K[6], int s, double dt, int nsteps, int every, const double *d_T0,
                      double *d_T) {
   if (!c || !c->diff_ready || !K || !d_T0 || !d_T || s < 2 || nsteps < 1 || every < 1) return SPH_ERR_BAD_ARG;
+  if (c->diff_op != 0) return SPH_ERR_UNSUPPORTED;  // stepped stage by stage by the caller (autograd.py)
   (void)hipSetDevice(c->device);
   const int n = (int)c->n;
   unsigned G = grid_for(n);
@@ -2010,6 +2228,7 @@ int sph_batch_create(sph_ctx *const *ctxs, int nprob, const double *K, int preci
   for (int p = 0; p < nprob; p++) {
     const sph_ctx *c = ctxs[p];
     if (!c || !c->diff_ready || c->device != ctxs[0]->device) return SPH_ERR_BAD_ARG;
+    if (c->diff_op != 0) return SPH_ERR_UNSUPPORTED;  // the assembled Hessian operators only
     n += c->n;
     np += c->stats.n_pairs;
     maxc = std::max(maxc, c->last_maxcnt);

@@ -105,12 +105,29 @@ struct Loop1Args {
   double4 *jrn;            // ... of sweep k+1
   double *eps;
   double *dtdx, *dtdy, *dtdz;
-  long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs
+  long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs, [2k] ran
   int *err;
   Box b;
   Phys p;
   double tol;
   int n, npad;
+  // sweeps decided on the device: the sweep runs only if the previous one
+  // (its stripes at `prev`, null for the first) left a particle active and no
+  // error stopped loop 1 or the search (`serr`); a sweep that runs records
+  // its parity in *which, the buffer its results went to
+  const long long *prev;
+  const int *serr;
+  int *which;
+  int sweep;
+};
+
+// Where k_post1 finds loop 1's result: st[*which] of the two sweep buffers
+// (fin0 alone when which is null); nothing is written when the search (serr)
+// or loop 1 (err1) raised an error bit: the step then stops or searches again.
+struct PostSel {
+  const double4 *fin0, *fin1;
+  const int *which;
+  const int *serr, *err1;
 };
 
 template <typename V4>
@@ -173,6 +190,7 @@ struct DiffOpArgs {
   double K[6];         // xx, yy, zz, xy, xz, yz
   double wcv, fwc;
   int ddw, n, npad;
+  const double *kap;   // opacity of the frozen state (sorted), flux-limited diffusion
 };
 
 }  // namespace sph

@@ -610,6 +610,21 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
 template <int DIM, int KT, int GRAD, typename PT>
 __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   __shared__ long long lds[4];
+  __shared__ int skip;
+  // while maxval(eps) > tol (circuit1.f90:114), decided here: the first
+  // wave reads the previous sweep's still-active stripes (set before this
+  // launch began) and the error words, so the whole block agrees
+  if (threadIdx.x < 64) {
+    const long long v = a.prev ? a.prev[threadIdx.x] : 1;
+    const unsigned long long any = __ballot(v != 0);
+    if (threadIdx.x == 0) skip = any == 0 || *a.serr != 0 || *a.err != 0;
+  }
+  __syncthreads();
+  if (skip) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *a.which = a.sweep & 1;
+    a.stripes[2 * kStripes] = 1;
+  }
   int s = gtid();
   bool inb = s < a.n;
   double4 c = inb ? a.cur[s] : make_double4(0, 0, 0, 0);
@@ -821,12 +836,14 @@ __global__ __launch_bounds__(256) void k_loop1_simple(Loop1Args<double4> a, Fiel
 // the EOS (eos.f90:21-45) and the loop-2 j-side packs; flags[stripe] bit 1 =
 // some velocity is non-zero, bit 2 = some B differs from particle 0's.
 template <typename V4>
-__global__ __launch_bounds__(256) void k_post1(Fields F, const double4 *fin, const uint8_t *real, int n,
+__global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint8_t *real, int n,
                                                int advanced, int do_eos, int iso, double c1, double gamma, V4 *q0,
                                                V4 *q1, V4 *q2, V4 *q3, V4 *q4, double4 *pd, double *trd,
                                                long long *flags, long long *hmax_red) {
   __shared__ long long lds[4];
   using T = decltype(V4().x);
+  if (*sel.serr != 0 || *sel.err1 != 0) return;  // set before this launch: uniform
+  const double4 *fin = (sel.which && *sel.which) ? sel.fin1 : sel.fin0;
   int s = gtid();
   int fl = 0;
   if (s < n) {
@@ -1413,6 +1430,13 @@ __global__ void k_prerun_check(Fields F, int n, int *bits) {
   if (b) atomicOr(bits, b);
 }
 
+// count long longs set to 0 (a kernel rather than a memset, so that a captured
+// graph replays it).
+__global__ void k_zero_ll(long long *p, int count) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = 0;
+}
+
 // One int64 set on the device (a kernel, not a memset, so a captured graph
 // replays it).
 __global__ void k_set_i64(int64_t *p, int64_t v) {
@@ -1591,10 +1615,10 @@ SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
 SPH_INST_NEIGH(1)
 SPH_INST_NEIGH(2)
 SPH_INST_NEIGH(3)
-template __global__ void k_post1<double4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
+template __global__ void k_post1<double4>(Fields, PostSel, const uint8_t *, int, int, int, int, double,
                                           double, double4 *, double4 *, double4 *, double4 *, double4 *, double4 *,
                                           double *, long long *, long long *);
-template __global__ void k_post1<float4>(Fields, const double4 *, const uint8_t *, int, int, int, int, double,
+template __global__ void k_post1<float4>(Fields, PostSel, const uint8_t *, int, int, int, int, double,
                                          double, float4 *, float4 *, float4 *, float4 *, float4 *, double4 *,
                                          double *, long long *, long long *);
 

@@ -244,14 +244,17 @@ def test_load_invalidates_prepared_operator(gpu):
 
 
 def test_frozen_geometry_forward_vs_reference_sts_golden(gpu):
-    # sph_diff_forward freezes h, rho and Omega after one iterate; the
-    # reference's sts_integrate re-runs iterate at every stage
-    # (sts_integrator.f90:100-108), whose Newton iteration moves h again by
-    # up to its tolerance (1e-8 relative) before the stage's dT/dt.  On the
-    # reference's own STS step (diff2d_fab_sts3_16: 3 stages, reference
-    # tensor difcond diag(b^2)) the frozen operator's T must stay within
-    # 1e-7 of the change of T (measured below); eng.sts, which re-iterates
-    # like the reference, matches it to 1e-12 in test_sts_matches_reference_golden
+    # sph_diff_forward freezes h, rho, Omega and the neighbour lists after one
+    # iterate; the reference's sts_integrate re-runs iterate at every stage
+    # (sts_integrator.f90:100-108): its Newton iteration moves h by up to its
+    # tolerance again, and each stage searches with the h the previous stage
+    # left (the first search used the set-up's h), so pairs at the kernel's
+    # edge (q near 3, where the M6 Hessian falls off as (3-q)^3) join or
+    # leave the lists.  On the reference's own STS step (diff2d_fab_sts3_16:
+    # 3 stages, reference tensor difcond diag(b^2)) the frozen operator's T
+    # stays within 1e-5 of the change of T (measured 4.1e-6); eng.sts, which
+    # re-iterates like the reference, matches it to 1e-12 in
+    # test_sts_matches_reference_golden
     import torch
     from tests import goldens as G
     from sph_playground_amd import autograd as AG
@@ -277,4 +280,4 @@ def test_frozen_geometry_forward_vs_reference_sts_golden(gpu):
     change = np.abs(ref - st[:, C.es_t - 1]).max()
     dev = np.abs(T - ref).max() / change
     print("frozen-geometry forward vs reference STS: %.2e of the change of T" % dev, flush=True)
-    assert change > 0 and dev <= 1e-7
+    assert change > 0 and dev <= 1e-5
