@@ -31,7 +31,7 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, double4 *, double *, long long *, long long *, int);
+                       uint8_t *, double4 *, double *, double4 *, double *, long long *, long long *, int, int);
 template <int DIM>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -212,6 +212,12 @@ struct sph_ctx {
   uint64_t bbox_version = 0, hmax_version = 0;
   double last_lo[3] = {0, 0, 0}, last_hi[3] = {0, 0, 0}, next_hmax = 0.0;
   uint64_t keys_version = 0;      // pos_version the keys and cell starts belong to
+  // the position records k_prep last wrote (pm, real, pf): for which
+  // positions, buffer, count and grid
+  uint64_t prep_version = 0;
+  int prep_cur = -1;
+  int64_t prep_n = -1;
+  Grid prep_grid{};
   Grid keys_grid{};
   int64_t keys_n = -1, keys_nkeys = -1;
   // per-sweep counters of loop 1 (still-active stripes, pair stripes, a
@@ -395,9 +401,21 @@ static int ensure_cells(sph_ctx *c, int64_t nkeys) {
   return SPH_OK;
 }
 
+// A HIP call whose failure is tolerated (timings of events a failed step never
+// recorded, best-effort clean-up) still leaves the thread's last error set,
+// and hipcub reports that error as its own after its next launch: clear it.
+static void tolerate(hipError_t e) {
+  if (e != hipSuccess) (void)hipGetLastError();
+}
+
 static float elapsed(sph_ctx *c, hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  hipError_t e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (e != hipSuccess) {
+    tolerate(e);
+    return 0.f;
+  }
   return ms;
 }
 
@@ -667,9 +685,15 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   // the v / B flags are computed again only when v or B may have changed
   c->flags_pending = c->flags_version != c->vb_version;
   if (c->flags_pending) HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
+  const bool pos_kept = keys_kept && c->prep_version == c->pos_version && c->prep_cur == c->cur &&
+                        c->prep_n == n && same_grid(g, c->prep_grid);
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
-                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL);
+                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
+  c->prep_version = c->pos_version;
+  c->prep_cur = c->cur;
+  c->prep_n = n;
+  c->prep_grid = g;
   // 4. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -796,7 +820,7 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
   const int nreg = (last - first + 1) * kSweepRegion;
   hipLaunchKernelGGL(k_zero_ll, dim3((nreg + 255) / 256), dim3(256), 0, c->stream,
                      c->d_sweep + (size_t)first * kSweepRegion, nreg);
-  (void)hipEventRecord(c->ev[9], c->stream);
+  tolerate(hipEventRecord(c->ev[9], c->stream));
   for (int it = first; it <= last; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
@@ -811,7 +835,7 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
     else launch_loop1_k<3>(c, a, grad, false, F);
   }
-  (void)hipEventRecord(c->ev[10], c->stream);
+  tolerate(hipEventRecord(c->ev[10], c->stream));
 }
 
 template <int DIM, int KT, int MODE>
@@ -1074,9 +1098,9 @@ static int run_step(sph_ctx *c, const double *state) {
   }
   c->neigh_pending = false;
   float t01 = 0, t12 = 0, t23 = 0;
-  (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
-  (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+  tolerate(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  tolerate(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  tolerate(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
   c->stats.ms_neigh = t01;
   c->stats.ms_loop1 = t12;
   c->stats.ms_loop2 = t23;
@@ -1270,6 +1294,7 @@ int sph_ctx_destroy(sph_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  (void)hipGetLastError();  // clean-up failures are not the next call's
   return SPH_OK;
 }
 
@@ -2198,12 +2223,12 @@ int batch_autotune(sph_batch *b) {
     batch_super_step(b, 4, blk);  // warm-up
     float t = 1e30f;
     for (int r = 0; r < 3; r++) {
-      (void)hipEventRecord(e0, b->stream);
+      tolerate(hipEventRecord(e0, b->stream));
       batch_super_step(b, 4, blk);
-      (void)hipEventRecord(e1, b->stream);
-      (void)hipEventSynchronize(e1);
+      tolerate(hipEventRecord(e1, b->stream));
+      tolerate(hipEventSynchronize(e1));
       float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
+      tolerate(hipEventElapsedTime(&ms, e0, e1));
       t = std::min(t, ms);
     }
     b->tune_ms[k] = t;

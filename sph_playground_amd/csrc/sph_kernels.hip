@@ -265,20 +265,27 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *keys, int n,
 // to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
+// pos = 0: the position records (pm, real, pf, band codes) of the previous
+// prep still hold (same positions, order and grid) and are not rewritten.
 __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, int64_t n_real, Box b, Grid g,
                                          int dim, double4 *pm, double *h0, float4 *pf, uint8_t *real, double4 *st,
-                                         double *tr, double4 *jr, double *eps) {
-  double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
-  pm[s] = make_double4(x, y, z, F.f[F_M][s]);
+                                         double *tr, double4 *jr, double *eps, int pos) {
   double h = F.f[F_H][s];
   h0[s] = h;
   bool r = orig[s] < n_real;
-  real[s] = r ? 1 : 0;
   st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
+  eps[s] = r ? 1.0 : 0.0;
   // T den for grad T: FP64 in tr, or in the FP32 path's packed record alone
+  if (!pos) {
+    if (jr) jr[s].w = pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s]));
+    else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
+    return;
+  }
+  double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
+  pm[s] = make_double4(x, y, z, F.f[F_M][s]);
+  real[s] = r ? 1 : 0;
   if (jr) jr[s] = make_double4(x, y, z, pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s])));
   else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
-  eps[s] = r ? 1.0 : 0.0;
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
   if (ty == 1 || ty == 2 || ty == 21) {
@@ -307,7 +314,7 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
 // kernels that count into them run (they all follow k_prep on its stream).
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
                        double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double4 *jr, double *eps,
-                       long long *flags, long long *zero, int nzero) {
+                       long long *flags, long long *zero, int nzero, int pos) {
   __shared__ long long lds[4];
   int s = gtid();
   if (blockIdx.x == 0)
@@ -319,7 +326,7 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
     }
-    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, jr, eps);
+    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, jr, eps, pos);
   }
   if (flags) {
     long long t = block_reduce<R_OR>(fl, lds);
