@@ -450,8 +450,8 @@ def main():
     sweeps64 = avg(per64, "sweeps")
     p1 = avg(permx, "loop1_pairs") / max(sweeps, 1)
     p164 = avg(per64, "loop1_pairs") / max(sweeps64, 1)
-    names = {"mixed": ("k_neigh<3>", "k_loop1<3, 1, 1, float>", "k_loop2_fab<3, 1, float, HIP_vector_type<float, 4u> >"),
-             "fp64": ("k_neigh<3>", "k_loop1<3, 1, 1, double>",
+    names = {"mixed": ("k_neigh<3, true>", "k_loop1<3, 1, 1, float>", "k_loop2_fab<3, 1, float, HIP_vector_type<float, 4u> >"),
+             "fp64": ("k_neigh<3, true>", "k_loop1<3, 1, 1, double>",
                       "k_loop2_fab<3, 1, double, HIP_vector_type<double, 4u> >")}
     roofs = {}
     for path, km, sw, pp, peak in (("mixed", kmx, sweeps, p1, PEAK_FP32_TFLOPS),
@@ -467,7 +467,7 @@ def main():
     # traffic of a whole steady mixed step: every kernel of it in the profile
     # (search prep, search, Newton sweep, post, loop 2, close) against the
     # compulsory bytes
-    step_kernels = ("k_prep", "k_neigh<3>", "k_loop1<3, 1, 1, float>", "k_post1<HIP_vector_type<float, 4u> >",
+    step_kernels = ("k_prep", "k_neigh<3, true>", "k_loop1<3, 1, 1, float>", "k_post1<HIP_vector_type<float, 4u> >",
                     "k_loop2_fab<3, 1, float, HIP_vector_type<float, 4u> >", "k_step_close")
     tr = profile_table(TRAFFIC_FILE)
     step_measured = sum(float(tr[k][-1]) for k in step_kernels if k in tr) if args.size == 128 else None
@@ -477,7 +477,7 @@ def main():
     dom = max(("ms_k_neigh", "ms_k_loop1", "ms_k_loop2"), key=lambda k: kmx[k])
     if dom == "ms_k_neigh":
         r = roofs["mixed"]["k_neigh"]
-        roof = dict(kernel="k_neigh<3> (cell-list neighbour build)", bound="hbm",
+        roof = dict(kernel="k_neigh<3, true> (cell-list neighbour search, extended grid)", bound="hbm",
                     achieved=r["algorithmic_bytes"] / (r["ms_per_launch"] * 1e-3) / 1e9, peak=PEAK_HBM_GBS,
                     unit="GB/s", frac=r["algorithmic_hbm_frac"], traffic=r.get("measured_bytes"),
                     bytes_per_launch=r["algorithmic_bytes"], ms_per_launch=r["ms_per_launch"],

@@ -67,6 +67,8 @@ typedef struct {
   double ms_neigh, ms_loop1, ms_loop2, ms_total; /* device time of the last step's phases */
   double ms_k_neigh, ms_k_loop1, ms_k_loop2;     /* device time of the neighbour kernel, all
                                                     Newton sweep kernels, the loop-2 kernel */
+  int64_t ext_records;  /* records of the search's extended grid (real cells plus the periodic
+                           images in ghost cells); 0 when the search read its ranges per lane */
 } sph_stats;
 
 /* Fill *opt with defaults (FP64, reference tensor, 100 sweeps, 1e-8). */

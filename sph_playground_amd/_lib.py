@@ -36,7 +36,8 @@ class Stats(ct.Structure):
                 ("loop1_pairs", ct.c_int64), ("sweeps", ct.c_int), ("sorted", ct.c_int),
                 ("energy", ct.c_double), ("ms_neigh", ct.c_double), ("ms_loop1", ct.c_double),
                 ("ms_loop2", ct.c_double), ("ms_total", ct.c_double), ("ms_k_neigh", ct.c_double),
-                ("ms_k_loop1", ct.c_double), ("ms_k_loop2", ct.c_double)]
+                ("ms_k_loop1", ct.c_double), ("ms_k_loop2", ct.c_double),
+                ("ext_records", ct.c_int64)]
 
 
 class BatchInfo(ct.Structure):

@@ -32,7 +32,7 @@ __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
                        uint8_t *, double4 *, double *, double4 *, double *, long long *, long long *, int, int);
-template <int DIM>
+template <int DIM, bool EXT>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
 template <int DIM, int KT, typename PT, typename V4>
@@ -49,6 +49,8 @@ __global__ void k_loop2(Loop2Args<V4>);
 __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
 __global__ void k_step_close(const double *, int, const int *, const long long *, long long *);
 __global__ void k_soa_to_aos(Fields, const int64_t *, int, double *, int64_t);
+__global__ void k_ext_count(Grid, ExtGrid, const int *, int, int *);
+__global__ void k_ext_fill(Grid, ExtGrid, const int *, const int *, const float4 *, int, float4 *);
 __global__ void k_neigh_export(const uint32_t *, const int *, const int64_t *, const uint8_t *, int, int,
                                const int64_t *, int32_t *, int8_t *, int64_t);
 __global__ void k_set_i64(int64_t *, int64_t);
@@ -156,6 +158,22 @@ struct sph_ctx {
   hipStream_t stream = nullptr;
   int kt = 1;
   sph_options opt{};
+  // SPH_NEIGH_TRAVERSAL=lane at creation: the search takes the per-lane
+  // traversal and builds no extended grid (the lists are the same; tests
+  // compare the two)
+  bool lane_search = false;
+  // the search's extended grid (ghost cells of the imaged axes): cell counts,
+  // cell starts, records; valid for the positions, order and grid it was
+  // built from
+  int *ext_count = nullptr, *ecs = nullptr;
+  float4 *erec = nullptr;
+  void *ext_tmp = nullptr;
+  size_t ext_cells_cap = 0, ext_rec_cap = 0, ext_tmp_bytes = 0;
+  ExtGrid ext_g{};
+  uint64_t ext_version = 0;  // pos_version it was built from (0: none)
+  int64_t ext_total = 0;
+  int ext_cur = -1, ext_n = -1;
+  Grid ext_grid{};
   Phys frozen{};
   Box box{};
   int process = 0, ics = 0, do_eos = 1, eos_iso = 0, gravity = 0;
@@ -491,6 +509,7 @@ constexpr int kNeighLL = 1024;   // h_ll slots of the search's read-back ([0, kF
 constexpr int kSweepRegion = 2 * kStripes + 8;
 constexpr int kSweepRegions = 242;  // sweeps 1..240 (the cap on max_sweeps) and slack
 constexpr int kWhichInt = 6;        // d_int slot: parity of the last sweep that ran
+constexpr int kExtTotalInt = 16;    // h_int slot: records of the extended grid
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
 
 // the error words of the step (ints in d_stripes), and the same words in a
@@ -694,7 +713,77 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   c->prep_cur = c->cur;
   c->prep_n = n;
   c->prep_grid = g;
-  // 4. neighbour lists; grow the storage and redo if any list overflowed
+  // rows and cells within r_max (1 + 1e-6) of a cell; imaged axes at most
+  // one wrap away
+  int rc[3];
+  for (int d = 0; d < 3; d++) {
+    rc[d] = d < dim && cs[d] > 0 ? (int)std::ceil(rmax * (1.0 + 1e-6) / cs[d]) : 0;
+    if (g.periodic[d]) rc[d] = std::min(rc[d], g.nc[d]);
+  }
+  // 4. the extended grid: rc ghost cells on each side of every imaged axis
+  //    hold the periodic images, so every stencil row of the search is one
+  //    range of records; rebuilt when the positions, their order, the grid
+  //    or the stencil change
+  bool use_ext = !c->lane_search;
+  ExtGrid xg{};
+  int64_t ncells = 1;
+  for (int d = 0; d < 3; d++) {
+    xg.gh[d] = g.periodic[d] ? rc[d] : 0;
+    xg.ne[d] = g.nc[d] + 2 * xg.gh[d];
+    xg.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
+    ncells *= xg.ne[d];
+  }
+  // offsets are ints: every record has at most 27 copies
+  if (ncells + 1 > INT32_MAX || (int64_t)n * 27 > INT32_MAX) use_ext = false;
+  if (use_ext) {
+    const bool ext_kept = c->ext_version == c->pos_version && c->ext_cur == c->cur && c->ext_n == n &&
+                          same_grid(g, c->ext_grid) && std::memcmp(&xg, &c->ext_g, sizeof(xg)) == 0;
+    if (!ext_kept) {
+      const int E = (int)ncells;
+      if ((size_t)E + 1 > c->ext_cells_cap) {
+        if (c->ext_count) { (void)hipFree(c->ext_count); c->ext_count = nullptr; }
+        if (c->ecs) { (void)hipFree(c->ecs); c->ecs = nullptr; }
+        c->ext_cells_cap = 0;
+        RCHK(dev_alloc((void **)&c->ext_count, sizeof(int) * ((size_t)E + 1)));
+        RCHK(dev_alloc((void **)&c->ecs, sizeof(int) * ((size_t)E + 1)));
+        c->ext_cells_cap = (size_t)E + 1;
+      }
+      size_t tb = 0;
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c->ext_count, c->ecs, E + 1, c->stream));
+      if (tb > c->ext_tmp_bytes) {
+        if (c->ext_tmp) { (void)hipFree(c->ext_tmp); c->ext_tmp = nullptr; }
+        c->ext_tmp_bytes = 0;
+        RCHK(dev_alloc(&c->ext_tmp, tb));
+        c->ext_tmp_bytes = tb;
+      }
+      hipLaunchKernelGGL(k_ext_count, dim3(grid_for(E + 1)), dim3(kBlock), 0, c->stream, g, xg, c->cstart, E,
+                         c->ext_count);
+      tb = c->ext_tmp_bytes;
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->ext_tmp, tb, c->ext_count, c->ecs, E + 1, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_int + kExtTotalInt, c->ecs + E, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipEventRecord(c->ev[8], c->stream));
+      HIPCHK(wait_event(c->ev[8]));
+      const int64_t total = c->h_int[kExtTotalInt];
+      // 8 records of padding: the search reads whole groups of 8
+      if ((size_t)total + 8 > c->ext_rec_cap) {
+        if (c->erec) { (void)hipFree(c->erec); c->erec = nullptr; }
+        c->ext_rec_cap = 0;
+        const size_t cap = (size_t)total + 8 + (size_t)total / 8;
+        RCHK(dev_alloc((void **)&c->erec, sizeof(float4) * cap));
+        c->ext_rec_cap = cap;
+      }
+      hipLaunchKernelGGL(k_ext_fill, dim3(grid_for(E)), dim3(kBlock), 0, c->stream, g, xg, c->cstart, c->ecs,
+                         c->pf, E, c->erec);
+      c->ext_total = total;
+      c->ext_version = c->pos_version;
+      c->ext_cur = c->cur;
+      c->ext_n = n;
+      c->ext_grid = g;
+      c->ext_g = xg;
+    }
+  }
+  c->stats.ext_records = use_ext ? c->ext_total : 0;
+  // 5. neighbour lists; grow the storage and redo if any list overflowed
   if (c->nb_cap == 0) RCHK(ensure_nb(c, std::min(c->maxneib, dim == 3 ? 160 : (dim == 2 ? 48 : 16))));
   for (int attempt = 0; attempt < 2; attempt++) {
     if (attempt > 0) {  // k_prep cleared the first search's counters
@@ -703,14 +792,10 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     }
     NeighArgs a;
     a.hs = c->h0; a.real = c->real; a.cstart = c->cstart;
-    // rows and cells within r_max (1 + 1e-6) of a cell; imaged axes at most
-    // one wrap away
-    int rc[3];
-    for (int d = 0; d < 3; d++) {
-      rc[d] = d < dim && cs[d] > 0 ? (int)std::ceil(rmax * (1.0 + 1e-6) / cs[d]) : 0;
-      if (g.periodic[d]) rc[d] = std::min(rc[d], g.nc[d]);
-    }
     a.rx = rc[0]; a.ry = rc[1]; a.rz = rc[2];
+    a.ecs = use_ext ? c->ecs : nullptr;
+    a.erec = use_ext ? c->erec : nullptr;
+    for (int d = 0; d < 3; d++) { a.ne[d] = xg.ne[d]; a.gh[d] = xg.gh[d]; }
     // a position in cell units is FP32: its rounding stays below 2^-23 of the
     // cell count along the axis
     for (int d = 0; d < 3; d++) a.mcell[d] = (float)(1e-4 + 1e-6 * g.nc[d]);
@@ -727,10 +812,12 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     a.maxneib = c->maxneib; a.err = step_err(c) + kErrSearch; a.stripes = c->d_stripes + kSearchLL;
     // the search's own events (loop 1 reuses 5 and 6 before a deferred read-back is taken)
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
-#define NEIGH(D) hipLaunchKernelGGL((k_neigh<D>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
-    if (dim == 1) NEIGH(1);
-    else if (dim == 2) NEIGH(2);
-    else NEIGH(3);
+#define NEIGH(D)                                                                                              \
+  if (use_ext) hipLaunchKernelGGL((k_neigh<D, true>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt); \
+  else hipLaunchKernelGGL((k_neigh<D, false>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
+    if (dim == 1) { NEIGH(1); }
+    else if (dim == 2) { NEIGH(2); }
+    else { NEIGH(3); }
 #undef NEIGH
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // pair counts, list lengths, error words and k_prep's flags in one read-back
@@ -1191,6 +1278,10 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   sph_ctx *c = new sph_ctx();
   if (opt) c->opt = *opt;
   else sph_default_options(&c->opt);
+  {
+    const char *t = std::getenv("SPH_NEIGH_TRAVERSAL");
+    c->lane_search = t && std::strcmp(t, "lane") == 0;
+  }
   if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1 ||
       c->opt.cell_div < 0 || c->opt.cell_div > 4) {
     delete c;
@@ -1283,7 +1374,8 @@ int sph_ctx_destroy(sph_ctx *c) {
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,
                   c->dvec, c->gpart, c->ckpt, c->rcnt, c->dpxh, c->dirho, c->dW, c->ddW, c->dR, c->dS, c->dZ, c->dLZ[0], c->dLZ[1], c->dLZ[2], c->dLZ[3],
-                  c->gkdev, c->nlF, c->nlU, c->nlUB, c->nlNu, c->nlMu, c->nlD, c->nlDD, c->nlR, c->nlSD};
+                  c->gkdev, c->nlF, c->nlU, c->nlUB, c->nlNu, c->nlMu, c->nlD, c->nlDD, c->nlR, c->nlSD,
+                  c->ext_count, c->ecs, c->erec, c->ext_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);

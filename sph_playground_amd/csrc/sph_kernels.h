@@ -62,6 +62,19 @@ struct NeighArgs {
   int n, npad, cap, maxneib;
   int *err;                // error bits
   long long *stripes;      // [kStripes] pair counts, [kStripes..2k) max counts
+  // extended grid: the real cells plus gh[a] ghost cells on each side of every
+  // imaged axis (ne[a] = nc[a] + 2 gh[a] cells), ecs the first record of every
+  // extended cell (row-major, x fastest; ne0 ne1 ne2 + 1 entries) and erec the
+  // records (the per-lane traversal reads pf and cstart instead)
+  const int *ecs;
+  const float4 *erec;
+  int ne[3], gh[3];
+};
+
+// The extended grid's geometry for its build kernels.
+struct ExtGrid {
+  int ne[3], gh[3];
+  float lenf[3];  // box lengths in FP32 (0 on axes that are not imaged)
 };
 
 // Neighbour iteration with batched gathers: entries come 8 per coalesced

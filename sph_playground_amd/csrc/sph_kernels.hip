@@ -142,7 +142,8 @@ __device__ __forceinline__ void cell_of(const Grid &g, double x, double y, doubl
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     double t = floor((p[a] - g.lo[a]) * g.inv_cs[a]);
-    c[a] = (t < 0.0) ? 0 : (t >= (double)g.nc[a] ? g.nc[a] - 1 : (int)t);
+    // clamped to the grid; a NaN coordinate takes cell 0
+    c[a] = !(t >= 0.0) ? 0 : (t >= (double)g.nc[a] ? g.nc[a] - 1 : (int)t);
   }
 }
 
@@ -370,81 +371,114 @@ __device__ __forceinline__ double kd_sqdist(double xj, double yj, double zj, dou
 // findneighboursKDT (neighboursearch.f90:74-194) as a cell-list search:
 // particles are sorted so that the cells of one (y, z) row lie contiguously
 // along x.  Every real i visits the rows within r_max in y and z (rows wrap
-// with an image shift along imaged axes; rows whose box lies farther than its
-// radius are skipped) and, in each, the one contiguous range of records of
-// the cells its sphere reaches along x at that row's distance (one more
-// range per wrapped part), and tests them with the kd-tree's FP64 inclusion
-// rule sd <= (krad*h_i)^2; self (unshifted) is dropped.
+// with an image shift along imaged axes) and, in each, the contiguous range
+// of records of the cells its sphere can reach along x at that row's distance
+// (wrapped parts carry their shift), and tests them with the kd-tree's FP64
+// inclusion rule sd <= (krad*h_i)^2; self (unshifted) is dropped.
 //
-// Candidates arrive as one 16-byte record each (pf: FP32 coordinates
-// relative to the grid origin, and the band bits), four at a time.  The FP32
-// squared distance decides every candidate farther than the error band
-// around the radius: |sd32 - |d|^2| <= 2 sqrt3 r delta + 3 delta^2 + 2^-20 sd
-// with delta = a.delta (the FP32 coordinate error bound, 2^-21 x the grid
-// span).  Candidates inside the band, and any lane whose radius is not a
-// finite number, take the exact FP64 rule on pm, so the sets are those of
-// the FP64 test.
+// Candidates are 16-byte records (pf: FP32 coordinates relative to the grid
+// origin, and the band bits).  The FP32 squared distance decides every
+// candidate farther than the error band around the radius:
+// |sd32 - |d|^2| <= 2 sqrt3 r delta + 3 delta^2 + 2^-20 sd with delta =
+// a.delta (the FP32 coordinate error bound, 2^-21 x the grid span).
+// Candidates inside the band, and any lane whose radius is not a finite
+// number, take the exact FP64 rule on pm, so the sets are those of the FP64
+// test.
+//
+// Two traversals give the same lists, entry for entry:
+// - extended grid (the default): the periodic images live in ghost cells
+//   around the real grid (built by k_ext_count / k_ext_fill whenever the
+//   particles or the grid change), so each stencil row is one contiguous
+//   range of records along x whose coordinates already carry the image shift,
+//   a missing image is a NaN x and the list entry is the record's fourth
+//   word.  Pre-shifting rounds q + s L once where the per-lane traversal
+//   rounds p - s L: the same count of 2^-24 S roundings, so the FP32 bound
+//   holds.
+// - per lane (SPH_NEIGH_TRAVERSAL=lane, or no extended grid): each lane splits
+//   a row's x range into the part left of the box, inside it and right of it,
+//   and applies the shift and the image test to every record it reads.
 // pending list entries per lane in k_neigh (a power of two, whole chunks)
 constexpr int kNbRing = 16;
-static_assert(kNbRing == 16, "ring_slot masks a 64-byte ring");
-
-// byte offset of ring slot (c4 / 4) mod 16 in a lane's 64-byte-aligned ring
-// at ring_off: (c4 & 60) | ring_off in one instruction (the compiler splits
-// the same expression into an and and an add)
-__device__ __forceinline__ uint32_t ring_slot(int c4, uint32_t ring_off) {
+// records the extended-grid scan reads and tests together
+#ifndef SPH_SCAN_BATCH
+#define SPH_SCAN_BATCH 4
+#endif
+constexpr int kScanBatch = SPH_SCAN_BATCH;
+static_assert(kScanBatch == 4 || kScanBatch == 8, "the ring holds 7 pending entries plus one batch");
+// The rings of a block's 256 lanes are interleaved slot by slot: slot k of
+// lane l is word 256 k + l, so when the 64 lanes of a wavefront write their
+// next slots they touch every bank once per 32 lanes, whatever slot each is
+// at (rings of one lane each, 64 bytes apart, put up to 32 lanes on a bank).
+// A lane's cursor counts its entries in slot rows of 1024 bytes: the next
+// slot's byte offset is (cursor & 0x3C00) | 4 threadIdx.x.
+constexpr int kRingRow = 1024;  // bytes per slot row: 256 lanes x 4
+constexpr int kRingShift = 10;
+static_assert(kNbRing * kRingRow == 0x4000 && kRingRow == 256 * 4, "ring_slot masks 16 slot rows");
+__device__ __forceinline__ uint32_t ring_slot(int cur, uint32_t lane_off) {
   uint32_t r;
-  asm("v_and_or_b32 %0, %1, 60, %2" : "=v"(r) : "v"(c4), "v"(ring_off));
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(cur), "s"(0x3C00), "v"(lane_off));
   return r;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
+#ifdef SPH_NEIGH_WAVES
+#define SPH_NEIGH_OCC __attribute__((amdgpu_waves_per_eu(SPH_NEIGH_WAVES)))
+#else
+#define SPH_NEIGH_OCC
+#endif
+template <int DIM, bool EXT>
+__global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const double4 *__restrict__ pm,
                                                const float4 *__restrict__ pf, uint32_t *__restrict__ nb,
                                                int *__restrict__ cnt) {
   __shared__ long long lds[4];
   __shared__ __align__(16) uint32_t nb_ring[256 * kNbRing];
-  int s = gtid();
-  int count = 0;
-  bool live = s < a.n && a.real[s];
-  if (live) {
-    const double4 pi = pm[s];
-    const float4 pif = pf[s];
-    const double rr = a.krad * a.hs[s];
-    const double r2 = rr * rr;
-    const double r2m = r2 * (1.0 + 1e-6);  // pruning bound, never tighter than the exact test
-    // FP32 decision thresholds; a radius that is not a finite number sends
-    // every candidate to the exact test
-    const double mabs = 2.0 * 1.7320508075688772 * 1.01 * rr * a.delta + 3.0 * a.delta * a.delta;
-    const bool finite_r = rr == rr && rr < HUGE_VAL;
-    const float hi_thr = finite_r ? (float)(r2 * (1.0 + 1e-5) + mabs) : HUGE_VALF;
-    const float lo_thr = finite_r ? (float)(r2 * (1.0 - 1e-5) - mabs) : -HUGE_VALF;
-    // entries [done, count) wait in the lane's ring and leave as whole
-    // 32-byte chunks: after each stencil row (the same point for every
-    // lane, so the lanes' chunk stores share instructions), when the ring
-    // is full, and the last partial chunk at the end
-    uint32_t *ring = nb_ring + threadIdx.x * kNbRing;
-    // the ring position in bytes: c4 = 4 x entries found, so a slot's LDS
-    // offset is the lane's ring offset ORed with c4's low bits
-    const uint32_t ring_off = (uint32_t)threadIdx.x * (kNbRing * 4);
-    char *const ring_bytes = reinterpret_cast<char *>(nb_ring);
-    int c4 = 0;
-    int done = 0;
-    auto flush_chunk = [&]() {
-      if (done < a.cap) {  // cap is whole chunks: the chunk fits
-        const uint4 *src = reinterpret_cast<const uint4 *>(ring + (done & (kNbRing - 1)));
-        uint4 *dst = reinterpret_cast<uint4 *>(nb + nb_slot(done, s, a.npad));
-        const uint4 lo = src[0], hi = src[1];
-        dst[0] = lo;
-        dst[1] = hi;
-      }
-      done += 8;
-    };
+  const int s = gtid();
+  const bool live = s < a.n && a.real[s];
+  const int sl = s < a.n ? s : 0;  // lanes past n read particle 0 and search nothing
+  const double4 pi = pm[sl];
+  const float4 pif = pf[sl];
+  const double rr = a.krad * a.hs[sl];
+  const double r2 = rr * rr;
+  const double r2m = r2 * (1.0 + 1e-6);  // pruning bound, never tighter than the exact test
+  // FP32 decision thresholds; a radius that is not a finite number sends
+  // every candidate to the exact test
+  const double mabs = 2.0 * 1.7320508075688772 * 1.01 * rr * a.delta + 3.0 * a.delta * a.delta;
+  const bool finite_r = rr == rr && rr < HUGE_VAL;
+  const float hi_thr = finite_r ? (float)(r2 * (1.0 + 1e-5) + mabs) : HUGE_VALF;
+  const float lo_thr = finite_r ? (float)(r2 * (1.0 - 1e-5) - mabs) : -HUGE_VALF;
+  // squared radius of the row culling (grown by 2e-5)
+  const float r2g = finite_r ? (float)r2m * 1.00002f : HUGE_VALF;
+  // entries [done, count) wait in the lane's ring and leave as whole 32-byte
+  // chunks (see the traversals for when); cur = count x kRingRow
+  const uint32_t lane_off = (uint32_t)threadIdx.x * 4;
+  char *const ring_bytes = reinterpret_cast<char *>(nb_ring);
+  int cur = 0;
+  int done = 0;
+  auto flush_chunk = [&]() {
+    if (done < a.cap) {  // cap is whole chunks: the chunk fits
+      // slots (done mod 16) .. + 7 of this lane, 256 words apart
+      const uint32_t *src = nb_ring + (done & (kNbRing - 1)) * 256 + threadIdx.x;
+      uint4 *dst = reinterpret_cast<uint4 *>(nb + nb_slot(done, s, a.npad));
+      const uint4 lo = make_uint4(src[0], src[256], src[512], src[768]);
+      const uint4 hi = make_uint4(src[1024], src[1280], src[1536], src[1792]);
+      dst[0] = lo;
+      dst[1] = hi;
+    }
+    done += 8;
+  };
+  int ci[3];
+  cell_of(a.g, pi.x, pi.y, pi.z, ci);
+  const int nx = a.g.nc[0];
+
+  // ------------------------------------------------ per-lane traversal
+  auto lane_rows = [&]() {
     // every candidate of one contiguous record range [t0, t1) of one image
     // shift: FP32 decision, the exact FP64 rule inside the band.  IMG is
     // false for the unshifted records, which always exist and never hold the
     // lane's own record (the caller splits the range around it); IMG true
     // tests the records' band bits for the image's existence.  One code path
-    // for full and partial batches keeps the wave's lanes together.
+    // for full and partial batches keeps the wave's lanes together.  Entries
+    // leave the ring after each stencil row (the same point for every lane,
+    // so the lanes' chunk stores share instructions) and when it is full.
     auto scan = [&](auto img, int t0, int t1, int sx, int sy, int sz) {
       constexpr bool IMG = decltype(img)::value;
       const uint32_t code = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
@@ -483,19 +517,16 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
           // every lane writes the next free ring slot and keeps it on a hit
           // (at most kNbRing - 1 entries are pending here, so that slot
           // holds none of them)
-          *reinterpret_cast<uint32_t *>(ring_bytes + ring_slot(c4, ring_off)) =
+          *reinterpret_cast<uint32_t *>(ring_bytes + ring_slot(cur, lane_off)) =
               e0 + ((uint32_t)u << kCodeBits);
-          c4 += hit ? 4 : 0;
+          cur += hit ? kRingRow : 0;
         }
         // at most 11 pending before the next batch's 4 candidates
-        if ((c4 >> 2) - done > kNbRing - 5) flush_chunk();
+        if ((cur >> kRingShift) - done > kNbRing - 5) flush_chunk();
       }
     };
     using Img = std::true_type;
     using Own = std::false_type;
-    int ci[3];
-    cell_of(a.g, pi.x, pi.y, pi.z, ci);
-    const int nx = a.g.nc[0];
     // row geometry in FP32 and in cell units, widened by margins that only
     // add rows and cells: positions shrink towards the row by mcell, the
     // squared radius grows by 2e-5
@@ -503,7 +534,6 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
     const float pcy = (float)((pi.y - a.g.lo[1]) * a.g.inv_cs[1]);
     const float pcz = (float)((pi.z - a.g.lo[2]) * a.g.inv_cs[2]);
     const float csy = (float)a.g.cs[1], csz = (float)a.g.cs[2], icx = (float)a.g.inv_cs[0];
-    const float r2g = finite_r ? (float)r2m * 1.00002f : HUGE_VALF;
     const int rz = DIM == 3 ? a.rz : 0, ry = DIM >= 2 ? a.ry : 0;
     for (int dz = -rz; dz <= rz; dz++) {
       // the rows (cy, cz) with their image shifts; rows outside a non-imaged
@@ -592,10 +622,136 @@ __global__ __launch_bounds__(256) void k_neigh(NeighArgs a, const double4 *__res
             scan(Img{}, w0, w1, 1, sy, sz);
           }
         }
-        while ((c4 >> 2) - done >= 8) flush_chunk();
+        while ((cur >> kRingShift) - done >= 8) flush_chunk();
       }
     }
-    count = c4 >> 2;
+  };
+
+  // ------------------------------------------------ extended-grid traversal
+  auto ext_rows = [&]() {
+    const uint32_t own_e = nb_pack((uint32_t)s, kNoImage);
+    // every record of [t0, t1), eight at a time (a row's records, usually
+    // fewer, are read together; erec holds 8 records of padding past its
+    // end): FP32 decisions first, the exact FP64 rule for those inside the
+    // band behind one branch, then the ring writes.  OWN (the lane's own
+    // row) drops its own record.  No branch per candidate.
+    auto scan = [&](auto own, int t0, int t1) {
+      constexpr bool OWN = decltype(own)::value;
+      for (int t = t0; t < t1; t += kScanBatch) {
+        const int rem = t1 - t;
+        const float4 *p = a.erec + t;
+        float4 q[kScanBatch];
+#pragma unroll
+        for (int u = 0; u < kScanBatch; u++) q[u] = p[u];
+        bool hit[kScanBatch], band[kScanBatch];
+        bool any_band = false;
+#pragma unroll
+        for (int u = 0; u < kScanBatch; u++) {
+          const float fx = q[u].x - pif.x, fy = q[u].y - pif.y, fz = q[u].z - pif.z;
+          const float sd32 = fx * fx + fy * fy + fz * fz;
+          bool ok = u < rem;
+          if (OWN) ok = ok && __float_as_uint(q[u].w) != own_e;
+          const bool in = sd32 < lo_thr;
+          hit[u] = ok && in;
+          band[u] = ok && !in && sd32 <= hi_thr;
+          any_band = any_band || band[u];
+        }
+        if (any_band) {
+#pragma unroll
+          for (int u = 0; u < kScanBatch; u++) {
+            if (band[u]) {
+              const uint32_t e = __float_as_uint(q[u].w);
+              const double4 pj = pm[nb_index(e)];
+              double xj = pj.x, yj = pj.y, zj = pj.z;
+              const uint32_t code = nb_code(e);
+              if (code != kNoImage) image_shift(xj, yj, zj, code, a.b);
+              hit[u] = kd_sqdist(xj, yj, zj, pi.x, pi.y, pi.z) <= r2;
+            }
+          }
+        }
+        // every lane writes the next free ring slot and keeps it on a hit
+        // (at most 7 + u entries are pending here: the slot is free)
+#pragma unroll
+        for (int u = 0; u < kScanBatch; u++) {
+          *reinterpret_cast<uint32_t *>(ring_bytes + ring_slot(cur, lane_off)) = __float_as_uint(q[u].w);
+          cur += hit[u] ? kRingRow : 0;
+        }
+        if ((cur >> kRingShift) - done >= 8) flush_chunk();  // at most 7 pending after it
+      }
+    };
+    // row geometry as in the per-lane traversal: FP32 cell units, margins
+    // that only add rows and cells
+    const float pcx = (float)((pi.x - a.g.lo[0]) * a.g.inv_cs[0]);
+    const float pcy = (float)((pi.y - a.g.lo[1]) * a.g.inv_cs[1]);
+    const float pcz = (float)((pi.z - a.g.lo[2]) * a.g.inv_cs[2]);
+    const float csy = (float)a.g.cs[1], csz = (float)a.g.cs[2], icx = (float)a.g.inv_cs[0];
+    const int rz = DIM == 3 ? a.rz : 0, ry = DIM >= 2 ? a.ry : 0;
+    const bool perx = a.g.periodic[0] != 0;
+    // the record range of stencil row (dy, dz): ghost rows hold the wrapped
+    // images, rows beyond a non-imaged axis or farther than the radius are
+    // empty
+    auto row_range = [&](int dy, int dz, int &t0, int &t1) {
+      const int cz = ci[2] + dz, cy = ci[1] + dy;
+      bool use = true;
+      float gyz = 0.f;
+      if (DIM == 3) {
+        const float t = (float)cz;
+        const float gz = fmaxf(fmaxf(t - pcz, pcz - (t + 1.f)) - a.mcell[2], 0.f) * csz;
+        gyz = gz * gz;
+        if (!a.g.periodic[2]) use = cz >= 0 && cz < a.g.nc[2];
+      }
+      if (DIM >= 2) {
+        const float t = (float)cy;
+        const float gy = fmaxf(fmaxf(t - pcy, pcy - (t + 1.f)) - a.mcell[1], 0.f) * csy;
+        gyz += gy * gy;
+        if (!a.g.periodic[1]) use = use && cy >= 0 && cy < a.g.nc[1];
+      }
+      use = use && gyz <= r2g;
+      t0 = t1 = 0;
+      if (use) {
+        const float hxc =
+            fminf(__builtin_amdgcn_sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], (float)(a.rx + 1));
+        int xl = max((int)floorf(pcx - hxc), ci[0] - a.rx);
+        int xh = min((int)floorf(pcx + hxc), ci[0] + a.rx);
+        if (!perx) {
+          xl = max(xl, 0);
+          xh = min(xh, nx - 1);
+        }
+        if (xl <= xh) {
+          // cells [xl, xh] of the row: one range of records (a.rx ghost
+          // cells on each side of an imaged x axis)
+          const int base = ((cz + a.gh[2]) * a.ne[1] + cy + a.gh[1]) * a.ne[0] + a.gh[0];
+          t0 = a.ecs[base + xl];
+          t1 = a.ecs[base + xh + 1];
+        }
+      }
+    };
+    // rows in (dz, dy) order; the next row's range is read while this row's
+    // records are tested
+    int dy = -ry, dz = -rz, t0n, t1n;
+    row_range(dy, dz, t0n, t1n);
+    for (;;) {
+      const int t0 = t0n, t1 = t1n;
+      const bool own_row = dy == 0 && dz == 0;
+      const bool last = dy == ry && dz == rz;
+      if (!last) {
+        if (++dy > ry) { dy = -ry; ++dz; }
+        row_range(dy, dz, t0n, t1n);
+      }
+      if (own_row) scan(std::true_type{}, t0, t1);
+      else scan(std::false_type{}, t0, t1);
+      while ((cur >> kRingShift) - done >= 8) flush_chunk();
+      if (last) break;
+    }
+  };
+
+  if (live) {
+    if constexpr (EXT) ext_rows();
+    else lane_rows();
+  }
+  int count = 0;
+  if (live) {
+    count = cur >> kRingShift;
     if (count > done) flush_chunk();  // slots past count in this chunk are never read
     cnt[s] = min(count, a.cap);  // an overflowed list is searched again; what reads it first stays in bounds
     if (count + 1 > a.maxneib) atomicOr(a.err, EB_TRUNC);
@@ -1408,6 +1564,63 @@ __global__ void k_soa_to_aos(Fields F, const int64_t *orig, int n, double *aos, 
 
 // Neighbour lists back in original numbering (for sph_neighbours); entries
 // at or past `cap` (the output's length) are not written.
+// ---------------------------------------------------------------- extended grid
+// Extended cell k = (ez * ne1 + ey) * ne0 + ex is real cell (ex - gh0, ey - gh1,
+// ez - gh2) taken modulo the cell counts, with the box shift of the wrap.
+__device__ __forceinline__ int ext_axis(int e, int gh, int nc, int &shift) {
+  const int c = e - gh;
+  shift = c < 0 ? -1 : (c >= nc ? 1 : 0);
+  return c - shift * nc;
+}
+
+__device__ __forceinline__ uint32_t ext_source(const Grid &g, const ExtGrid &x, int k, int sh[3]) {
+  const int ex = k % x.ne[0], r = k / x.ne[0];
+  const int ey = r % x.ne[1], ez = r / x.ne[1];
+  const int cx = ext_axis(ex, x.gh[0], g.nc[0], sh[0]);
+  const int cy = ext_axis(ey, x.gh[1], g.nc[1], sh[1]);
+  const int cz = ext_axis(ez, x.gh[2], g.nc[2], sh[2]);
+  return row_of(g, cy, cz) * (uint32_t)g.nc[0] + (uint32_t)cx;
+}
+
+// records per extended cell (count[ncells] = 0 closes the exclusive scan)
+__global__ void k_ext_count(Grid g, ExtGrid x, const int *cstart, int ncells, int *count) {
+  const int k = gtid();
+  if (k > ncells) return;
+  if (k == ncells) {
+    count[k] = 0;
+    return;
+  }
+  int sh[3];
+  const uint32_t key = ext_source(g, x, k, sh);
+  count[k] = cstart[key + 1] - cstart[key];
+}
+
+// Copy every extended cell's records: the source records with the image
+// shift added to the FP32 coordinates, a NaN x where the reference holds no
+// such image (the band test of createPeriodicBorder, BC.f90:342-355), and the
+// list entry (sorted index << 5 | image code) in the fourth word.
+__global__ void k_ext_fill(Grid g, ExtGrid x, const int *cstart, const int *ecs, const float4 *pf, int ncells,
+                           float4 *erec) {
+  const int k = gtid();
+  if (k >= ncells) return;
+  int sh[3];
+  const uint32_t key = ext_source(g, x, k, sh);
+  uint32_t bm = 0, bv = 0;
+  image_mask(sh[0], sh[1], sh[2], bm, bv);
+  const uint32_t code = (uint32_t)((sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1));
+  const float dx = (float)sh[0] * x.lenf[0], dy = (float)sh[1] * x.lenf[1], dz = (float)sh[2] * x.lenf[2];
+  const int t0 = cstart[key], t1 = cstart[key + 1], o = ecs[k];
+  for (int t = t0; t < t1; t++) {
+    float4 q = pf[t];
+    const bool there = (__float_as_uint(q.w) & bm) == bv;
+    q.x = there ? q.x + dx : __int_as_float(0x7fc00000);
+    q.y = q.y + dy;
+    q.z = q.z + dz;
+    q.w = __uint_as_float(nb_pack((uint32_t)t, code));
+    erec[o + (t - t0)] = q;
+  }
+}
+
 __global__ void k_neigh_export(const uint32_t *nb, const int *cnt, const int64_t *orig, const uint8_t *real,
                                int n, int npad, const int64_t *row_off_by_orig, int32_t *jout, int8_t *cout,
                                int64_t cap) {
@@ -1616,9 +1829,11 @@ SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
 SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
 
-#define SPH_INST_NEIGH(DIM)                                                                      \
-  template __global__ void k_neigh<DIM>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
-                                        uint32_t *__restrict__, int *__restrict__);
+#define SPH_INST_NEIGH(DIM)                                                                              \
+  template __global__ void k_neigh<DIM, true>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
+                                              uint32_t *__restrict__, int *__restrict__);                       \
+  template __global__ void k_neigh<DIM, false>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
+                                               uint32_t *__restrict__, int *__restrict__);
 SPH_INST_NEIGH(1)
 SPH_INST_NEIGH(2)
 SPH_INST_NEIGH(3)
