@@ -31,7 +31,8 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, double4 *, double *, long long *, long long *, int, int);
+                       uint8_t *, double4 *, double *, float4 *, float2 *, double *, long long *, long long *, int,
+                       int);
 template <int DIM, bool EXT>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -43,7 +44,7 @@ template <int DIM, int KT, int GRAD, typename PT>
 __global__ void k_loop1_simple(Loop1Args<double4>, Fields);
 template <typename V4>
 __global__ void k_post1(Fields, PostSel, const uint8_t *, int, int, int, int, double, double, V4 *, V4 *,
-                        V4 *, V4 *, V4 *, double4 *, double *, long long *, long long *);
+                        V4 *, V4 *, V4 *, double4 *, double *, float2 *, long long *, long long *);
 template <int DIM, int KT, typename PT, typename V4, int MODE>
 __global__ void k_loop2(Loop2Args<V4>);
 __global__ void k_aos_to_soa(const double *, int64_t, int, Fields, int64_t *);
@@ -191,7 +192,11 @@ struct sph_ctx {
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
   double *tr[2] = {nullptr, nullptr};  // T den of the loop-1 sweep state st[k]
-  double4 *jr[2] = {nullptr, nullptr};  // mixed path: (x, y, z, [m, T den] FP32) of sweep state st[k]
+  // mixed path: position records (FP32 offsets in the search cell, the
+  // cell), (m, T den) of sweep state st[k] and (m/den, T den) for loop 2
+  float4 *pcell = nullptr;
+  float2 *s1[2] = {nullptr, nullptr};
+  float2 *s2 = nullptr;
   double4 *pd = nullptr;                // loop 2, diffusion mode: x, y, z, m/den
   double *trd = nullptr;                // loop 2, diffusion mode: T den
   double *h0 = nullptr, *eps = nullptr;
@@ -242,6 +247,7 @@ struct sph_ctx {
   // ran flag), one region per sweep of a step, and their pinned mirror
   long long *d_sweep = nullptr, *h_sweep = nullptr;
   int sweep_hint = 4;             // sweeps the next step launches before it synchronises
+  int last_sweeps = -1;           // Newton sweeps of the previous step
   sph_stats stats{};
   bool have_step = false;
   // the resident store holds the derivatives of the last successful iterate
@@ -371,7 +377,11 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
     RCHK(dalloc(&c->orig[b], npad));
     RCHK(dalloc(&c->st[b], npad));
     RCHK(dalloc(&c->tr[b], npad));
-    if (c->opt.precision == 1) RCHK(dalloc(&c->jr[b], npad));
+    if (c->opt.precision == 1) RCHK(dalloc(&c->s1[b], npad));
+  }
+  if (c->opt.precision == 1) {
+    RCHK(dalloc(&c->pcell, npad));
+    RCHK(dalloc(&c->s2, npad));
   }
   RCHK(dalloc(&c->pd, npad));
   RCHK(dalloc(&c->trd, npad));
@@ -491,6 +501,16 @@ constexpr int kWrapFlag = 4;
 // and their pinned h_ll slots
 constexpr int kCloseRed = 9 * kStripes;
 constexpr int kCloseLL = 1536;
+
+// the mixed path's position-record frame of a search grid
+static CellFrame cell_frame(const Grid &g) {
+  CellFrame f{};
+  for (int d = 0; d < 3; d++) {
+    f.cs[d] = (float)g.cs[d];
+    f.nc[d] = g.nc[d];
+  }
+  return f;
+}
 
 static bool same_grid(const Grid &a, const Grid &b) {
   for (int d = 0; d < 3; d++)
@@ -707,7 +727,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   const bool pos_kept = keys_kept && c->prep_version == c->pos_version && c->prep_cur == c->cur &&
                         c->prep_n == n && same_grid(g, c->prep_grid);
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->jr[0], c->eps,
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->pcell, c->s1[0], c->eps,
                      c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
   c->prep_version = c->pos_version;
   c->prep_cur = c->cur;
@@ -913,8 +933,10 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
     a.next = c->st[it & 1];
     a.trc = c->tr[(it - 1) & 1];
     a.trn = c->tr[it & 1];
-    a.jrc = c->jr[(it - 1) & 1];
-    a.jrn = c->jr[it & 1];
+    a.s1c = c->s1[(it - 1) & 1];
+    a.s1n = c->s1[it & 1];
+    a.pcell = c->pcell;
+    a.cf = cell_frame(c->grid);
     a.stripes = c->d_sweep + (size_t)it * kSweepRegion;
     a.prev = it > 1 ? c->d_sweep + (size_t)(it - 1) * kSweepRegion : nullptr;
     a.sweep = it;
@@ -998,8 +1020,10 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   sel.serr = step_err(c) + kErrSearch;
   sel.err1 = step_err(c) + kErrLoop1;
   hipLaunchKernelGGL(k_post1<V4>, dim3(G), dim3(kBlock), 0, c->stream, F, sel, c->real, n, P.adden ? 1 : 0,
-                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4, generic ? nullptr : c->pd,
-                     generic ? nullptr : c->trd, none, c->d_stripes + kHmaxLL);
+                     c->do_eos, c->eos_iso, c1, P.gamma, q0, q1, q2, q3, q4,
+                     generic || std::is_same_v<V4, float4> ? nullptr : c->pd,
+                     generic ? nullptr : c->trd, generic || !std::is_same_v<V4, float4> ? nullptr : c->s2, none,
+                     c->d_stripes + kHmaxLL);
   if (P.diff && P.fld && P.diso != 1) return SPH_ERR_FLD_ANISO;
   if (P.diff && !(P.ddw >= 400 && P.ddw <= 404)) return SPH_ERR_BAD_DDW;
   Loop2Args<V4> a{};
@@ -1007,6 +1031,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
   a.pd = generic ? nullptr : c->pd;
   a.trd = generic ? nullptr : c->trd;
+  a.pcell = c->pcell;
+  a.s2 = c->s2;
+  a.cf = cell_frame(c->grid);
   a.v_nonzero = (fl & 1) ? 1 : 0;
   a.b_uniform = (fl & 2) ? 0 : 1;
   // loop 2's error word and k_post1's h_max stripes: cleared by this step's k_prep
@@ -1148,7 +1175,11 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
   c->stats.sweeps = sweeps;
   c->stats.loop1_pairs = pairs;
   c->stats.ms_k_loop1 = ms1;
-  c->sweep_hint = std::min(maxsw, std::max(2, sweeps + 1));
+  // the next step launches as many sweeps as the last two steps took when
+  // they agree (a sweep past convergence still costs its launch), one more
+  // otherwise (a step that needs more pays a round trip for them)
+  c->sweep_hint = sweeps == c->last_sweeps ? std::max(1, sweeps) : std::min(maxsw, std::max(2, sweeps + 1));
+  c->last_sweeps = sweeps;
   const int rc2 = finish_loop2(c);
   const int r1 = map_err(bits);
   return r1 != SPH_OK ? r1 : rc2;
@@ -1369,7 +1400,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->cstart, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->jr[0], c->jr[1], c->pd, c->trd,
+                  c->sort_tmp, c->cstart, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->s1[0], c->s1[1], c->pcell, c->s2, c->pd, c->trd,
                   c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,

@@ -103,6 +103,23 @@ __device__ __forceinline__ void for_neighbours(const uint32_t *nb, int s, int np
   }
 }
 
+// The mixed path's 16-byte position record (pcell): the FP32 offsets of a
+// particle from the corner of its search cell and, in the fourth word, the
+// cell packed as x | y << 10 | z << 20 (3D), x | y << 15 (2D) or x (1D).  A
+// pair's separation is (c_a - c_b - s n) cs + (o_a - o_b) in FP32, s the
+// image shift along the axis and n its cell count.
+struct CellFrame {
+  float cs[3];  // cell widths
+  int nc[3];    // cells per axis
+};
+
+template <int DIM>
+__device__ __forceinline__ void cell_unpack(uint32_t v, int &cx, int &cy, int &cz) {
+  if (DIM == 1) { cx = (int)v; cy = 0; cz = 0; }
+  else if (DIM == 2) { cx = (int)(v & 0x7FFFu); cy = (int)(v >> 15); cz = 0; }
+  else { cx = (int)(v & 1023u); cy = (int)((v >> 10) & 1023u); cz = (int)(v >> 20); }
+}
+
 template <typename V4>
 struct Loop1Args {
   const double4 *pm;
@@ -114,8 +131,12 @@ struct Loop1Args {
   V4 *next;                // sweep k+1
   const double *trc;       // T den of sweep k (the only j-side value grad T needs)
   double *trn;             // T den of sweep k+1
-  const double4 *jrc;      // mixed path, grad T: (x, y, z, [m, T den] as FP32) of sweep k
-  double4 *jrn;            // ... of sweep k+1
+  // mixed path, grad T: the position records, and (m, T den) in FP32 of
+  // sweep k and of sweep k+1
+  const float4 *pcell;
+  const float2 *s1c;
+  float2 *s1n;
+  CellFrame cf;
   double *eps;
   double *dtdx, *dtdy, *dtdz;
   long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs, [2k] ran
@@ -154,8 +175,11 @@ struct Loop2Args {
   const V4 *q2;  // c, u, kappa, -
   const V4 *q3;  // dtdx, dtdy, dtdz, -
   const V4 *q4;  // bx, by, bz, -
-  const double4 *pd;  // MODE_DIFF: x, y, z, m / den
-  const double *trd;  // MODE_DIFF: T den
+  const double4 *pd;  // MODE_DIFF, FP64: x, y, z, m / den
+  const double *trd;  // MODE_DIFF, FP64: T den
+  const float4 *pcell;  // MODE_DIFF, mixed: the position records
+  const float2 *s2;     // MODE_DIFF, mixed: m / den, T den in FP32
+  CellFrame cf;
   int v_nonzero;  // 0: every velocity is exactly 0 (v_ab = 0 for all pairs)
   int b_uniform;  // 1: every particle carries the same B bit for bit (kappa_b = kappa_a)
   Fields out;

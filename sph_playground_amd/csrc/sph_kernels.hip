@@ -266,27 +266,35 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *keys, int n,
 // to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
-// pos = 0: the position records (pm, real, pf, band codes) of the previous
-// prep still hold (same positions, order and grid) and are not rewritten.
+// pos = 0: the position records (pm, real, pf, pcell, band codes) of the
+// previous prep still hold (same positions, order and grid) and are not
+// rewritten.
 __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, int64_t n_real, Box b, Grid g,
                                          int dim, double4 *pm, double *h0, float4 *pf, uint8_t *real, double4 *st,
-                                         double *tr, double4 *jr, double *eps, int pos) {
+                                         double *tr, float4 *pcell, float2 *s1, double *eps, int pos) {
   double h = F.f[F_H][s];
   h0[s] = h;
   bool r = orig[s] < n_real;
   st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
   eps[s] = r ? 1.0 : 0.0;
-  // T den for grad T: FP64 in tr, or in the FP32 path's packed record alone
-  if (!pos) {
-    if (jr) jr[s].w = pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s]));
-    else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
-    return;
-  }
+  // T den for grad T: FP64 in tr, or with m in FP32 on the mixed path
+  if (s1) s1[s] = make_float2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s]));
+  else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
+  if (!pos) return;
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
   pm[s] = make_double4(x, y, z, F.f[F_M][s]);
   real[s] = r ? 1 : 0;
-  if (jr) jr[s] = make_double4(x, y, z, pack_f2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s])));
-  else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
+  if (pcell) {  // the mixed path's position record: offsets from the cell corner, the cell
+    int c[3];
+    cell_of(g, x, y, z, c);
+    const double xs[3] = {x, y, z};
+    float o[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) o[a] = (float)((xs[a] - g.lo[a]) - (double)c[a] * g.cs[a]);
+    const uint32_t pk = dim == 3 ? (uint32_t)c[0] | ((uint32_t)c[1] << 10) | ((uint32_t)c[2] << 20)
+                                 : (dim == 2 ? (uint32_t)c[0] | ((uint32_t)c[1] << 15) : (uint32_t)c[0]);
+    pcell[s] = make_float4(o[0], o[1], o[2], __uint_as_float(pk));
+  }
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
   if (ty == 1 || ty == 2 || ty == 21) {
@@ -314,8 +322,8 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
 // zero[0, nzero): the step's counters, cleared by the first block before the
 // kernels that count into them run (they all follow k_prep on its stream).
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
-                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, double4 *jr, double *eps,
-                       long long *flags, long long *zero, int nzero, int pos) {
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, float4 *pcell, float2 *s1,
+                       double *eps, long long *flags, long long *zero, int nzero, int pos) {
   __shared__ long long lds[4];
   int s = gtid();
   if (blockIdx.x == 0)
@@ -327,7 +335,7 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
     }
-    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, jr, eps, pos);
+    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, pcell, s1, eps, pos);
   }
   if (flags) {
     long long t = block_reduce<R_OR>(fl, lds);
@@ -765,6 +773,27 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
   if (threadIdx.x == 0 && mx) atomicMax(&a.stripes[kStripes + stripe()], mx);
 }
 
+// Separation a - b of two mixed-path position records (pcell), b at its
+// image: (c_a - c_b - s n) cs + (o_a - o_b) per axis in FP32.  Each offset is
+// within 2^-24 cs of its FP64 value, so the separation is within about
+// 12 x 2^-24 cs of the FP64 one (cs = a third of the search radius).
+template <int DIM>
+__device__ __forceinline__ void cell_sep(const float4 &qa, int cax, int cay, int caz, const float4 &qb,
+                                         uint32_t code, const CellFrame &f, float &r0, float &r1, float &r2) {
+  int cbx, cby, cbz;
+  cell_unpack<DIM>(__float_as_uint(qb.w), cbx, cby, cbz);
+  int dx = cax - cbx, dy = cay - cby, dz = caz - cbz;
+  if (code != kNoImage) {
+    const int sx = (int)(code % 3) - 1, sy = (int)((code / 3) % 3) - 1, sz = (int)(code / 9) - 1;
+    dx -= sx * f.nc[0];
+    dy -= sy * f.nc[1];
+    dz -= sz * f.nc[2];
+  }
+  r0 = fmaf((float)dx, f.cs[0], qa.x - qb.x);
+  r1 = DIM >= 2 ? fmaf((float)dy, f.cs[1], qa.y - qb.y) : 0.f;
+  r2 = DIM == 3 ? fmaf((float)dz, f.cs[2], qa.z - qb.z) : 0.f;
+}
+
 // ---------------------------------------------------------------- loop 1
 // One Jacobi sweep of c1advanced (circuit1.f90:114-211) for the particles
 // still active (eps > tol); frozen ones carry their state to the next buffer.
@@ -794,12 +823,12 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   bool act = inb && a.real[s] && a.eps[s] > a.tol;
   bool still = false;
   int nn = 0;
-  // the FP32 path gathers one 32-byte record per pair: FP64 (x, y, z) and the
-  // FP32 m and T den it converts them to anyway
+  // the FP32 path gathers 24 bytes per pair: the 16-byte position record
+  // (offsets in the search cell, the cell) and the FP32 m and T den
   constexpr bool PACK = GRAD == 1 && std::is_same_v<PT, float>;
   if (inb && !act) {
     a.next[s] = c;
-    if constexpr (PACK) a.jrn[s] = a.jrc[s];
+    if constexpr (PACK) a.s1n[s] = a.s1c[s];
     else if constexpr (GRAD == 1) a.trn[s] = a.trc[s];
   }
   if (act) {
@@ -814,13 +843,21 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     nn = a.cnt[s];
     // j side: x, y, z, m, and what grad T needs of the previous sweep (GRAD 1:
     // T_b rho_b alone; GRAD 2: h_b, rho_b, om_b, T_b)
-    struct J { double4 pb, sb; double tr; uint32_t code; };
+    struct J { double4 pb, sb; float4 pc; float2 sc; double tr; uint32_t code; };
+    // the FP32 path's own position record and cell
+    float4 qa{};
+    int cax = 0, cay = 0, caz = 0;
+    if constexpr (PACK) {
+      qa = a.pcell[s];
+      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
+    }
     auto load = [&](uint32_t ent) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
       if constexpr (PACK) {
-        d.pb = a.jrc[j];
+        d.pc = a.pcell[j];
+        d.sc = a.s1c[j];
       } else {
         d.pb = a.pm[j];
         if constexpr (GRAD == 1) d.tr = a.trc[j];
@@ -831,9 +868,16 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     // four gathers in flight (with particles in row order, eight measured
     // slower in the FP32 path: 0.55 against 0.51 ms at 128^3)
     for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
-      double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
-      if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
-      PT r0 = (PT)(pa.x - xb), r1 = (PT)(pa.y - yb), r2 = (PT)(pa.z - zb);
+      PT r0, r1, r2;
+      if constexpr (PACK) {
+        cell_sep<DIM>(qa, cax, cay, caz, d.pc, d.code, a.cf, r0, r1, r2);
+      } else {
+        double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
+        if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
+        r0 = (PT)(pa.x - xb);
+        r1 = (PT)(pa.y - yb);
+        r2 = (PT)(pa.z - zb);
+      }
       PT rsq = r0 * r0 + r1 * r1 + r2 * r2;
       // FP64, and FP32 with grad T: 1/|r| from one reciprocal square root
       // (FP64: refined), |r| from it (FP64: 0 at r = 0, as the square root
@@ -848,7 +892,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
       PT f, df;
       K::f_df(q, f, df);
       PT mb;
-      if constexpr (PACK) mb = f2_lo(d.pb.w);
+      if constexpr (PACK) mb = d.sc.x;
       else mb = (PT)d.pb.w;
       dsum += mb * f;
       osum += mb * (PT(DIM) * f + q * df);
@@ -859,7 +903,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
         const PT inv_dr = inv_dr_rsq;
         if constexpr (GRAD == 1) {
           PT trb;
-          if constexpr (PACK) trb = f2_hi(d.pb.w);
+          if constexpr (PACK) trb = d.sc.y;
           else trb = (PT)d.tr;
           PT w = mb * (trb - (PT)ta * (PT)rhoa) * df * inv_dr;
           g0 += w * r0; g1 += w * r1; g2 += w * r2;
@@ -902,7 +946,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     a.eps[s] = ep;
     still = ep > a.tol;
     a.next[s] = make_double4(hn, den, osm, ta);
-    if constexpr (PACK) a.jrn[s] = make_double4(pa.x, pa.y, pa.z, pack_f2((float)ma, (float)(ta * den)));
+    if constexpr (PACK) a.s1n[s] = make_float2((float)ma, (float)(ta * den));
     else if constexpr (GRAD == 1) a.trn[s] = ta * den;
     double gx = 0, gy = 0, gz = 0;
     if constexpr (GRAD == 1) {
@@ -1002,7 +1046,7 @@ template <typename V4>
 __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint8_t *real, int n,
                                                int advanced, int do_eos, int iso, double c1, double gamma, V4 *q0,
                                                V4 *q1, V4 *q2, V4 *q3, V4 *q4, double4 *pd, double *trd,
-                                               long long *flags, long long *hmax_red) {
+                                               float2 *s2, long long *flags, long long *hmax_red) {
   __shared__ long long lds[4];
   using T = decltype(V4().x);
   if (*sel.serr != 0 || *sel.err1 != 0) return;  // set before this launch: uniform
@@ -1038,14 +1082,11 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
     if (q4) q4[s] = V4{(T)bx, (T)by, (T)bz, (T)0};
-    if (pd) {
-      if constexpr (std::is_same_v<V4, float4>) {  // FP32 path: m/den and T den as FP32 in the fourth slot
-        pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s],
-                             pack_f2((float)(F.f[F_M][s] / den), (float)(F.f[F_T][s] * den)));
-      } else {
-        pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
-        trd[s] = F.f[F_T][s] * den;
-      }
+    if constexpr (std::is_same_v<V4, float4>) {  // FP32 path: m/den and T den in FP32 beside pcell
+      if (s2) s2[s] = make_float2((float)(F.f[F_M][s] / den), (float)(F.f[F_T][s] * den));
+    } else if (pd) {
+      pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
+      trd[s] = F.f[F_T][s] * den;
     }
   }
   if (flags) {
@@ -1178,15 +1219,28 @@ void k_loop2(Loop2Args<V4> a) {
     // the FP32 path's record holds m/den and T den as FP32 in its fourth slot
     constexpr bool PACK = std::is_same_v<V4, float4>;
     struct JG { double4 pb; V4 q0; uint32_t j, code; };
-    struct JD { double4 pb; double tr; uint32_t j, code; };
+    struct JD { double4 pb; float4 pc; float2 sc; double tr; uint32_t j, code; };
     using J = std::conditional_t<MODE == MODE_DIFF, JD, JG>;
+    // MODE_DIFF's FP32 path: the own position record and cell
+    constexpr bool CELL = MODE == MODE_DIFF && PACK;
+    float4 qa{};
+    int cax = 0, cay = 0, caz = 0;
+    if constexpr (CELL) {
+      qa = a.pcell[s];
+      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
+    }
     auto load = [&](uint32_t ent) {
       J d;
       d.j = nb_index(ent);
       d.code = nb_code(ent);
       if constexpr (MODE == MODE_DIFF) {
-        d.pb = a.pd[d.j];
-        if constexpr (!PACK) d.tr = a.trd[d.j];
+        if constexpr (PACK) {
+          d.pc = a.pcell[d.j];
+          d.sc = a.s2[d.j];
+        } else {
+          d.pb = a.pd[d.j];
+          d.tr = a.trd[d.j];
+        }
       } else {
         d.pb = a.pm[d.j];
         d.q0 = a.q0[d.j];
@@ -1195,25 +1249,32 @@ void k_loop2(Loop2Args<V4> a) {
     };
     constexpr int NBATCH = MODE == MODE_DIFF ? 4 : 1;
     for_neighbours<NBATCH>(a.nb, s, a.npad, nn, load, [&](const J &d) {
-      const double4 pb = d.pb;
       const uint32_t code = d.code;
-      double xb = pb.x, yb = pb.y, zb = pb.z;
-      if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
-      const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
+      PT rx, ry, rz;
+      if constexpr (CELL) {
+        cell_sep<DIM>(qa, cax, cay, caz, d.pc, code, a.cf, rx, ry, rz);
+      } else {
+        const double4 pb = d.pb;
+        double xb = pb.x, yb = pb.y, zb = pb.z;
+        if (code != kNoImage) image_shift(xb, yb, zb, code, a.b);
+        rx = (PT)(pa.x - xb);
+        ry = (PT)(pa.y - yb);
+        rz = (PT)(pa.z - zb);
+      }
       // j-side scalars; in MODE_DIFF mb_rhob = m_b / rho_b and trb = T_b rho_b
       PT mb = 0, inv_hb = 0, rhob = 0, omb = 0, tb = 0, mb_rhob = 0, trb = 0;
       if constexpr (MODE == MODE_DIFF) {
         if constexpr (PACK) {
-          mb_rhob = f2_lo(pb.w);
-          trb = f2_hi(pb.w);
+          mb_rhob = d.sc.x;
+          trb = d.sc.y;
         } else {
-          mb_rhob = (PT)pb.w;
+          mb_rhob = (PT)d.pb.w;
           trb = (PT)d.tr;
         }
         if (need_v && P.adden) mb = (PT)a.pm[d.j].w;  // d rho/dt of moving particles only
       } else {
         const V4 B0 = d.q0;
-        mb = (PT)pb.w;
+        mb = (PT)d.pb.w;
         inv_hb = (PT)B0.x; rhob = (PT)B0.y; omb = (PT)B0.z; tb = (PT)B0.w;
       }
       const PT r2 = rx * rx + ry * ry + rz * rz;
@@ -1427,23 +1488,41 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     PT ddt = 0;
     bool qbad = !(inv_ha > PT(0));
     const int nn = a.cnt[s];
-    // the FP32 record holds m/den and T den as FP32 in its fourth slot
-    struct J { double4 pb; double tr; uint32_t code; };
+    // FP64: (x, y, z, m/den) and T den; FP32: the 16-byte position record and
+    // (m/den, T den) in FP32
+    struct J { double4 pb; float4 pc; float2 sc; double tr; uint32_t code; };
+    float4 qa{};
+    int cax = 0, cay = 0, caz = 0;
+    if constexpr (F32) {
+      qa = a.pcell[s];
+      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
+    }
     auto load = [&](uint32_t ent) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
-      d.pb = a.pd[j];
-      if constexpr (!F32) d.tr = a.trd[j];
+      if constexpr (F32) {
+        d.pc = a.pcell[j];
+        d.sc = a.s2[j];
+      } else {
+        d.pb = a.pd[j];
+        d.tr = a.trd[j];
+      }
       return d;
     };
     for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
-      const double4 pb = d.pb;
-      double xb = pb.x, yb = pb.y, zb = pb.z;
-      if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
-      const PT rx = (PT)(pa.x - xb), ry = (PT)(pa.y - yb), rz = (PT)(pa.z - zb);
+      PT rx, ry, rz;
       if constexpr (F32) {
-        const float mb_rhob = f2_lo(pb.w), trb = f2_hi(pb.w);
+        cell_sep<DIM>(qa, cax, cay, caz, d.pc, d.code, a.cf, rx, ry, rz);
+      } else {
+        double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
+        if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
+        rx = (PT)(pa.x - xb);
+        ry = (PT)(pa.y - yb);
+        rz = (PT)(pa.z - zb);
+      }
+      if constexpr (F32) {
+        const float mb_rhob = d.sc.x, trb = d.sc.y;
         const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
         const float inv_dr = __builtin_amdgcn_rsqf(r2);
         const float qa = r2 * inv_dr * inv_ha;
@@ -1459,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
         const float x = fmaf(S, inv_dr * inv_dr, -tr);
         ddt = fmaf(mb_rhob * (trb - tra), dfa * inv_dr * x, ddt);
       } else {
-        const double mb_rhob = pb.w, trb = d.tr;
+        const double mb_rhob = d.pb.w, trb = d.tr;
         const double r2 = rx * rx + ry * ry + rz * rz;
         // 1/|r| refined from the hardware estimate; |r| = 0 at r = 0, as the
         // square root gives (1/|r| is then NaN, and so was the pair term
@@ -1839,9 +1918,9 @@ SPH_INST_NEIGH(2)
 SPH_INST_NEIGH(3)
 template __global__ void k_post1<double4>(Fields, PostSel, const uint8_t *, int, int, int, int, double,
                                           double, double4 *, double4 *, double4 *, double4 *, double4 *, double4 *,
-                                          double *, long long *, long long *);
+                                          double *, float2 *, long long *, long long *);
 template __global__ void k_post1<float4>(Fields, PostSel, const uint8_t *, int, int, int, int, double,
                                          double, float4 *, float4 *, float4 *, float4 *, float4 *, double4 *,
-                                         double *, long long *, long long *);
+                                         double *, float2 *, long long *, long long *);
 
 }  // namespace sph
