@@ -247,7 +247,6 @@ struct sph_ctx {
   // ran flag), one region per sweep of a step, and their pinned mirror
   long long *d_sweep = nullptr, *h_sweep = nullptr;
   int sweep_hint = 4;             // sweeps the next step launches before it synchronises
-  int last_sweeps = -1;           // Newton sweeps of the previous step
   sph_stats stats{};
   bool have_step = false;
   // the resident store holds the derivatives of the last successful iterate
@@ -1175,11 +1174,11 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
   c->stats.sweeps = sweeps;
   c->stats.loop1_pairs = pairs;
   c->stats.ms_k_loop1 = ms1;
-  // the next step launches as many sweeps as the last two steps took when
-  // they agree (a sweep past convergence still costs its launch), one more
-  // otherwise (a step that needs more pays a round trip for them)
-  c->sweep_hint = sweeps == c->last_sweeps ? std::max(1, sweeps) : std::min(maxsw, std::max(2, sweeps + 1));
-  c->last_sweeps = sweeps;
+  // one sweep more than this step took: a sweep past convergence exits at
+  // once (its launch costs about 16 us at 2M particles), while a step that
+  // needs more sweeps than launched runs its queued tail (EOS, packs,
+  // loop 2) again after them
+  c->sweep_hint = std::min(maxsw, std::max(2, sweeps + 1));
   const int rc2 = finish_loop2(c);
   const int r1 = map_err(bits);
   return r1 != SPH_OK ? r1 : rc2;
