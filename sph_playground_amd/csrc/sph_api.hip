@@ -31,8 +31,8 @@ __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
-                       uint8_t *, double4 *, double *, float4 *, float2 *, double *, long long *, long long *, int,
-                       int);
+                       uint8_t *, double4 *, double *, uint3 *, FixFrame, float2 *, double *, long long *,
+                       long long *, int, int);
 template <int DIM, bool EXT>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -192,9 +192,10 @@ struct sph_ctx {
   int64_t ncell_cap = 0;
   double4 *pm = nullptr, *st[2] = {nullptr, nullptr};
   double *tr[2] = {nullptr, nullptr};  // T den of the loop-1 sweep state st[k]
-  // mixed path: position records (FP32 offsets in the search cell, the
-  // cell), (m, T den) of sweep state st[k] and (m/den, T den) for loop 2
-  float4 *pcell = nullptr;
+  // mixed path: position records (31-bit fractions, `ff` their frame),
+  // (m, T den) of sweep state st[k] and (m/den, T den) for loop 2
+  uint3 *pfix = nullptr;
+  FixFrame ff{};
   float2 *s1[2] = {nullptr, nullptr};
   float2 *s2 = nullptr;
   double4 *pd = nullptr;                // loop 2, diffusion mode: x, y, z, m/den
@@ -379,7 +380,7 @@ static int ensure_particles(sph_ctx *c, int64_t n) {
     if (c->opt.precision == 1) RCHK(dalloc(&c->s1[b], npad));
   }
   if (c->opt.precision == 1) {
-    RCHK(dalloc(&c->pcell, npad));
+    RCHK(dalloc(&c->pfix, npad));
     RCHK(dalloc(&c->s2, npad));
   }
   RCHK(dalloc(&c->pd, npad));
@@ -501,12 +502,31 @@ constexpr int kWrapFlag = 4;
 constexpr int kCloseRed = 9 * kStripes;
 constexpr int kCloseLL = 1536;
 
-// the mixed path's position-record frame of a search grid
-static CellFrame cell_frame(const Grid &g) {
-  CellFrame f{};
-  for (int d = 0; d < 3; d++) {
-    f.cs[d] = (float)g.cs[d];
-    f.nc[d] = g.nc[d];
+// The mixed path's position-record frame: an imaged axis maps its box length
+// L onto 2^(31-k), k the least with 2^k L beyond any listed separation along
+// it (the stencil's reach), so wrap-around differences of the fractions stay
+// exact; other axes map the grid span onto 2^31.
+static FixFrame fix_frame(const sph_ctx *c, const Grid &g, int dim, const int rc[3], double rmax) {
+  FixFrame f{};
+  for (int a = 0; a < dim; a++) {
+    if (g.periodic[a]) {
+      const double L = c->box.emax[a] - c->box.emin[a];
+      const double reach = std::max(rc[a] * g.cs[a], rmax) + g.cs[a];
+      int k = 0;
+      while (k < 24 && std::ldexp(L, k) <= reach) k++;
+      const double m = std::ldexp(1.0, 31 - k);
+      f.base[a] = c->box.emin[a];
+      f.mul[a] = m / L;
+      f.scale[a] = (float)(L / m);
+      f.step[a] = (uint32_t)m;
+    } else {
+      const double span = g.nc[a] * g.cs[a];
+      const double m = std::ldexp(1.0, 31);
+      f.base[a] = g.lo[a];
+      f.mul[a] = span > 0.0 ? m / span : 0.0;
+      f.scale[a] = (float)(span / m);
+      f.step[a] = 0;
+    }
   }
   return f;
 }
@@ -723,15 +743,6 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   // the v / B flags are computed again only when v or B may have changed
   c->flags_pending = c->flags_version != c->vb_version;
   if (c->flags_pending) HIPCHK(hipMemsetAsync(c->d_stripes + kFlagSlots, 0, sizeof(long long) * kStripes, c->stream));
-  const bool pos_kept = keys_kept && c->prep_version == c->pos_version && c->prep_cur == c->cur &&
-                        c->prep_n == n && same_grid(g, c->prep_grid);
-  hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->pcell, c->s1[0], c->eps,
-                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
-  c->prep_version = c->pos_version;
-  c->prep_cur = c->cur;
-  c->prep_n = n;
-  c->prep_grid = g;
   // rows and cells within r_max (1 + 1e-6) of a cell; imaged axes at most
   // one wrap away
   int rc[3];
@@ -739,6 +750,18 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     rc[d] = d < dim && cs[d] > 0 ? (int)std::ceil(rmax * (1.0 + 1e-6) / cs[d]) : 0;
     if (g.periodic[d]) rc[d] = std::min(rc[d], g.nc[d]);
   }
+  const FixFrame ff = fix_frame(c, g, dim, rc, rmax);
+  const bool pos_kept = keys_kept && c->prep_version == c->pos_version && c->prep_cur == c->cur &&
+                        c->prep_n == n && same_grid(g, c->prep_grid) &&
+                        std::memcmp(&ff, &c->ff, sizeof(ff)) == 0;
+  c->ff = ff;
+  hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
+                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->pfix, ff, c->s1[0], c->eps,
+                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
+  c->prep_version = c->pos_version;
+  c->prep_cur = c->cur;
+  c->prep_n = n;
+  c->prep_grid = g;
   // 4. the extended grid: rc ghost cells on each side of every imaged axis
   //    hold the periodic images, so every stencil row of the search is one
   //    range of records; rebuilt when the positions, their order, the grid
@@ -934,8 +957,8 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
     a.trn = c->tr[it & 1];
     a.s1c = c->s1[(it - 1) & 1];
     a.s1n = c->s1[it & 1];
-    a.pcell = c->pcell;
-    a.cf = cell_frame(c->grid);
+    a.pfix = c->pfix;
+    a.ff = c->ff;
     a.stripes = c->d_sweep + (size_t)it * kSweepRegion;
     a.prev = it > 1 ? c->d_sweep + (size_t)(it - 1) * kSweepRegion : nullptr;
     a.sweep = it;
@@ -1030,9 +1053,9 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   a.q0 = q0; a.q1 = q1; a.q2 = q2; a.q3 = q3; a.q4 = q4;
   a.pd = generic ? nullptr : c->pd;
   a.trd = generic ? nullptr : c->trd;
-  a.pcell = c->pcell;
+  a.pfix = c->pfix;
   a.s2 = c->s2;
-  a.cf = cell_frame(c->grid);
+  a.ff = c->ff;
   a.v_nonzero = (fl & 1) ? 1 : 0;
   a.b_uniform = (fl & 2) ? 0 : 1;
   // loop 2's error word and k_post1's h_max stripes: cleared by this step's k_prep
@@ -1399,7 +1422,7 @@ int sph_ctx_destroy(sph_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->fbuf[0], c->fbuf[1], c->orig[0], c->orig[1], c->keys, c->keys2, c->vals, c->vals2,
-                  c->sort_tmp, c->cstart, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->s1[0], c->s1[1], c->pcell, c->s2, c->pd, c->trd,
+                  c->sort_tmp, c->cstart, c->pm, c->st[0], c->st[1], c->tr[0], c->tr[1], c->s1[0], c->s1[1], c->pfix, c->s2, c->pd, c->trd,
                   c->h0, c->eps, c->pf,
                   c->real, c->nb, c->cnt, c->q[0], c->q[1], c->q[2], c->q[3], c->q[4], c->energy_part,
                   c->d_red, c->d_int, c->d_stripes, c->d_staging, c->roff, c->rtmp, c->rent, c->scan_tmp,

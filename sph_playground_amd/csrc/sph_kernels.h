@@ -103,22 +103,18 @@ __device__ __forceinline__ void for_neighbours(const uint32_t *nb, int s, int np
   }
 }
 
-// The mixed path's 16-byte position record (pcell): the FP32 offsets of a
-// particle from the corner of its search cell and, in the fourth word, the
-// cell packed as x | y << 10 | z << 20 (3D), x | y << 15 (2D) or x (1D).  A
-// pair's separation is (c_a - c_b - s n) cs + (o_a - o_b) in FP32, s the
-// image shift along the axis and n its cell count.
-struct CellFrame {
-  float cs[3];  // cell widths
-  int nc[3];    // cells per axis
+// The mixed path's 12-byte position record (pfix): per axis the position as
+// a 31-bit fraction, u = (x - base) mul.  An imaged axis maps its box length L
+// onto 2^(31-k) with 2^k L above the stencil's reach, an axis without images
+// maps the grid span onto 2^31.  A pair's separation along an axis is the
+// 32-bit wrap-around difference u_a - u_b - s step (s the image shift, step
+// the mapped box length), exact while the separation is under 2^k L, times
+// scale in FP32.
+struct FixFrame {
+  double base[3], mul[3];  // positions -> fractions (k_prep)
+  float scale[3];          // fraction units -> length
+  uint32_t step[3];        // mapped box length of an imaged axis (0 otherwise)
 };
-
-template <int DIM>
-__device__ __forceinline__ void cell_unpack(uint32_t v, int &cx, int &cy, int &cz) {
-  if (DIM == 1) { cx = (int)v; cy = 0; cz = 0; }
-  else if (DIM == 2) { cx = (int)(v & 0x7FFFu); cy = (int)(v >> 15); cz = 0; }
-  else { cx = (int)(v & 1023u); cy = (int)((v >> 10) & 1023u); cz = (int)(v >> 20); }
-}
 
 template <typename V4>
 struct Loop1Args {
@@ -133,10 +129,10 @@ struct Loop1Args {
   double *trn;             // T den of sweep k+1
   // mixed path, grad T: the position records, and (m, T den) in FP32 of
   // sweep k and of sweep k+1
-  const float4 *pcell;
+  const uint3 *pfix;
   const float2 *s1c;
   float2 *s1n;
-  CellFrame cf;
+  FixFrame ff;
   double *eps;
   double *dtdx, *dtdy, *dtdz;
   long long *stripes;      // [kStripes] still-active counts, [kStripes..2k) evaluated pairs, [2k] ran
@@ -177,9 +173,9 @@ struct Loop2Args {
   const V4 *q4;  // bx, by, bz, -
   const double4 *pd;  // MODE_DIFF, FP64: x, y, z, m / den
   const double *trd;  // MODE_DIFF, FP64: T den
-  const float4 *pcell;  // MODE_DIFF, mixed: the position records
+  const uint3 *pfix;    // MODE_DIFF, mixed: the position records
   const float2 *s2;     // MODE_DIFF, mixed: m / den, T den in FP32
-  CellFrame cf;
+  FixFrame ff;
   int v_nonzero;  // 0: every velocity is exactly 0 (v_ab = 0 for all pairs)
   int b_uniform;  // 1: every particle carries the same B bit for bit (kappa_b = kappa_a)
   Fields out;

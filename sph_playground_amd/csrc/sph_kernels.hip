@@ -266,12 +266,13 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *keys, int n,
 // to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
 // gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
-// pos = 0: the position records (pm, real, pf, pcell, band codes) of the
+// pos = 0: the position records (pm, real, pf, pfix, band codes) of the
 // previous prep still hold (same positions, order and grid) and are not
 // rewritten.
 __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, int64_t n_real, Box b, Grid g,
                                          int dim, double4 *pm, double *h0, float4 *pf, uint8_t *real, double4 *st,
-                                         double *tr, float4 *pcell, float2 *s1, double *eps, int pos) {
+                                         double *tr, uint3 *pfix, const FixFrame &ff, float2 *s1, double *eps,
+                                         int pos) {
   double h = F.f[F_H][s];
   h0[s] = h;
   bool r = orig[s] < n_real;
@@ -284,16 +285,12 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
   pm[s] = make_double4(x, y, z, F.f[F_M][s]);
   real[s] = r ? 1 : 0;
-  if (pcell) {  // the mixed path's position record: offsets from the cell corner, the cell
-    int c[3];
-    cell_of(g, x, y, z, c);
+  if (pfix) {  // the mixed path's position record: 31-bit fractions (the wrap keeps differences)
     const double xs[3] = {x, y, z};
-    float o[3];
+    uint32_t u[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) o[a] = (float)((xs[a] - g.lo[a]) - (double)c[a] * g.cs[a]);
-    const uint32_t pk = dim == 3 ? (uint32_t)c[0] | ((uint32_t)c[1] << 10) | ((uint32_t)c[2] << 20)
-                                 : (dim == 2 ? (uint32_t)c[0] | ((uint32_t)c[1] << 15) : (uint32_t)c[0]);
-    pcell[s] = make_float4(o[0], o[1], o[2], __uint_as_float(pk));
+    for (int a = 0; a < 3; a++) u[a] = (uint32_t)(long long)floor((xs[a] - ff.base[a]) * ff.mul[a]);
+    pfix[s] = make_uint3(u[0], u[1], u[2]);
   }
   int ty = (int)F.f[F_TYPE][s];
   uint32_t code = 0;
@@ -322,8 +319,8 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
 // zero[0, nzero): the step's counters, cleared by the first block before the
 // kernels that count into them run (they all follow k_prep on its stream).
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
-                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, float4 *pcell, float2 *s1,
-                       double *eps, long long *flags, long long *zero, int nzero, int pos) {
+                       double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, uint3 *pfix, FixFrame ff,
+                       float2 *s1, double *eps, long long *flags, long long *zero, int nzero, int pos) {
   __shared__ long long lds[4];
   int s = gtid();
   if (blockIdx.x == 0)
@@ -335,7 +332,7 @@ __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(F.f[F_BX][s] == F.f[F_BX][0] && F.f[F_BY][s] == F.f[F_BY][0] && F.f[F_BZ][s] == F.f[F_BZ][0])) fl |= 2;
     }
-    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, pcell, s1, eps, pos);
+    prep_one(F, orig, s, n_real, b, g, dim, pm, h0, pf, real, st, tr, pfix, ff, s1, eps, pos);
   }
   if (flags) {
     long long t = block_reduce<R_OR>(fl, lds);
@@ -773,25 +770,24 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
   if (threadIdx.x == 0 && mx) atomicMax(&a.stripes[kStripes + stripe()], mx);
 }
 
-// Separation a - b of two mixed-path position records (pcell), b at its
-// image: (c_a - c_b - s n) cs + (o_a - o_b) per axis in FP32.  Each offset is
-// within 2^-24 cs of its FP64 value, so the separation is within about
-// 12 x 2^-24 cs of the FP64 one (cs = a third of the search radius).
+// Separation a - b of two mixed-path position records (pfix), b at its
+// image: per axis the 32-bit wrap-around difference of the fractions, less
+// the image's mapped box length, times the axis scale in FP32.  Fractions are
+// within one unit (at most L / 2^31) of the positions, so the separation is
+// within two units before its FP32 conversion and product (each 2^-24).
 template <int DIM>
-__device__ __forceinline__ void cell_sep(const float4 &qa, int cax, int cay, int caz, const float4 &qb,
-                                         uint32_t code, const CellFrame &f, float &r0, float &r1, float &r2) {
-  int cbx, cby, cbz;
-  cell_unpack<DIM>(__float_as_uint(qb.w), cbx, cby, cbz);
-  int dx = cax - cbx, dy = cay - cby, dz = caz - cbz;
+__device__ __forceinline__ void fix_sep(const uint3 &qa, const uint3 &qb, uint32_t code, const FixFrame &f,
+                                        float &r0, float &r1, float &r2) {
+  uint32_t dx = qa.x - qb.x, dy = qa.y - qb.y, dz = qa.z - qb.z;
   if (code != kNoImage) {
     const int sx = (int)(code % 3) - 1, sy = (int)((code / 3) % 3) - 1, sz = (int)(code / 9) - 1;
-    dx -= sx * f.nc[0];
-    dy -= sy * f.nc[1];
-    dz -= sz * f.nc[2];
+    dx -= (uint32_t)sx * f.step[0];
+    dy -= (uint32_t)sy * f.step[1];
+    dz -= (uint32_t)sz * f.step[2];
   }
-  r0 = fmaf((float)dx, f.cs[0], qa.x - qb.x);
-  r1 = DIM >= 2 ? fmaf((float)dy, f.cs[1], qa.y - qb.y) : 0.f;
-  r2 = DIM == 3 ? fmaf((float)dz, f.cs[2], qa.z - qb.z) : 0.f;
+  r0 = (float)(int)dx * f.scale[0];
+  r1 = DIM >= 2 ? (float)(int)dy * f.scale[1] : 0.f;
+  r2 = DIM == 3 ? (float)(int)dz * f.scale[2] : 0.f;
 }
 
 // ---------------------------------------------------------------- loop 1
@@ -823,8 +819,8 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   bool act = inb && a.real[s] && a.eps[s] > a.tol;
   bool still = false;
   int nn = 0;
-  // the FP32 path gathers 24 bytes per pair: the 16-byte position record
-  // (offsets in the search cell, the cell) and the FP32 m and T den
+  // the FP32 path gathers 20 bytes per pair: the 12-byte position record
+  // (31-bit fractions) and the FP32 m and T den
   constexpr bool PACK = GRAD == 1 && std::is_same_v<PT, float>;
   if (inb && !act) {
     a.next[s] = c;
@@ -843,20 +839,16 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     nn = a.cnt[s];
     // j side: x, y, z, m, and what grad T needs of the previous sweep (GRAD 1:
     // T_b rho_b alone; GRAD 2: h_b, rho_b, om_b, T_b)
-    struct J { double4 pb, sb; float4 pc; float2 sc; double tr; uint32_t code; };
-    // the FP32 path's own position record and cell
-    float4 qa{};
-    int cax = 0, cay = 0, caz = 0;
-    if constexpr (PACK) {
-      qa = a.pcell[s];
-      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
-    }
+    struct J { double4 pb, sb; uint3 pc; float2 sc; double tr; uint32_t code; };
+    // the FP32 path's own position record
+    uint3 qa{};
+    if constexpr (PACK) qa = a.pfix[s];
     auto load = [&](uint32_t ent) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
       if constexpr (PACK) {
-        d.pc = a.pcell[j];
+        d.pc = a.pfix[j];
         d.sc = a.s1c[j];
       } else {
         d.pb = a.pm[j];
@@ -870,7 +862,7 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
       PT r0, r1, r2;
       if constexpr (PACK) {
-        cell_sep<DIM>(qa, cax, cay, caz, d.pc, d.code, a.cf, r0, r1, r2);
+        fix_sep<DIM>(qa, d.pc, d.code, a.ff, r0, r1, r2);
       } else {
         double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
         if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
@@ -1082,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
     if (q4) q4[s] = V4{(T)bx, (T)by, (T)bz, (T)0};
-    if constexpr (std::is_same_v<V4, float4>) {  // FP32 path: m/den and T den in FP32 beside pcell
+    if constexpr (std::is_same_v<V4, float4>) {  // FP32 path: m/den and T den in FP32 beside pfix
       if (s2) s2[s] = make_float2((float)(F.f[F_M][s] / den), (float)(F.f[F_T][s] * den));
     } else if (pd) {
       pd[s] = make_double4(F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], F.f[F_M][s] / den);
@@ -1219,23 +1211,19 @@ void k_loop2(Loop2Args<V4> a) {
     // the FP32 path's record holds m/den and T den as FP32 in its fourth slot
     constexpr bool PACK = std::is_same_v<V4, float4>;
     struct JG { double4 pb; V4 q0; uint32_t j, code; };
-    struct JD { double4 pb; float4 pc; float2 sc; double tr; uint32_t j, code; };
+    struct JD { double4 pb; uint3 pc; float2 sc; double tr; uint32_t j, code; };
     using J = std::conditional_t<MODE == MODE_DIFF, JD, JG>;
-    // MODE_DIFF's FP32 path: the own position record and cell
+    // MODE_DIFF's FP32 path: the own position record
     constexpr bool CELL = MODE == MODE_DIFF && PACK;
-    float4 qa{};
-    int cax = 0, cay = 0, caz = 0;
-    if constexpr (CELL) {
-      qa = a.pcell[s];
-      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
-    }
+    uint3 qa{};
+    if constexpr (CELL) qa = a.pfix[s];
     auto load = [&](uint32_t ent) {
       J d;
       d.j = nb_index(ent);
       d.code = nb_code(ent);
       if constexpr (MODE == MODE_DIFF) {
         if constexpr (PACK) {
-          d.pc = a.pcell[d.j];
+          d.pc = a.pfix[d.j];
           d.sc = a.s2[d.j];
         } else {
           d.pb = a.pd[d.j];
@@ -1252,7 +1240,7 @@ void k_loop2(Loop2Args<V4> a) {
       const uint32_t code = d.code;
       PT rx, ry, rz;
       if constexpr (CELL) {
-        cell_sep<DIM>(qa, cax, cay, caz, d.pc, code, a.cf, rx, ry, rz);
+        fix_sep<DIM>(qa, d.pc, code, a.ff, rx, ry, rz);
       } else {
         const double4 pb = d.pb;
         double xb = pb.x, yb = pb.y, zb = pb.z;
@@ -1488,21 +1476,17 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     PT ddt = 0;
     bool qbad = !(inv_ha > PT(0));
     const int nn = a.cnt[s];
-    // FP64: (x, y, z, m/den) and T den; FP32: the 16-byte position record and
+    // FP64: (x, y, z, m/den) and T den; FP32: the 12-byte position record and
     // (m/den, T den) in FP32
-    struct J { double4 pb; float4 pc; float2 sc; double tr; uint32_t code; };
-    float4 qa{};
-    int cax = 0, cay = 0, caz = 0;
-    if constexpr (F32) {
-      qa = a.pcell[s];
-      cell_unpack<DIM>(__float_as_uint(qa.w), cax, cay, caz);
-    }
+    struct J { double4 pb; uint3 pc; float2 sc; double tr; uint32_t code; };
+    uint3 qa{};
+    if constexpr (F32) qa = a.pfix[s];
     auto load = [&](uint32_t ent) {
       J d;
       const uint32_t j = nb_index(ent);
       d.code = nb_code(ent);
       if constexpr (F32) {
-        d.pc = a.pcell[j];
+        d.pc = a.pfix[j];
         d.sc = a.s2[j];
       } else {
         d.pb = a.pd[j];
@@ -1513,7 +1497,7 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     for_neighbours<4>(a.nb, s, a.npad, nn, load, [&](const J &d) {
       PT rx, ry, rz;
       if constexpr (F32) {
-        cell_sep<DIM>(qa, cax, cay, caz, d.pc, d.code, a.cf, rx, ry, rz);
+        fix_sep<DIM>(qa, d.pc, d.code, a.ff, rx, ry, rz);
       } else {
         double xb = d.pb.x, yb = d.pb.y, zb = d.pb.z;
         if (d.code != kNoImage) image_shift(xb, yb, zb, d.code, a.b);
