@@ -50,7 +50,7 @@ typedef struct {
   double kappa_perp;  /* perpendicular conductivity for tensor_mode 1 */
   int max_sweeps;     /* Newton sweeps cap; 0 -> 100 (circuit1.f90:114) */
   double newton_tol;  /* relative h tolerance; 0 -> 1e-8 (circuit1.f90:104) FP64, 1e-6 mixed */
-  int neigh_cap;      /* neighbour cap incl. self; 0 -> reference maxneibnum(dim) */
+  int neigh_cap;      /* neighbour cap incl. self, at most 2^20; 0 -> reference maxneibnum(dim) */
   int device;         /* HIP device ordinal */
   int cell_div;       /* search cells are r_max / cell_div wide (1..4); 0 -> 3.  The
                          neighbour sets do not depend on it, only the search's speed */

@@ -762,21 +762,23 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   c->prep_cur = c->cur;
   c->prep_n = n;
   c->prep_grid = g;
-  // 4. the extended grid: rc ghost cells on each side of every imaged axis
-  //    hold the periodic images, so every stencil row of the search is one
-  //    range of records; rebuilt when the positions, their order, the grid
-  //    or the stencil change
+  // 4. the extended grid: rc ghost cells on each side of every axis (the
+  //    periodic images on an imaged axis, empty on a walled one), so every
+  //    stencil row of the search is one range of records and no row or cell
+  //    needs a bounds test; rebuilt when the positions, their order, the
+  //    grid or the stencil change
   bool use_ext = !c->lane_search;
   ExtGrid xg{};
   int64_t ncells = 1;
   for (int d = 0; d < 3; d++) {
-    xg.gh[d] = g.periodic[d] ? rc[d] : 0;
+    xg.gh[d] = rc[d];
     xg.ne[d] = g.nc[d] + 2 * xg.gh[d];
     xg.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
     ncells *= xg.ne[d];
   }
-  // offsets are ints: every record has at most 27 copies
-  if (ncells + 1 > INT32_MAX || (int64_t)n * 27 > INT32_MAX) use_ext = false;
+  // offsets are ints (every record has at most 27 copies) and the search
+  // addresses the cell starts with 32-bit byte offsets
+  if (ncells + 1 > INT32_MAX / 4 || (int64_t)n * 27 > INT32_MAX) use_ext = false;
   if (use_ext) {
     const bool ext_kept = c->ext_version == c->pos_version && c->ext_cur == c->cur && c->ext_n == n &&
                           same_grid(g, c->ext_grid) && std::memcmp(&xg, &c->ext_g, sizeof(xg)) == 0;
@@ -805,17 +807,23 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       HIPCHK(hipMemcpyAsync(c->h_int + kExtTotalInt, c->ecs + E, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipEventRecord(c->ev[8], c->stream));
       HIPCHK(wait_event(c->ev[8]));
-      const int64_t total = c->h_int[kExtTotalInt];
-      // 8 records of padding: the search reads whole groups of 8
-      if ((size_t)total + 8 > c->ext_rec_cap) {
-        if (c->erec) { (void)hipFree(c->erec); c->erec = nullptr; }
-        c->ext_rec_cap = 0;
-        const size_t cap = (size_t)total + 8 + (size_t)total / 8;
-        RCHK(dev_alloc((void **)&c->erec, sizeof(float4) * cap));
-        c->ext_rec_cap = cap;
+      int64_t total = c->h_int[kExtTotalInt];
+      // 8 records of padding: the search reads whole batches past a row's
+      // end.  The search addresses records with 32-bit byte offsets, so a
+      // larger set (over 2^28 records) takes the per-lane traversal.
+      if ((total + 8) * (int64_t)sizeof(float4) > (int64_t)UINT32_MAX) {
+        total = -1;
+      } else {
+        if ((size_t)total + 8 > c->ext_rec_cap) {
+          if (c->erec) { (void)hipFree(c->erec); c->erec = nullptr; }
+          c->ext_rec_cap = 0;
+          const size_t cap = std::min<size_t>((size_t)total + 8 + (size_t)total / 8, UINT32_MAX / sizeof(float4));
+          RCHK(dev_alloc((void **)&c->erec, sizeof(float4) * cap));
+          c->ext_rec_cap = cap;
+        }
+        hipLaunchKernelGGL(k_ext_fill, dim3(grid_for(E)), dim3(kBlock), 0, c->stream, g, xg, c->cstart, c->ecs,
+                           c->pf, E, c->erec);
       }
-      hipLaunchKernelGGL(k_ext_fill, dim3(grid_for(E)), dim3(kBlock), 0, c->stream, g, xg, c->cstart, c->ecs,
-                         c->pf, E, c->erec);
       c->ext_total = total;
       c->ext_version = c->pos_version;
       c->ext_cur = c->cur;
@@ -823,6 +831,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       c->ext_grid = g;
       c->ext_g = xg;
     }
+    if (c->ext_total < 0) use_ext = false;
   }
   c->stats.ext_records = use_ext ? c->ext_total : 0;
   // 5. neighbour lists; grow the storage and redo if any list overflowed
@@ -1336,7 +1345,7 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
     c->lane_search = t && std::strcmp(t, "lane") == 0;
   }
   if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1 ||
-      c->opt.cell_div < 0 || c->opt.cell_div > 4) {
+      c->opt.cell_div < 0 || c->opt.cell_div > 4 || c->opt.neigh_cap < 0 || c->opt.neigh_cap > (1 << 20)) {
     delete c;
     return SPH_ERR_BAD_ARG;
   }

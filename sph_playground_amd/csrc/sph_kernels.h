@@ -63,7 +63,8 @@ struct NeighArgs {
   int *err;                // error bits
   long long *stripes;      // [kStripes] pair counts, [kStripes..2k) max counts
   // extended grid: the real cells plus gh[a] ghost cells on each side of every
-  // imaged axis (ne[a] = nc[a] + 2 gh[a] cells), ecs the first record of every
+  // axis (ne[a] = nc[a] + 2 gh[a] cells; empty on a walled axis, the periodic
+  // images on an imaged one), ecs the first record of every
   // extended cell (row-major, x fastest; ne0 ne1 ne2 + 1 entries) and erec the
   // records (the per-lane traversal reads pf and cstart instead)
   const int *ecs;

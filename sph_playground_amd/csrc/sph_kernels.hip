@@ -635,20 +635,30 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
   // ------------------------------------------------ extended-grid traversal
   auto ext_rows = [&]() {
     const uint32_t own_e = nb_pack((uint32_t)s, kNoImage);
-    // every record of [t0, t1), eight at a time (a row's records, usually
-    // fewer, are read together; erec holds 8 records of padding past its
-    // end): FP32 decisions first, the exact FP64 rule for those inside the
-    // band behind one branch, then the ring writes.  OWN (the lane's own
+    // 32-bit byte offsets from the arrays' bases (the host keeps both arrays
+    // under 4 GiB), so every load takes one offset register
+    const char *const erec_b = reinterpret_cast<const char *>(a.erec);
+    const char *const ecs_b = reinterpret_cast<const char *>(a.ecs);
+    // the cursor at which 8 entries are pending; a list far past the
+    // reference's cap (at most 2^20) stops early, since the step fails on it
+    // anyway, which keeps the cursor in range
+    int flush_at = 8 * kRingRow;
+    const int stop_at = (a.maxneib + 8) * kRingRow;
+    // every record of [t0, t1), kScanBatch at a time (a row's records,
+    // usually fewer, are read together; erec holds 8 records of padding past
+    // its end): FP32 decisions first, the exact FP64 rule for those inside
+    // the band behind one branch, then the ring writes.  OWN (the lane's own
     // row) drops its own record.  No branch per candidate.
     auto scan = [&](auto own, int t0, int t1) {
       constexpr bool OWN = decltype(own)::value;
-      for (int t = t0; t < t1; t += kScanBatch) {
-        const int rem = t1 - t;
-        const float4 *p = a.erec + t;
+      uint32_t off = (uint32_t)t0 * (uint32_t)sizeof(float4);
+      for (int rem = t1 - t0; rem > 0; rem -= kScanBatch, off += kScanBatch * (uint32_t)sizeof(float4)) {
+        const float4 *p = reinterpret_cast<const float4 *>(erec_b + off);
         float4 q[kScanBatch];
 #pragma unroll
         for (int u = 0; u < kScanBatch; u++) q[u] = p[u];
-        bool hit[kScanBatch], band[kScanBatch];
+        // hit: inside the band's lower edge; near: inside its upper edge
+        bool hit[kScanBatch], near[kScanBatch];
         bool any_band = false;
 #pragma unroll
         for (int u = 0; u < kScanBatch; u++) {
@@ -656,15 +666,14 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
           const float sd32 = fx * fx + fy * fy + fz * fz;
           bool ok = u < rem;
           if (OWN) ok = ok && __float_as_uint(q[u].w) != own_e;
-          const bool in = sd32 < lo_thr;
-          hit[u] = ok && in;
-          band[u] = ok && !in && sd32 <= hi_thr;
-          any_band = any_band || band[u];
+          hit[u] = ok && sd32 < lo_thr;
+          near[u] = ok && sd32 <= hi_thr;
+          any_band = any_band || (hit[u] != near[u]);
         }
         if (any_band) {
 #pragma unroll
           for (int u = 0; u < kScanBatch; u++) {
-            if (band[u]) {
+            if (hit[u] != near[u]) {
               const uint32_t e = __float_as_uint(q[u].w);
               const double4 pj = pm[nb_index(e)];
               double xj = pj.x, yj = pj.y, zj = pj.z;
@@ -681,72 +690,80 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
           *reinterpret_cast<uint32_t *>(ring_bytes + ring_slot(cur, lane_off)) = __float_as_uint(q[u].w);
           cur += hit[u] ? kRingRow : 0;
         }
-        if ((cur >> kRingShift) - done >= 8) flush_chunk();  // at most 7 pending after it
+        if (cur >= flush_at) {  // at most 7 pending after it
+          flush_chunk();
+          flush_at += 8 * kRingRow;
+          if (cur >= stop_at) break;
+        }
       }
     };
-    // row geometry as in the per-lane traversal: FP32 cell units, margins
-    // that only add rows and cells
-    const float pcx = (float)((pi.x - a.g.lo[0]) * a.g.inv_cs[0]);
+    // Row geometry in FP32 cell units, widened by margins that only add rows
+    // and cells (as in the per-lane traversal).  x counts extended cells, in
+    // which a lane inside the grid sits at x >= gh0 >= rx, so truncation
+    // floors every bound the clamps keep.  On y and z the gap between the lane and the cells at
+    // offset d is max(d - (f + m), -d - (1 - f + m), 0) cells, f the lane's
+    // offset inside its own cell and m the margin.  Walled axes have empty
+    // ghost cells, so no row or cell needs a bounds test.
+    const float pcx = (float)((pi.x - a.g.lo[0]) * a.g.inv_cs[0] + (double)a.gh[0]);
     const float pcy = (float)((pi.y - a.g.lo[1]) * a.g.inv_cs[1]);
     const float pcz = (float)((pi.z - a.g.lo[2]) * a.g.inv_cs[2]);
+    const float fy = pcy - (float)ci[1], fz = pcz - (float)ci[2];
+    const float ylo = fy + a.mcell[1], yhi = (1.f - fy) + a.mcell[1];
+    const float zlo = fz + a.mcell[2], zhi = (1.f - fz) + a.mcell[2];
     const float csy = (float)a.g.cs[1], csz = (float)a.g.cs[2], icx = (float)a.g.inv_cs[0];
+    const float hcap = (float)(a.rx + 1);
     const int rz = DIM == 3 ? a.rz : 0, ry = DIM >= 2 ? a.ry : 0;
-    const bool perx = a.g.periodic[0] != 0;
-    // the record range of stencil row (dy, dz): ghost rows hold the wrapped
-    // images, rows beyond a non-imaged axis or farther than the radius are
-    // empty
-    auto row_range = [&](int dy, int dz, int &t0, int &t1) {
-      const int cz = ci[2] + dz, cy = ci[1] + dy;
-      bool use = true;
-      float gyz = 0.f;
-      if (DIM == 3) {
-        const float t = (float)cz;
-        const float gz = fmaxf(fmaxf(t - pcz, pcz - (t + 1.f)) - a.mcell[2], 0.f) * csz;
-        gyz = gz * gz;
-        if (!a.g.periodic[2]) use = cz >= 0 && cz < a.g.nc[2];
-      }
+    const int xlo = ci[0] + a.gh[0] - a.rx, xhi = ci[0] + a.gh[0] + a.rx;
+    auto gap = [](float d, float lo, float hi, float cs) { return fmaxf(fmaxf(d - lo, -d - hi), 0.f) * cs; };
+    // byte offset in ecs of cell 0 of row (dy, dz), stepped along the rows
+    const uint32_t row_bytes = 4u * (uint32_t)a.ne[0];
+    uint32_t rowb = (uint32_t)((ci[2] - rz + a.gh[2]) * a.ne[1] + ci[1] - ry + a.gh[1]) * row_bytes;
+    const uint32_t plane_step = (uint32_t)(a.ne[1] - 2 * ry) * row_bytes;
+    float gz2 = 0.f;
+    if (DIM == 3) {
+      const float gz = gap((float)-rz, zlo, zhi, csz);
+      gz2 = gz * gz;
+    }
+    // the record range of row (dy, dz); a row farther than the radius is empty
+    auto row_range = [&](int dy, int &t0, int &t1) {
+      float gyz = gz2;
       if (DIM >= 2) {
-        const float t = (float)cy;
-        const float gy = fmaxf(fmaxf(t - pcy, pcy - (t + 1.f)) - a.mcell[1], 0.f) * csy;
-        gyz += gy * gy;
-        if (!a.g.periodic[1]) use = use && cy >= 0 && cy < a.g.nc[1];
+        const float gy = gap((float)dy, ylo, yhi, csy);
+        gyz = fmaf(gy, gy, gz2);
       }
-      use = use && gyz <= r2g;
-      t0 = t1 = 0;
-      if (use) {
-        const float hxc =
-            fminf(__builtin_amdgcn_sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], (float)(a.rx + 1));
-        int xl = max((int)floorf(pcx - hxc), ci[0] - a.rx);
-        int xh = min((int)floorf(pcx + hxc), ci[0] + a.rx);
-        if (!perx) {
-          xl = max(xl, 0);
-          xh = min(xh, nx - 1);
-        }
-        if (xl <= xh) {
-          // cells [xl, xh] of the row: one range of records (a.rx ghost
-          // cells on each side of an imaged x axis)
-          const int base = ((cz + a.gh[2]) * a.ne[1] + cy + a.gh[1]) * a.ne[0] + a.gh[0];
-          t0 = a.ecs[base + xl];
-          t1 = a.ecs[base + xh + 1];
-        }
-      }
+      const float hxc = fminf(__builtin_amdgcn_sqrtf(fmaxf(r2g - gyz, 0.f)) * icx + a.mcell[0], hcap);
+      // both bounds inside the stencil's cells [xlo, xhi] (xh one below xl
+      // for an empty range), whatever the lane's position, so both loads
+      // stay in the row
+      const int xl = min(max((int)(pcx - hxc), xlo), xhi);
+      int xh = min(max((int)(pcx + hxc), xlo - 1), xhi);
+      if (!(gyz <= r2g)) xh = xl - 1;
+      t0 = *reinterpret_cast<const int *>(ecs_b + (rowb + 4u * (uint32_t)xl));
+      t1 = *reinterpret_cast<const int *>(ecs_b + (rowb + 4u * (uint32_t)xh + 4u));
     };
     // rows in (dz, dy) order; the next row's range is read while this row's
     // records are tested
     int dy = -ry, dz = -rz, t0n, t1n;
-    row_range(dy, dz, t0n, t1n);
+    row_range(dy, t0n, t1n);
     for (;;) {
       const int t0 = t0n, t1 = t1n;
       const bool own_row = dy == 0 && dz == 0;
       const bool last = dy == ry && dz == rz;
       if (!last) {
-        if (++dy > ry) { dy = -ry; ++dz; }
-        row_range(dy, dz, t0n, t1n);
+        if (++dy > ry) {
+          dy = -ry;
+          ++dz;
+          rowb += plane_step;
+          const float gz = gap((float)dz, zlo, zhi, csz);
+          gz2 = gz * gz;
+        } else {
+          rowb += row_bytes;
+        }
+        row_range(dy, t0n, t1n);
       }
       if (own_row) scan(std::true_type{}, t0, t1);
       else scan(std::false_type{}, t0, t1);
-      while ((cur >> kRingShift) - done >= 8) flush_chunk();
-      if (last) break;
+      if (last || cur >= stop_at) break;
     }
   };
 
@@ -1629,20 +1646,26 @@ __global__ void k_soa_to_aos(Fields F, const int64_t *orig, int n, double *aos, 
 // at or past `cap` (the output's length) are not written.
 // ---------------------------------------------------------------- extended grid
 // Extended cell k = (ez * ne1 + ey) * ne0 + ex is real cell (ex - gh0, ey - gh1,
-// ez - gh2) taken modulo the cell counts, with the box shift of the wrap.
-__device__ __forceinline__ int ext_axis(int e, int gh, int nc, int &shift) {
-  const int c = e - gh;
+// ez - gh2): on an imaged axis taken modulo the cell count, with the box
+// shift of the wrap; past the end of a walled axis it holds nothing.
+__device__ __forceinline__ bool ext_axis(int e, int gh, int nc, bool periodic, int &c, int &shift) {
+  c = e - gh;
   shift = c < 0 ? -1 : (c >= nc ? 1 : 0);
-  return c - shift * nc;
+  if (!periodic) return shift == 0;
+  c -= shift * nc;
+  return true;
 }
 
-__device__ __forceinline__ uint32_t ext_source(const Grid &g, const ExtGrid &x, int k, int sh[3]) {
+// the cell key an extended cell copies, or false for an empty ghost cell
+__device__ __forceinline__ bool ext_source(const Grid &g, const ExtGrid &x, int k, int sh[3], uint32_t &key) {
   const int ex = k % x.ne[0], r = k / x.ne[0];
   const int ey = r % x.ne[1], ez = r / x.ne[1];
-  const int cx = ext_axis(ex, x.gh[0], g.nc[0], sh[0]);
-  const int cy = ext_axis(ey, x.gh[1], g.nc[1], sh[1]);
-  const int cz = ext_axis(ez, x.gh[2], g.nc[2], sh[2]);
-  return row_of(g, cy, cz) * (uint32_t)g.nc[0] + (uint32_t)cx;
+  int cx, cy, cz;
+  const bool ux = ext_axis(ex, x.gh[0], g.nc[0], g.periodic[0] != 0, cx, sh[0]);
+  const bool uy = ext_axis(ey, x.gh[1], g.nc[1], g.periodic[1] != 0, cy, sh[1]);
+  const bool uz = ext_axis(ez, x.gh[2], g.nc[2], g.periodic[2] != 0, cz, sh[2]);
+  key = row_of(g, cy, cz) * (uint32_t)g.nc[0] + (uint32_t)cx;
+  return ux && uy && uz;
 }
 
 // records per extended cell (count[ncells] = 0 closes the exclusive scan)
@@ -1654,8 +1677,8 @@ __global__ void k_ext_count(Grid g, ExtGrid x, const int *cstart, int ncells, in
     return;
   }
   int sh[3];
-  const uint32_t key = ext_source(g, x, k, sh);
-  count[k] = cstart[key + 1] - cstart[key];
+  uint32_t key;
+  count[k] = ext_source(g, x, k, sh, key) ? cstart[key + 1] - cstart[key] : 0;
 }
 
 // Copy every extended cell's records: the source records with the image
@@ -1667,7 +1690,8 @@ __global__ void k_ext_fill(Grid g, ExtGrid x, const int *cstart, const int *ecs,
   const int k = gtid();
   if (k >= ncells) return;
   int sh[3];
-  const uint32_t key = ext_source(g, x, k, sh);
+  uint32_t key;
+  if (!ext_source(g, x, k, sh, key)) return;
   uint32_t bm = 0, bv = 0;
   image_mask(sh[0], sh[1], sh[2], bm, bv);
   const uint32_t code = (uint32_t)((sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1));
