@@ -94,3 +94,9 @@ def test_bad_arguments_rejected():
     s3 = s.copy()
     s3[6] = 2.0  # cylindrical (kernel.f90:135-137)
     assert L.sph_ctx_create(s3.ctypes.data_as(ct.POINTER(ct.c_double)), 1, None, ct.byref(h)) == 10
+    # a neighbour cap outside [0, 2^20] is refused before any device call
+    for cap in (-1, (1 << 20) + 1):
+        o = _lib.Options()
+        L.sph_default_options(ct.byref(o))
+        o.neigh_cap = cap
+        assert L.sph_ctx_create(s.ctypes.data_as(ct.POINTER(ct.c_double)), 1, ct.byref(o), ct.byref(h)) == 7

@@ -79,3 +79,15 @@ def test_extended_grid_and_per_lane_traversals_agree(gpu, name, make, kw):
     assert np.array_equal(j_a, j_b)
     assert np.array_equal(code_a, code_b)
     assert np.array_equal(out_a, out_b, equal_nan=True)
+
+
+@pytest.mark.parametrize("lane", [False, True], ids=["extended_grid", "per_lane"])
+def test_lists_past_the_cap_report_truncation(gpu, lane):
+    # 114 neighbours per particle against a cap of 16: the extended-grid
+    # traversal stops a list 8 entries past the cap, and both traversals
+    # report the reference's truncation (neighboursearch.f90:127-132)
+    from sph_playground_amd._lib import SphError
+    state, st = P.diffusion3d(32, ddw="fab")
+    with pytest.raises(SphError) as e:
+        _step(state, st, lane, kernel="m6", neigh_cap=16)
+    assert e.value.code == 1
