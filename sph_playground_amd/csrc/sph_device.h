@@ -45,6 +45,10 @@ __device__ __forceinline__ float rcp_pair(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ double rcp_pair(double x) { return 1.0 / x; }
 __device__ __forceinline__ float sqrt_pair(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sqrt_pair(double x) { return sqrt(x); }
+// a / b in the pair arithmetic: the FP32 path multiplies by the 1-ulp
+// hardware reciprocal, the FP64 path divides (one rounding, as the reference)
+__device__ __forceinline__ float div_pair(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double div_pair(double a, double b) { return a / b; }
 
 // 1/sqrt(x) in FP64 from the hardware estimate (v_rsq_f64, about 23 correct
 // bits) and two Newton-Raphson steps y <- y (3/2 - x y^2 / 2), each doubling

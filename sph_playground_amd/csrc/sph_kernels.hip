@@ -1227,7 +1227,7 @@ void k_loop2(Loop2Args<V4> a) {
     // optional packs (v, c/u/kappa, grad T, B) are gathered in the body.
     // the FP32 path's record holds m/den and T den as FP32 in its fourth slot
     constexpr bool PACK = std::is_same_v<V4, float4>;
-    struct JG { double4 pb; V4 q0; uint32_t j, code; };
+    struct JG { double4 pb; V4 q0, q1, q2, q3, q4; uint32_t j, code; };
     struct JD { double4 pb; uint3 pc; float2 sc; double tr; uint32_t j, code; };
     using J = std::conditional_t<MODE == MODE_DIFF, JD, JG>;
     // MODE_DIFF's FP32 path: the own position record
@@ -1247,8 +1247,14 @@ void k_loop2(Loop2Args<V4> a) {
           d.tr = a.trd[d.j];
         }
       } else {
+        // every pack this step reads, gathered together with the position
+        // (one round trip per neighbour)
         d.pb = a.pm[d.j];
         d.q0 = a.q0[d.j];
+        if (need_v) d.q1 = a.q1[d.j];
+        if (need_q2) d.q2 = a.q2[d.j];
+        if (need_q3) d.q3 = a.q3[d.j];
+        if (need_q4) d.q4 = a.q4[d.j];
       }
       return d;
     };
@@ -1314,7 +1320,9 @@ void k_loop2(Loop2Args<V4> a) {
       const PT nwbx = gb * rx, nwby = gb * ry, nwbz = gb * rz;
       PT vabx = 0, vaby = 0, vabz = 0, pbb = 0;
       if (need_v) {
-        const V4 t = a.q1[d.j];
+        V4 t;
+        if constexpr (MODE == MODE_GENERIC) t = d.q1;
+        else t = a.q1[d.j];
         vabx = vax - (PT)t.x; vaby = vay - (PT)t.y; vabz = vaz - (PT)t.z; pbb = (PT)t.w;
       }
       const PT vnwa = vabx * nwax + vaby * nway + vabz * nwaz;
@@ -1325,7 +1333,12 @@ void k_loop2(Loop2Args<V4> a) {
       if (MODE == MODE_GENERIC && (artts || hydro || magneto || (diff && P.ddw == 403)))
         i_rb2_ob = rcp_pair(rhob * rhob * omb);
       PT cb = 0, ub = 0, kappab = 0;
-      if (need_q2) { const V4 t = a.q2[d.j]; cb = (PT)t.x; ub = (PT)t.y; kappab = (PT)t.z; }
+      if (need_q2) {
+        V4 t;
+        if constexpr (MODE == MODE_GENERIC) t = d.q2;
+        else t = a.q2[d.j];
+        cb = (PT)t.x; ub = (PT)t.y; kappab = (PT)t.z;
+      }
       if (artts) {
         // Fab = wcv df dr / h^(dim+2) / q = wcv df / h^(dim+1)   (kernel.f90:251-259)
         const PT faba = wcv * dfa * ihd1;
@@ -1342,7 +1355,7 @@ void k_loop2(Loop2Args<V4> a) {
         dvz -= mb * (ta_ * nwaz + tb_ * nwbz);
         PT qc = 0;
         if (P.au != -1.0) {  // art_termcond, circuit2.f90:520-529
-          PT vs = sqrt(fabs(pa_ - pbb) / (PT(0.5) * (rhoa + rhob)));
+          PT vs = sqrt_pair(div_pair((PT)fabs(pa_ - pbb), PT(0.5) * (rhoa + rhob)));
           qc = (PT)P.au * (vs * (ua - ub) * PT(0.5) * (faba * i_oa_ra + fabb * (rhob * i_rb2_ob)));
         }
         du += mb * (dvr * qva * faba * i_oa_ra + qc);
@@ -1355,16 +1368,26 @@ void k_loop2(Loop2Args<V4> a) {
         du += mb * pa_ * vnwa * i_ra2_oa;
       }
       PT bbx = 0, bby = 0, bbz = 0;
-      if (need_q4) { const V4 t = a.q4[d.j]; bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z; }
+      if (need_q4) {
+        V4 t;
+        if constexpr (MODE == MODE_GENERIC) t = d.q4;
+        else t = a.q4[d.j];
+        bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z;
+      }
       if (diff) {
         Sym3<PT> ktb = ktb_fixed;
         PT gbx = 0, gby = 0, gbz = 0;
-        if (need_q3) { const V4 t = a.q3[d.j]; gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z; }
+        if (need_q3) {
+          V4 t;
+          if constexpr (MODE == MODE_GENERIC) t = d.q3;
+          else t = a.q3[d.j];
+          gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z;
+        }
         if (fld) {
-          PT gn = sqrt(gbx * gbx + gby * gby + gbz * gbz);
-          PT fr = gn / (kappab * rhob * rhob * tb);
-          PT lam = (PT(2) + fr) / (PT(6) + PT(3) * fr + fr * fr);
-          PT dfl = (PT)kLightspeed * lam / kappab / rhob;
+          PT gn = sqrt_pair(gbx * gbx + gby * gby + gbz * gbz);
+          PT fr = div_pair(gn, kappab * rhob * rhob * tb);
+          PT lam = div_pair(PT(2) + fr, PT(6) + PT(3) * fr + fr * fr);
+          PT dfl = div_pair(div_pair((PT)kLightspeed * lam, kappab), rhob);
           ktb.xx = ktb.yy = ktb.zz = dfl; ktb.xy = ktb.xz = ktb.yz = 0;
         } else if (tensor_b) {
           ktb = tensor_of_b<PT>(P, bbx, bby, bbz);
@@ -1397,7 +1420,7 @@ void k_loop2(Loop2Args<V4> a) {
           if constexpr (MODE == MODE_DIFF) {
             ddt += mb_rhob * inv_rhoa * (trb - ta * rhoa) * contr;
           }
-          else ddt += mb / rhob * inv_rhoa * (tb * rhob - ta * rhoa) * contr;
+          else ddt += div_pair(mb, rhob) * inv_rhoa * (tb * rhob - ta * rhoa) * contr;
         } else if (MODE == MODE_GENERIC) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
