@@ -28,7 +28,7 @@ namespace sph {
 __global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, long long *, int *);
 __global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
 __global__ void k_check_sorted(const uint32_t *, int, int *);
-__global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int);
+__global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
                        uint8_t *, double4 *, double *, uint3 *, FixFrame, float2 *, double *, long long *,
@@ -715,18 +715,17 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       Fields dst;
       for (int f = 0; f < F_TOTAL; f++) dst.f[f] = c->fbuf[nxt] + (size_t)f * c->npad;
       hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, src, dst, c->vals2, c->orig[c->cur],
-                         c->orig[nxt], n);
-      // auxiliary arrays ride along in groups of 32
+                         c->orig[nxt], n, (int)F_TOTAL);
+      // auxiliary arrays ride along in groups of up to 32
       for (int k0 = 0; k0 < c->naux; k0 += F_TOTAL) {
         Fields as{}, ad{};
-        int m = std::min((int)F_TOTAL, c->naux - k0);
-        for (int f = 0; f < F_TOTAL; f++) {
-          int k = k0 + std::min(f, m - 1);
-          as.f[f] = c->fbuf[c->cur] + (size_t)(F_TOTAL + k) * c->npad;
-          ad.f[f] = c->fbuf[nxt] + (size_t)(F_TOTAL + k) * c->npad;
+        const int m = std::min((int)F_TOTAL, c->naux - k0);
+        for (int f = 0; f < m; f++) {
+          as.f[f] = c->fbuf[c->cur] + (size_t)(F_TOTAL + k0 + f) * c->npad;
+          ad.f[f] = c->fbuf[nxt] + (size_t)(F_TOTAL + k0 + f) * c->npad;
         }
         hipLaunchKernelGGL(k_reorder, dim3(G), dim3(kBlock), 0, c->stream, as, ad, c->vals2, c->orig[c->cur],
-                           c->orig[nxt], n);
+                           (int64_t *)nullptr, n, m);
       }
       c->cur = nxt;
       std::swap(c->keys, c->keys2);

@@ -221,18 +221,25 @@ __global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) 
 __global__ void k_check_sorted(const uint32_t *keys, int n, int *flag) {
   int s = gtid();
   bool bad = (s + 1 < n) && keys[s] > keys[s + 1];
-  if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+  // one atomic per wave that finds a break, skipped once the flag is seen
+  // set (after particles move nearly every wave finds one, and atomics on
+  // one word serialise)
+  if (__ballot(bad) && (threadIdx.x & 63) == 0 &&
+      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(flag, 1);
 }
 
-// Gather every field into the new order: dst[s] = src[perm[s]].
+// Gather the first nf fields into the new order: dst[s] = src[perm[s]], and
+// the original indices with them when orig_dst is given.
 __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const int64_t *orig_src,
-                          int64_t *orig_dst, int n) {
+                          int64_t *orig_dst, int n, int nf) {
   int s = gtid();
   if (s >= n) return;
   uint32_t p = perm[s];
 #pragma unroll 4
-  for (int f = 0; f < F_TOTAL; f++) dst.f[f][s] = src.f[f][p];
-  orig_dst[s] = orig_src[p];
+  for (int f = 0; f < F_TOTAL; f++)
+    if (f < nf) dst.f[f][s] = src.f[f][p];
+  if (orig_dst) orig_dst[s] = orig_src[p];
 }
 
 // First sorted index of every cell key (cstart[K] = number of particles with
