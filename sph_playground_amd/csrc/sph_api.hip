@@ -704,7 +704,8 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   if (!keys_kept) {
     hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
     HIPCHK(hipMemsetAsync(c->d_int + 2, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_check_sorted, dim3(G), dim3(kBlock), 0, c->stream, c->keys, n, c->d_int + 2);
+    hipLaunchKernelGGL(k_check_sorted, dim3(std::min(G, 256u)), dim3(kBlock), 0, c->stream, c->keys, n,
+                       c->d_int + 2);
     RCHK(read_ints(c, 2, 1));
     if (c->h_int[2]) {
       size_t bytes = c->sort_tmp_bytes;
@@ -1007,6 +1008,12 @@ static void launch_loop2_d(sph_ctx *c, int dim, void *args) {
   else if (dim == 2) launch_loop2_k<2, MODE>(c, args);
   else launch_loop2_k<3, MODE>(c, args);
 }
+// modes built in 2D and 3D only
+template <int MODE>
+static void launch_loop2_d23(sph_ctx *c, int dim, void *args) {
+  if (dim == 2) launch_loop2_k<2, MODE>(c, args);
+  else launch_loop2_k<3, MODE>(c, args);
+}
 
 template <typename V4>
 static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
@@ -1073,7 +1080,14 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   // fab with one tensor for every pair and no velocity term: the loop
   // without run-time choices
   const bool fab = !generic && P.ddw == 400 && q1 == nullptr && q4 == nullptr;
-  if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
+  // the generic loop with its equation set fixed at compile time where one
+  // is built for it (2D and 3D: hydro, magnetohydro, hydro with FLD)
+  const int terms = (P.hydro ? 1 : 0) | (P.magneto ? 2 : 0) | (P.diff ? 4 : 0) | (P.fld ? 8 : 0);
+  if (generic && P.dim >= 2 && terms == kSpecHydro) launch_loop2_d23<MODE_SPEC + kSpecHydro>(c, P.dim, &a);
+  else if (generic && P.dim >= 2 && terms == kSpecMhd) launch_loop2_d23<MODE_SPEC + kSpecMhd>(c, P.dim, &a);
+  else if (generic && P.dim >= 2 && terms == kSpecHydroFld)
+    launch_loop2_d23<MODE_SPEC + kSpecHydroFld>(c, P.dim, &a);
+  else if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
   else if (fab) launch_loop2_d<MODE_FAB>(c, P.dim, &a);
   else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
   HIPCHK(hipEventRecord(c->ev[6], c->stream));

@@ -44,7 +44,13 @@ constexpr int kStripes = 64;
 // hydro/AV/MHD/FLD terms, the density+force step of the diffusion workloads.
 // MODE_FAB: MODE_DIFF with the fab operator, one tensor for every pair and
 // no velocity term (k_loop2_fab).
-enum { MODE_GENERIC = 0, MODE_DIFF = 1, MODE_FAB = 2 };  // counters are striped over 64 slots to avoid same-address atomics
+enum { MODE_GENERIC = 0, MODE_DIFF = 1, MODE_FAB = 2 };
+// MODE_SPEC + mask: MODE_GENERIC with its equation set fixed at compile time,
+// mask = hydro | magneto << 1 | diff << 2 | fld << 3 (terms outside the set
+// are dead code, so the kernel holds fewer registers); built in 2D and 3D for
+// hydro (1), magnetohydro (3) and hydro with flux-limited diffusion (13)
+constexpr int MODE_SPEC = 16;
+constexpr int kSpecHydro = 1, kSpecMhd = 3, kSpecHydroFld = 13;  // counters are striped over 64 slots to avoid same-address atomics
 
 // Positions, search records, lists and counts travel as separate
 // __restrict__ kernel arguments.

@@ -218,15 +218,13 @@ __global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) 
   vals[s] = (uint32_t)s;
 }
 
-__global__ void k_check_sorted(const uint32_t *keys, int n, int *flag) {
-  int s = gtid();
-  bool bad = (s + 1 < n) && keys[s] > keys[s + 1];
-  // one atomic per wave that finds a break, skipped once the flag is seen
-  // set (after particles move nearly every wave finds one, and atomics on
-  // one word serialise)
-  if (__ballot(bad) && (threadIdx.x & 63) == 0 &&
-      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-    atomicOr(flag, 1);
+// Grid-stride over few blocks, one atomic per block that found a break
+// (after particles move nearly every wave finds one, and atomics on one word
+// serialise: one per wave took 130 us for a million particles).
+__global__ __launch_bounds__(256) void k_check_sorted(const uint32_t *keys, int n, int *flag) {
+  bool bad = false;
+  for (int s = gtid(); s + 1 < n; s += gridDim.x * blockDim.x) bad = bad || keys[s] > keys[s + 1];
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
 // Gather the first nf fields into the new order: dst[s] = src[perm[s]], and
@@ -1176,18 +1174,25 @@ void k_loop2(Loop2Args<V4> a) {
   if (s < a.n && a.real[s]) {
     using K = KernelFn<KT, PT>;
     const Phys &P = a.p;
-    const bool hydro = MODE == MODE_GENERIC && P.hydro, artts = MODE == MODE_GENERIC && P.artts;
-    const bool magneto = MODE == MODE_GENERIC && P.magneto, fld = MODE == MODE_GENERIC && P.fld;
-    const bool diff = MODE == MODE_DIFF || P.diff;
+    // GEN: every term under its flag (MODE_GENERIC), or the equation set
+    // fixed at compile time (MODE_SPEC + mask)
+    constexpr bool SPEC = MODE >= MODE_SPEC;
+    constexpr bool GEN = MODE == MODE_GENERIC || SPEC;
+    constexpr int TM = SPEC ? MODE - MODE_SPEC : 0;
+    const bool hydro = SPEC ? (TM & 1) != 0 : (MODE == MODE_GENERIC && P.hydro);
+    const bool magneto = SPEC ? (TM & 2) != 0 : (MODE == MODE_GENERIC && P.magneto);
+    const bool diff = SPEC ? (TM & 4) != 0 : (MODE == MODE_DIFF || P.diff);
+    const bool fld = SPEC ? (TM & 8) != 0 : (MODE == MODE_GENERIC && P.fld);
+    const bool artts = GEN && P.artts;
     // optional j-side packs; a pack the host did not build is never read
     const bool need_v = (P.adden || artts || hydro || magneto) && (a.v_nonzero || artts || hydro || magneto) &&
                         a.q1 != nullptr;
-    const bool need_q2 = MODE == MODE_GENERIC && ((diff && fld) || artts) && a.q2 != nullptr;
-    const bool need_q3 = MODE == MODE_GENERIC && diff && (fld || P.ddw == 403 || P.ddw == 404) && a.q3 != nullptr;
+    const bool need_q2 = GEN && ((diff && fld) || artts) && a.q2 != nullptr;
+    const bool need_q3 = GEN && diff && (fld || P.ddw == 403 || P.ddw == 404) && a.q3 != nullptr;
     const bool tensor_b = diff && !fld && P.diso == 0 && !a.b_uniform && a.q4 != nullptr;
     const bool need_q4 = (magneto || tensor_b) && a.q4 != nullptr;
     const bool hess = diff && (P.ddw == 400 || P.ddw == 401 || P.ddw == 402);
-    const bool need_dfb = MODE == MODE_GENERIC && (hydro || artts || magneto || P.ddw == 403);
+    const bool need_dfb = GEN && (hydro || artts || magneto || P.ddw == 403);
     const double4 pa = a.pm[s];
     // i-side values straight from the FP64 fields (coalesced, read once)
     const double *const *Fi = a.out.f;
@@ -1328,7 +1333,7 @@ void k_loop2(Loop2Args<V4> a) {
       PT vabx = 0, vaby = 0, vabz = 0, pbb = 0;
       if (need_v) {
         V4 t;
-        if constexpr (MODE == MODE_GENERIC) t = d.q1;
+        if constexpr (GEN) t = d.q1;
         else t = a.q1[d.j];
         vabx = vax - (PT)t.x; vaby = vay - (PT)t.y; vabz = vaz - (PT)t.z; pbb = (PT)t.w;
       }
@@ -1337,12 +1342,12 @@ void k_loop2(Loop2Args<V4> a) {
       // j-side 1/(rho_b^2 om_b), once per pair where a term divides by it
       // (FP64: a division, so rho_b om_b = 0 still gives inf)
       PT i_rb2_ob = 0;
-      if (MODE == MODE_GENERIC && (artts || hydro || magneto || (diff && P.ddw == 403)))
+      if (GEN && (artts || hydro || magneto || (diff && P.ddw == 403)))
         i_rb2_ob = rcp_pair(rhob * rhob * omb);
       PT cb = 0, ub = 0, kappab = 0;
       if (need_q2) {
         V4 t;
-        if constexpr (MODE == MODE_GENERIC) t = d.q2;
+        if constexpr (GEN) t = d.q2;
         else t = a.q2[d.j];
         cb = (PT)t.x; ub = (PT)t.y; kappab = (PT)t.z;
       }
@@ -1377,7 +1382,7 @@ void k_loop2(Loop2Args<V4> a) {
       PT bbx = 0, bby = 0, bbz = 0;
       if (need_q4) {
         V4 t;
-        if constexpr (MODE == MODE_GENERIC) t = d.q4;
+        if constexpr (GEN) t = d.q4;
         else t = a.q4[d.j];
         bbx = (PT)t.x; bby = (PT)t.y; bbz = (PT)t.z;
       }
@@ -1386,7 +1391,7 @@ void k_loop2(Loop2Args<V4> a) {
         PT gbx = 0, gby = 0, gbz = 0;
         if (need_q3) {
           V4 t;
-          if constexpr (MODE == MODE_GENERIC) t = d.q3;
+          if constexpr (GEN) t = d.q3;
           else t = a.q3[d.j];
           gbx = (PT)t.x; gby = (PT)t.y; gbz = (PT)t.z;
         }
@@ -1428,7 +1433,7 @@ void k_loop2(Loop2Args<V4> a) {
             ddt += mb_rhob * inv_rhoa * (trb - ta * rhoa) * contr;
           }
           else ddt += div_pair(mb, rhob) * inv_rhoa * (tb * rhob - ta * rhoa) * contr;
-        } else if (MODE == MODE_GENERIC) {
+        } else if (GEN) {
           PT kdbx = ktb.xx * gbx + ktb.xy * gby + ktb.xz * gbz;
           PT kdby = ktb.xy * gbx + ktb.yy * gby + ktb.yz * gbz;
           PT kdbz = ktb.xz * gbx + ktb.yz * gby + ktb.zz * gbz;
@@ -1945,6 +1950,14 @@ __global__ void k_axpy_copy(double *dst, const double *x, const double *y, doubl
 SPH_INST_DK(1, 0) SPH_INST_DK(1, 1) SPH_INST_DK(1, 2)
 SPH_INST_DK(2, 0) SPH_INST_DK(2, 1) SPH_INST_DK(2, 2)
 SPH_INST_DK(3, 0) SPH_INST_DK(3, 1) SPH_INST_DK(3, 2)
+
+#define SPH_INST_SPEC1(DIM, KT, M)                                                                    \
+  template __global__ void k_loop2<DIM, KT, double, double4, MODE_SPEC + M>(Loop2Args<double4>); \
+  template __global__ void k_loop2<DIM, KT, float, float4, MODE_SPEC + M>(Loop2Args<float4>);
+#define SPH_INST_SPEC(DIM, KT) \
+  SPH_INST_SPEC1(DIM, KT, kSpecHydro) SPH_INST_SPEC1(DIM, KT, kSpecMhd) SPH_INST_SPEC1(DIM, KT, kSpecHydroFld)
+SPH_INST_SPEC(2, 0) SPH_INST_SPEC(2, 1) SPH_INST_SPEC(2, 2)
+SPH_INST_SPEC(3, 0) SPH_INST_SPEC(3, 1) SPH_INST_SPEC(3, 2)
 
 #define SPH_INST_NEIGH(DIM)                                                                              \
   template __global__ void k_neigh<DIM, true>(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, \
