@@ -163,6 +163,10 @@ struct sph_ctx {
   // traversal and builds no extended grid (the lists are the same; tests
   // compare the two)
   bool lane_search = false;
+  // SPH_LOOP2_GENERIC=1 at creation: loop 2 takes the kernel with every term
+  // under its run-time flag, never a compile-time equation set (tests compare
+  // the two)
+  bool loop2_generic = false;
   // the search's extended grid (ghost cells of the imaged axes): cell counts,
   // cell starts, records; valid for the positions, order and grid it was
   // built from
@@ -1082,7 +1086,7 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   const bool fab = !generic && P.ddw == 400 && q1 == nullptr && q4 == nullptr;
   // the generic loop with its equation set fixed at compile time where one
   // is built for it (2D and 3D: hydro, magnetohydro, hydro with FLD)
-  const int terms = (P.hydro ? 1 : 0) | (P.magneto ? 2 : 0) | (P.diff ? 4 : 0) | (P.fld ? 8 : 0);
+  const int terms = c->loop2_generic ? -1 : (P.hydro ? 1 : 0) | (P.magneto ? 2 : 0) | (P.diff ? 4 : 0) | (P.fld ? 8 : 0);
   if (generic && P.dim >= 2 && terms == kSpecHydro) launch_loop2_d23<MODE_SPEC + kSpecHydro>(c, P.dim, &a);
   else if (generic && P.dim >= 2 && terms == kSpecMhd) launch_loop2_d23<MODE_SPEC + kSpecMhd>(c, P.dim, &a);
   else if (generic && P.dim >= 2 && terms == kSpecHydroFld)
@@ -1356,6 +1360,8 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   {
     const char *t = std::getenv("SPH_NEIGH_TRAVERSAL");
     c->lane_search = t && std::strcmp(t, "lane") == 0;
+    const char *g = std::getenv("SPH_LOOP2_GENERIC");
+    c->loop2_generic = g && std::strcmp(g, "1") == 0;
   }
   if (c->opt.precision < 0 || c->opt.precision > 1 || c->opt.tensor_mode < 0 || c->opt.tensor_mode > 1 ||
       c->opt.cell_div < 0 || c->opt.cell_div > 4 || c->opt.neigh_cap < 0 || c->opt.neigh_cap > (1 << 20)) {
