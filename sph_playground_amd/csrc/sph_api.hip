@@ -1086,7 +1086,8 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   const bool fab = !generic && P.ddw == 400 && q1 == nullptr && q4 == nullptr;
   // the generic loop with its equation set fixed at compile time where one
   // is built for it (2D and 3D: hydro, magnetohydro, hydro with FLD)
-  const int terms = c->loop2_generic ? -1 : (P.hydro ? 1 : 0) | (P.magneto ? 2 : 0) | (P.diff ? 4 : 0) | (P.fld ? 8 : 0);
+  const int terms =
+      c->loop2_generic ? -1 : (P.hydro ? 1 : 0) | (P.magneto ? 2 : 0) | (P.diff ? 4 : 0) | (P.fld ? 8 : 0);
   if (generic && P.dim >= 2 && terms == kSpecHydro) launch_loop2_d23<MODE_SPEC + kSpecHydro>(c, P.dim, &a);
   else if (generic && P.dim >= 2 && terms == kSpecMhd) launch_loop2_d23<MODE_SPEC + kSpecMhd>(c, P.dim, &a);
   else if (generic && P.dim >= 2 && terms == kSpecHydroFld)

@@ -705,10 +705,11 @@ __global__ __launch_bounds__(256) SPH_NEIGH_OCC void k_neigh(NeighArgs a, const 
     // Row geometry in FP32 cell units, widened by margins that only add rows
     // and cells (as in the per-lane traversal).  x counts extended cells, in
     // which a lane inside the grid sits at x >= gh0 >= rx, so truncation
-    // floors every bound the clamps keep.  On y and z the gap between the lane and the cells at
-    // offset d is max(d - (f + m), -d - (1 - f + m), 0) cells, f the lane's
-    // offset inside its own cell and m the margin.  Walled axes have empty
-    // ghost cells, so no row or cell needs a bounds test.
+    // floors every bound the clamps keep.  On y and z the gap between the
+    // lane and the cells at offset d is max(d - (f + m), -d - (1 - f + m), 0)
+    // cells, f the lane's offset inside its own cell and m the margin.
+    // Walled axes have empty ghost cells, so no row or cell needs a bounds
+    // test.
     const float pcx = (float)((pi.x - a.g.lo[0]) * a.g.inv_cs[0] + (double)a.gh[0]);
     const float pcy = (float)((pi.y - a.g.lo[1]) * a.g.inv_cs[1]);
     const float pcz = (float)((pi.z - a.g.lo[2]) * a.g.inv_cs[2]);
