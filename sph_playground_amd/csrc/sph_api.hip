@@ -759,8 +759,11 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
                         c->prep_n == n && same_grid(g, c->prep_grid) &&
                         std::memcmp(&ff, &c->ff, sizeof(ff)) == 0;
   c->ff = ff;
+  // the start-state copy only for c1simple (a deferred search is
+  // c1advanced's, whose first sweep reads the fields)
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
-                     c->pm, c->h0, c->pf, c->real, c->st[0], c->tr[0], c->pfix, ff, c->s1[0], c->eps,
+                     c->pm, c->h0, c->pf, c->real, defer ? nullptr : c->st[0], c->tr[0], c->pfix, ff, c->s1[0],
+                     c->eps,
                      c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
   c->prep_version = c->pos_version;
   c->prep_cur = c->cur;
@@ -966,6 +969,13 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
   for (int it = first; it <= last; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
+    // the first sweep reads its start state from the fields (unchanged until
+    // the tail's k_post1, which a redone search leaves idle)
+    const bool s1st = it == 1;
+    a.f_h = s1st ? F.f[F_H] : nullptr;
+    a.f_den = s1st ? F.f[F_DEN] : nullptr;
+    a.f_om = s1st ? F.f[F_OM] : nullptr;
+    a.f_t = s1st ? F.f[F_T] : nullptr;
     a.trc = c->tr[(it - 1) & 1];
     a.trn = c->tr[it & 1];
     a.s1c = c->s1[(it - 1) & 1];

@@ -131,6 +131,8 @@ struct Loop1Args {
   const uint32_t *nb;
   const int *cnt;
   const V4 *cur;           // h, den, om, T of sweep k
+  // sweep 1 only: the fields it starts from (h, den, om, T; null later)
+  const double *f_h, *f_den, *f_om, *f_t;
   V4 *next;                // sweep k+1
   const double *trc;       // T den of sweep k (the only j-side value grad T needs)
   double *trn;             // T den of sweep k+1

@@ -270,7 +270,10 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *keys, int n,
 // Positions/masses, search h, the FP32 search record (coordinates relative
 // to the grid origin + ghost-band bits), ghost-band codes (BC.f90:196-309: strict
 // padding test for a particle's own band, the looser one a chained ghost
-// gets, BC.f90:343-353), loop-1 start state (circuit1.f90:105-109).
+// gets, BC.f90:343-353), h at entry and T den for loop 1 (circuit1.f90:105-109),
+// and c1simple's start state when st is given (c1advanced's first sweep reads
+// h, den, om and T from the fields and takes every real particle as active,
+// so eps needs no start value).
 // pos = 0: the position records (pm, real, pf, pfix, band codes) of the
 // previous prep still hold (same positions, order and grid) and are not
 // rewritten.
@@ -281,8 +284,9 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
   double h = F.f[F_H][s];
   h0[s] = h;
   bool r = orig[s] < n_real;
-  st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
-  eps[s] = r ? 1.0 : 0.0;
+  // the start state for c1simple (c1advanced's first sweep reads the fields)
+  if (st) st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
+  (void)eps;
   // T den for grad T: FP64 in tr, or with m in FP32 on the mixed path
   if (s1) s1[s] = make_float2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s]));
   else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
@@ -838,8 +842,13 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
   }
   int s = gtid();
   bool inb = s < a.n;
-  double4 c = inb ? a.cur[s] : make_double4(0, 0, 0, 0);
-  bool act = inb && a.real[s] && a.eps[s] > a.tol;
+  // the first sweep starts from the fields themselves (h, den, om, T as the
+  // step found them) with every real particle active; later sweeps from the
+  // previous sweep's state and residuals
+  const bool first = a.f_h != nullptr;
+  double4 c = make_double4(0, 0, 0, 0);
+  if (inb) c = first ? make_double4(a.f_h[s], a.f_den[s], a.f_om[s], a.f_t[s]) : a.cur[s];
+  bool act = inb && a.real[s] && (first || a.eps[s] > a.tol);
   bool still = false;
   int nn = 0;
   // the FP32 path gathers 20 bytes per pair: the 12-byte position record
