@@ -533,8 +533,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic and per-particle sums (f64 positions, separations, Newton update, "
-                 "energy sum)",
+        "dtype": "f32 pair arithmetic and per-particle sums (separations from 31-bit fixed-point positions, "
+                 "f64 Newton update and energy sum)",
         "data": "synthetic (seeded lattice, T = sin sin sin)",
         "config": {"workload": "3D anisotropic diffusion density+force step, %d^3 = %d particles, M6, fab, "
                                "kappa_par=1 kappa_perp=1e-3 b=(1,1,0)/sqrt2, periodic unit cube" % (args.size, n),
