@@ -106,8 +106,9 @@ def run_path(state, st, precision, steps, warmup, torch):
         eng.step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    # the per-kernel breakdown (the library's HIP events) from a few more
-    # steps of the same run, outside the timed region
+    # the per-kernel breakdown (the library's HIP events, recorded only with
+    # timing on) from a few more steps of the same run, outside the timed region
+    eng.set_timing(True)
     per = []
     for _ in range(min(steps, 5)):
         eng.step()

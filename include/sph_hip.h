@@ -64,7 +64,8 @@ typedef struct {
   int sorted;         /* 1 if the last step had to reorder particles */
   double energy;      /* FP64 energy sum of loop 2 (circuit2.f90:490-492); the
                          reference computes and discards it */
-  double ms_neigh, ms_loop1, ms_loop2, ms_total; /* device time of the last step's phases */
+  double ms_neigh, ms_loop1, ms_loop2, ms_total; /* device time of the last step's phases
+                                                    (0 unless sph_set_timing turned timing on) */
   double ms_k_neigh, ms_k_loop1, ms_k_loop2;     /* device time of the neighbour kernel, all
                                                     Newton sweep kernels, the loop-2 kernel */
   int64_t ext_records;  /* records of the search's extended grid (real cells plus the periodic
@@ -212,6 +213,14 @@ int sph_batch_get_info(sph_batch *batch, sph_batch_info *info);
 int sph_batch_offsets(sph_batch *batch, int64_t *off);
 
 int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
+
+/* No counterpart in the reference (this library's own profiling): on = 1 makes
+ * every later step record HIP events around its phases and kernels, so that
+ * sph_get_stats reports their device times (the ms_* fields); on = 0, the
+ * default, leaves those fields 0.  Each event leaves the stream idle for a few
+ * microseconds between the kernels it separates (about 40 us per 128^3 step).
+ * Any other value returns SPH_ERR_BAD_ARG. */
+int sph_set_timing(sph_ctx *ctx, int on);
 const char *sph_error_string(int code);
 
 #ifdef __cplusplus

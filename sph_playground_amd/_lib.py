@@ -15,7 +15,7 @@ EXPORTS = (
     "sph_get_stats", "sph_error_string", "sph_diff_prepare", "sph_diff_apply", "sph_diff_apply_adjoint",
     "sph_diff_forward", "sph_diff_backward", "sph_rad_exchange", "sph_batch_create", "sph_batch_destroy",
     "sph_batch_sts", "sph_batch_get_info", "sph_batch_offsets", "sph_neighbours_device_noalloc",
-    "sph_prerun_check",
+    "sph_prerun_check", "sph_set_timing",
 )
 
 SPH_OK = 0
@@ -84,6 +84,7 @@ def lib():
     L.sph_rad_exchange.argtypes = [P, D, ct.c_double]
     L.sph_sts_coeffs.argtypes = [ct.c_int, ct.c_int, D]
     L.sph_get_stats.argtypes = [P, ct.POINTER(Stats)]
+    L.sph_set_timing.argtypes = [P, ct.c_int]
     L.sph_diff_prepare.argtypes = [P, D]
     L.sph_batch_create.argtypes = [ct.POINTER(P), ct.c_int, D, ct.c_int, ct.POINTER(P)]
     L.sph_batch_destroy.argtypes = [P]

@@ -52,6 +52,7 @@ def tune_cells(problems, reps=3, cell_divs=CELL_DIVS, device=0):
         for d in cell_divs:
             eng = Engine(state, kernel="m6", tensor="full", kappa_perp=kperp, cell_div=d, device=device)
             eng.load(st)
+            eng.set_timing(True)
             eng.step()
             ms = []
             for _ in range(reps):

@@ -116,19 +116,19 @@ namespace {
 
 constexpr int kBlock = 256;
 // Per-step counters in d_stripes (long long slots, kStripes per value): the
-// search's pair counts and list lengths, loop 1's active counts and pairs of
-// odd sweeps, k_post1's h_max, the error words (ints), k_prep's v / B flags,
-// loop 1's counters of even sweeps.  k_prep zeroes [0, kZeroLL) as it starts
-// a step, so the step's first search, first sweep, k_post1 and loop 2 find
-// their counters cleared without a fill of their own.
+// search's pair counts and list lengths, k_step_close's record (energy bits,
+// loop 2's error bits, h_max), k_post1's h_max, the error words (ints),
+// k_prep's v / B flags; loop 1's per-sweep regions follow at kStepLL.
+// k_prep zeroes [0, kZeroLL) as it starts a step, so the step's first
+// search, first sweep, k_post1 and loop 2 find their counters cleared
+// without a fill of their own.
 constexpr int kSearchLL = 0;
-constexpr int kSweepOddLL = 2 * kStripes;
+constexpr int kCloseRec = 2 * kStripes;
 constexpr int kHmaxLL = 4 * kStripes;
 constexpr int kErrLL = 5 * kStripes;      // ints: 0 search, 1 loop 1, 2 loop 2
 constexpr int kZeroLL = kErrLL + 4;
 constexpr int kFlagSlots = 6 * kStripes;
-constexpr int kSweepEvenLL = 7 * kStripes;
-constexpr int kStepLL = 9 * kStripes;     // d_stripes size
+constexpr int kStepLL = 9 * kStripes;     // the counters before the sweep regions
 enum { kErrSearch = 0, kErrLoop1 = 1, kErrLoop2 = 2 };
 constexpr int kAux = 16;  // auxiliary per-particle arrays permuted with the fields
 
@@ -220,7 +220,8 @@ struct sph_ctx {
   // small device scratch: [0..7] long long reductions, then ints
   long long *d_red = nullptr;
   int *d_int = nullptr;  // 0 err, 1 maxcnt, 2 sorted flag, 3 pos, 8.. sweep counters (max_sweeps+2)
-  long long *d_stripes = nullptr;         // striped counters (kStripes slots per value)
+  long long *d_stripes = nullptr;         // striped counters (kStripes slots per value), sweep regions
+  long long *h_rec = nullptr;             // pinned mirror of d_stripes and the sweep regions
   int *h_int = nullptr;                   // pinned mirror of d_int
   long long *h_ll = nullptr;              // pinned scratch for small host<->device values (2048)
   double *d_staging = nullptr;
@@ -249,7 +250,8 @@ struct sph_ctx {
   Grid keys_grid{};
   int64_t keys_n = -1, keys_nkeys = -1;
   // per-sweep counters of loop 1 (still-active stripes, pair stripes, a
-  // ran flag), one region per sweep of a step, and their pinned mirror
+  // ran flag), one region per sweep of a step, and their pinned mirror: the
+  // parts of d_stripes and h_rec from kStepLL on
   long long *d_sweep = nullptr, *h_sweep = nullptr;
   int sweep_hint = 4;             // sweeps the next step launches before it synchronises
   sph_stats stats{};
@@ -259,6 +261,7 @@ struct sph_ctx {
   // predictor, as every pass after main.f90:219's initial iterate does
   bool derivs_ready = false;
   hipEvent_t ev[11]{};            // 8: the short waits of read_ints; 9, 10: a batch of Newton sweeps
+  bool timing = false;            // record the phase and kernel events (all but 8) for the stats' times
   // differentiable diffusion (sph_diff_*)
   bool diff_ready = false;
   Phys diff_phys{};
@@ -440,7 +443,15 @@ static void tolerate(hipError_t e) {
   if (e != hipSuccess) (void)hipGetLastError();
 }
 
+// The events around a step's phases and kernels, recorded only with timing
+// on: every record leaves the stream idle for a few microseconds between the
+// two kernels it separates (about 40 us over a 128^3 step's ten of them).
+static hipError_t mark(sph_ctx *c, int k) {
+  return c->timing ? hipEventRecord(c->ev[k], c->stream) : hipSuccess;
+}
+
 static float elapsed(sph_ctx *c, hipEvent_t a, hipEvent_t b) {
+  if (!c->timing) return 0.f;
   float ms = 0.f;
   hipError_t e = hipEventSynchronize(b);
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
@@ -501,10 +512,6 @@ static const int kNeighRS = SPH_NEIGH_RS;
 
 // d_int slot k_wrap_reduce sets when it moved a particle back into the box
 constexpr int kWrapFlag = 4;
-// d_red slots of k_step_close's record (energy bits, loop-2 error bits, h_max)
-// and their pinned h_ll slots
-constexpr int kCloseRed = 9 * kStripes;
-constexpr int kCloseLL = 1536;
 
 // The mixed path's position-record frame: an imaged axis maps its box length
 // L onto 2^(31-k), k the least with 2^k L beyond any listed separation along
@@ -543,14 +550,21 @@ static bool same_grid(const Grid &a, const Grid &b) {
   return a.mdim == b.mdim && a.ty == b.ty;
 }
 
-// The search's read-back (pair counts, list lengths, k_prep's flags and the
-// error bits) can be left to a later synchronisation: copies to these pinned
-// slots, taken by finish_neighbours.
-constexpr int kNeighLL = 1024;   // h_ll slots of the search's read-back ([0, kFlagSlots + kStripes) of d_stripes)
 // loop 1's per-sweep counter regions: [0, kStripes) still active,
 // [kStripes, 2 kStripes) pairs, [2 kStripes] ran; region k for sweep k
 constexpr int kSweepRegion = 2 * kStripes + 8;
 constexpr int kSweepRegions = 242;  // sweeps 1..240 (the cap on max_sweeps) and slack
+constexpr int kRecLL = kStepLL + kSweepRegions * kSweepRegion;  // d_stripes and h_rec size
+
+// The step's read-back, one copy into the pinned mirror h_rec: d_stripes'
+// counters (the search's pair counts and list lengths, k_prep's flags, the
+// error words, k_step_close's record) and the regions of sweeps 1..last_sweep
+// (none for 0).  The search's part can wait for a later synchronisation (the
+// step's last), as loop 1 and loop 2 write only their own words of it.
+static hipError_t copy_record(sph_ctx *c, int last_sweep) {
+  const size_t ll = kStepLL + (last_sweep > 0 ? (size_t)(last_sweep + 1) * kSweepRegion : 0);
+  return hipMemcpyAsync(c->h_rec, c->d_stripes, sizeof(long long) * ll, hipMemcpyDeviceToHost, c->stream);
+}
 constexpr int kWhichInt = 6;        // d_int slot: parity of the last sweep that ran
 constexpr int kExtTotalInt = 16;    // h_int slot: records of the extended grid
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
@@ -581,7 +595,7 @@ static int finish_neighbours(sph_ctx *c, const long long *ll, int bits) {
 
 // The deferred read-back after a synchronisation the caller made.
 static int take_neighbours_now(sph_ctx *c) {
-  const long long *ll = c->h_ll + kNeighLL;
+  const long long *ll = c->h_rec;
   return finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
 }
 
@@ -589,13 +603,13 @@ static int take_neighbours_now(sph_ctx *c) {
 static int take_neighbours(sph_ctx *c) {
   if (!c->neigh_pending) return SPH_OK;
   c->neigh_pending = false;
-  const long long *ll = c->h_ll + kNeighLL;
+  const long long *ll = c->h_rec;
   return finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
 }
 
-// defer: launch the search and its read-back copies without waiting; the
-// lists then hold at most nb_cap entries per particle, so what runs on them
-// before take_neighbours stays in bounds.
+// defer: launch the search without waiting for its read-back (a later
+// copy_record brings it); the lists then hold at most nb_cap entries per
+// particle, so what runs on them before take_neighbours stays in bounds.
 static int build_neighbours(sph_ctx *c, bool defer = false) {
   const int n = (int)c->n;
   const int dim = c->frozen.dim;
@@ -869,7 +883,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       a.lenf[d] = g.periodic[d] ? (float)(c->box.emax[d] - c->box.emin[d]) : 0.f;
     a.maxneib = c->maxneib; a.err = step_err(c) + kErrSearch; a.stripes = c->d_stripes + kSearchLL;
     // the search's own events (loop 1 reuses 5 and 6 before a deferred read-back is taken)
-    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(mark(c, 4));
 #define NEIGH(D)                                                                                              \
   if (use_ext) hipLaunchKernelGGL((k_neigh<D, true>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt); \
   else hipLaunchKernelGGL((k_neigh<D, false>), dim3(G), dim3(kBlock), 0, c->stream, a, c->pm, c->pf, c->nb, c->cnt)
@@ -877,17 +891,17 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
     else if (dim == 2) { NEIGH(2); }
     else { NEIGH(3); }
 #undef NEIGH
-    HIPCHK(hipEventRecord(c->ev[7], c->stream));
-    // pair counts, list lengths, error words and k_prep's flags in one read-back
-    HIPCHK(hipMemcpyAsync(c->h_ll + kNeighLL, c->d_stripes, sizeof(long long) * (kFlagSlots + kStripes),
-                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(mark(c, 7));
+    // pair counts, list lengths, error words and k_prep's flags: deferred,
+    // they come back with the step's next record copy
     if (defer) {
       c->neigh_pending = true;
       return SPH_OK;
     }
+    HIPCHK(copy_record(c, 0));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     HIPCHK(wait_event(c->ev[8]));
-    const long long *ll = c->h_ll + kNeighLL;
+    const long long *ll = c->h_rec;
     const int fin = finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
     if (fin != kRedoNeigh) return fin;
   }
@@ -930,11 +944,11 @@ static int run_loop1_simple(sph_ctx *c, const Phys &P) {
   const int grad = (P.diff && P.ddw == 403) ? 1 : ((P.diff && P.ddw == 404) ? 2 : 0);
   a.cur = c->st[0];
   a.next = c->st[1];
-  HIPCHK(hipEventRecord(c->ev[5], c->stream));
+  HIPCHK(mark(c, 5));
   if (P.dim == 1) launch_loop1_k<1>(c, a, grad, true, F);
   else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, true, F);
   else launch_loop1_k<3>(c, a, grad, true, F);
-  HIPCHK(hipEventRecord(c->ev[6], c->stream));
+  HIPCHK(mark(c, 6));
   c->final_st = c->st[0];
   c->stats.sweeps = 0;
   HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -965,7 +979,7 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
   const int nreg = (last - first + 1) * kSweepRegion;
   hipLaunchKernelGGL(k_zero_ll, dim3((nreg + 255) / 256), dim3(256), 0, c->stream,
                      c->d_sweep + (size_t)first * kSweepRegion, nreg);
-  tolerate(hipEventRecord(c->ev[9], c->stream));
+  tolerate(mark(c, 9));
   for (int it = first; it <= last; it++) {
     a.cur = c->st[(it - 1) & 1];
     a.next = c->st[it & 1];
@@ -989,7 +1003,7 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
     else if (P.dim == 2) launch_loop1_k<2>(c, a, grad, false, F);
     else launch_loop1_k<3>(c, a, grad, false, F);
   }
-  tolerate(hipEventRecord(c->ev[10], c->stream));
+  tolerate(mark(c, 10));
 }
 
 template <int DIM, int KT, int MODE>
@@ -1090,7 +1104,7 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   // loop 2's error word and k_post1's h_max stripes: cleared by this step's k_prep
   a.out = F; a.energy_part = c->energy_part; a.err = step_err(c) + kErrLoop2; a.b = c->box; a.p = P; a.n = n;
   a.npad = (int)c->npad;
-  HIPCHK(hipEventRecord(c->ev[5], c->stream));
+  HIPCHK(mark(c, 5));
   // fab with one tensor for every pair and no velocity term: the loop
   // without run-time choices
   const bool fab = !generic && P.ddw == 400 && q1 == nullptr && q4 == nullptr;
@@ -1105,34 +1119,32 @@ static int run_post_and_loop2(sph_ctx *c, const Phys &P) {
   else if (generic) launch_loop2_d<MODE_GENERIC>(c, P.dim, &a);
   else if (fab) launch_loop2_d<MODE_FAB>(c, P.dim, &a);
   else launch_loop2_d<MODE_DIFF>(c, P.dim, &a);
-  HIPCHK(hipEventRecord(c->ev[6], c->stream));
-  // energy partials and error bits come back with the step's closing sync
-  // energy, error bits and h_max in one record, one small copy
+  HIPCHK(mark(c, 6));
+  // energy, error bits and h_max in one record of d_stripes, which the
+  // caller's copy_record brings back with the step's closing sync
   hipLaunchKernelGGL(k_step_close, dim3(1), dim3(256), 0, c->stream, c->energy_part, (int)G,
-                     step_err(c) + kErrLoop2, c->d_stripes + kHmaxLL, c->d_red + kCloseRed);
-  HIPCHK(hipMemcpyAsync(c->h_ll + kCloseLL, c->d_red + kCloseRed, sizeof(long long) * 3, hipMemcpyDeviceToHost,
-                        c->stream));
+                     step_err(c) + kErrLoop2, c->d_stripes + kHmaxLL, c->d_stripes + kCloseRec);
   c->loop2_pending = true;
   return SPH_OK;
 }
 
-// After the step's closing sync: loop 2's energy sum (block partials summed
-// in block order) and its error bits.
+// After the step's closing copy_record and sync: loop 2's energy sum (block
+// partials summed in block order) and its error bits.
 static int finish_loop2(sph_ctx *c) {
   if (!c->loop2_pending) return SPH_OK;
   c->loop2_pending = false;
   double e;
-  std::memcpy(&e, c->h_ll + kCloseLL, sizeof e);
+  std::memcpy(&e, c->h_rec + kCloseRec, sizeof e);
   c->stats.energy = e;
   // h_max after the step's own h update: the next step's grid needs no
   // bounding-box pass while positions and h change no other way
-  const double hm = ord2d(c->h_ll[kCloseLL + 2]);
+  const double hm = ord2d(c->h_rec[kCloseRec + 2]);
   if (hm > 0.0 && std::isfinite(hm)) {
     c->next_hmax = hm;
     c->hmax_version = c->h_version;
   }
   c->stats.ms_k_loop2 = elapsed(c, c->ev[5], c->ev[6]);
-  return map_err((int)c->h_ll[kCloseLL + 1]);
+  return map_err((int)c->h_rec[kCloseRec + 1]);
 }
 
 static Phys call_phys(const sph_ctx *c, const double *state) {
@@ -1149,16 +1161,14 @@ static Phys call_phys(const sph_ctx *c, const double *state) {
 // The tail of a step after loop 1: Newton bookkeeping, EOS and packs
 // (k_post1), loop 2, mti gravity and the closing record's copy.
 static int enqueue_tail(sph_ctx *c, const Phys &P) {
-  HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(mark(c, 2));
   int rc = c->opt.precision == 1 ? run_post_and_loop2<float4>(c, P) : run_post_and_loop2<double4>(c, P);
   if (rc == SPH_OK && c->gravity)
     hipLaunchKernelGGL(k_gravity, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->fields(), c->real, (int)c->n,
                        c->gravity);
-  HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(mark(c, 3));
   return rc;
 }
-
-constexpr int kErrCopyLL = 1700;  // h_ll slots of the error words' read-back
 
 // c1advanced steps: the search (its read-back left pending), a batch of
 // sweeps, the tail, then one synchronisation.  Only when k_prep's v / B
@@ -1173,12 +1183,17 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
   long long pairs = 0;
   float ms1 = 0.f;
   if (c->n_real <= 0) {  // no sweep, no active particle (circuit1.f90:114)
+    if (c->neigh_pending) HIPCHK(copy_record(c, 0));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->neigh_pending) { c->neigh_pending = false; RCHK(take_neighbours(c)); }
+    const int fin = take_neighbours(c);
+    if (fin == kRedoNeigh) RCHK(build_neighbours(c));
+    else RCHK(fin);
     c->stats.sweeps = 0;
     c->stats.loop1_pairs = 0;
     const int rc = enqueue_tail(c, P);
-    HIPCHK(wait_event(c->ev[3]));
+    HIPCHK(copy_record(c, 0));
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
     const int rc2 = finish_loop2(c);
     return rc != SPH_OK ? rc : rc2;
   }
@@ -1186,6 +1201,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
     const int last = std::min(maxsw, first + std::max(1, c->sweep_hint) - 1);
     launch_sweeps(c, P, first, last);
     if (c->neigh_pending && c->flags_pending) {
+      HIPCHK(copy_record(c, 0));
       HIPCHK(hipEventRecord(c->ev[8], c->stream));
       HIPCHK(wait_event(c->ev[8]));
       c->neigh_pending = false;
@@ -1198,11 +1214,9 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
       RCHK(fin);
     }
     const int rc = enqueue_tail(c, P);
-    const size_t nreg = (size_t)(last - first + 1) * kSweepRegion;
-    HIPCHK(hipMemcpyAsync(c->h_sweep + (size_t)first * kSweepRegion, c->d_sweep + (size_t)first * kSweepRegion,
-                          sizeof(long long) * nreg, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_ll + kErrCopyLL, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost,
-                          c->stream));
+    // the search's read-back if still pending, error words, the tail's record
+    // and the sweep regions: one copy
+    HIPCHK(copy_record(c, last));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     HIPCHK(wait_event(c->ev[8]));
     ms1 += elapsed(c, c->ev[9], c->ev[10]);
@@ -1223,7 +1237,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
       sweeps = it;
       pairs += stripe_sum(r, 1);
     }
-    bits = copied_err(c->h_ll + kErrCopyLL, kErrLL, kErrLoop1);
+    bits = copied_err(c->h_rec, 0, kErrLoop1);
     if (bits) break;  // circuit1.f90:202-204 stops inside the sweep
     if (sweeps == last && last < maxsw && stripe_sum(c->h_sweep + (size_t)last * kSweepRegion, 0) > 0) {
       first = last + 1;  // still active: more sweeps, then the tail again
@@ -1254,13 +1268,13 @@ static int run_step(sph_ctx *c, const double *state) {
   c->derivs_ready = false;
   Phys P = call_phys(c, state);
   c->stats.ms_k_neigh = c->stats.ms_k_loop1 = c->stats.ms_k_loop2 = 0.0;
-  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(mark(c, 0));
   // the search's read-back is taken with the step's closing synchronisation
   // (c1advanced); c1simple writes h, rho and om in place, so its search is
   // checked before it runs
   RCHK(build_neighbours(c, P.adden != 0));
   c->hmax_version = 0;  // h changes from here; this step's k_post1 reports the new h_max
-  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(mark(c, 1));
   int rc;
   if (P.adden) {
     rc = run_advanced(c, P);
@@ -1269,15 +1283,19 @@ static int run_step(sph_ctx *c, const double *state) {
     c->neigh_pending = false;
     RCHK(r1);
     rc = enqueue_tail(c, P);
-    HIPCHK(wait_event(c->ev[3]));
+    HIPCHK(copy_record(c, 0));
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(wait_event(c->ev[8]));
     const int rc2 = finish_loop2(c);
     if (rc == SPH_OK) rc = rc2;
   }
   c->neigh_pending = false;
   float t01 = 0, t12 = 0, t23 = 0;
-  tolerate(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
-  tolerate(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
-  tolerate(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+  if (c->timing) {
+    tolerate(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+    tolerate(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+    tolerate(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+  }
   c->stats.ms_neigh = t01;
   c->stats.ms_loop1 = t12;
   c->stats.ms_loop2 = t23;
@@ -1443,14 +1461,15 @@ int sph_ctx_create(const double state[49], int kernel_id, const sph_options *opt
   if (c->n_real < 0) { delete c; return SPH_ERR_BAD_ARG; }
   if (dev_alloc((void **)&c->d_red, 16 * kStripes * sizeof(long long)) != SPH_OK ||
       dev_alloc((void **)&c->d_int, 256 * sizeof(int)) != SPH_OK ||
-      dev_alloc((void **)&c->d_stripes, kStepLL * sizeof(long long)) != SPH_OK ||
+      dev_alloc((void **)&c->d_stripes, kRecLL * sizeof(long long)) != SPH_OK ||
       hipHostMalloc((void **)&c->h_int, 256 * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_ll, 2048 * sizeof(long long)) != hipSuccess ||
-      dev_alloc((void **)&c->d_sweep, kSweepRegions * kSweepRegion * sizeof(long long)) != SPH_OK ||
-      hipHostMalloc((void **)&c->h_sweep, kSweepRegions * kSweepRegion * sizeof(long long)) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_rec, kRecLL * sizeof(long long)) != hipSuccess) {
     sph_ctx_destroy(c);
     return SPH_ERR_NOMEM;
   }
+  c->d_sweep = c->d_stripes + kStepLL;
+  c->h_sweep = c->h_rec + kStepLL;
   if (c->opt.max_sweeps > 240) c->opt.max_sweeps = 240;
   *out = c;
   return SPH_OK;
@@ -1472,8 +1491,7 @@ int sph_ctx_destroy(sph_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->h_int) (void)hipHostFree(c->h_int);
   if (c->h_ll) (void)hipHostFree(c->h_ll);
-  if (c->h_sweep) (void)hipHostFree(c->h_sweep);
-  if (c->d_sweep) (void)hipFree(c->d_sweep);
+  if (c->h_rec) (void)hipHostFree(c->h_rec);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1800,6 +1818,12 @@ int sph_sts(sph_ctx *c, const double state[49], int s, double dt) {
   HIPCHK(hipMemcpyAsync(T(), c->aux(5), sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->naux = 0;
+  return SPH_OK;
+}
+
+int sph_set_timing(sph_ctx *c, int on) {
+  if (!c || (on != 0 && on != 1)) return SPH_ERR_BAD_ARG;
+  c->timing = on != 0;
   return SPH_OK;
 }
 

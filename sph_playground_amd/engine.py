@@ -147,6 +147,12 @@ class Engine:
                                j.shape[0], ct.byref(npairs)), "sph_neighbours")
         return off, j[:npairs.value], code[:npairs.value]
 
+    def set_timing(self, on=True):
+        """Record device times of the phases and kernels of later steps (the
+        stats' ms_* fields, 0 while off, the default); the events cost each
+        step a few microseconds per kernel boundary."""
+        check(lib().sph_set_timing(self._h, 1 if on else 0), "sph_set_timing")
+
     def stats(self):
         s = Stats()
         check(lib().sph_get_stats(self._h, ct.byref(s)), "sph_get_stats")

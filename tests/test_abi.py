@@ -100,3 +100,5 @@ def test_bad_arguments_rejected():
         L.sph_default_options(ct.byref(o))
         o.neigh_cap = cap
         assert L.sph_ctx_create(s.ctypes.data_as(ct.POINTER(ct.c_double)), 1, ct.byref(o), ct.byref(h)) == 7
+    # timing needs a context
+    assert L.sph_set_timing(None, 1) == 7

@@ -268,6 +268,30 @@ def test_kept_cell_keys_vs_oracle(gpu):
     assert max(errs.values()) <= FP64_TOL, errs
 
 
+@pytest.mark.parametrize("adden", [True, False], ids=["c1advanced", "c1simple"])
+def test_timing_events_change_nothing(gpu, adden):
+    # sph_set_timing only adds event records around the kernels: the stores
+    # of two resident steps are bit for bit those of a context without them,
+    # and only the timed context reports device times
+    state, st = P.diffusion3d(12, ddw="fab", b=(1.0, 0.3, 0.2), h_scale=1.03)
+    if not adden:
+        state[C.ec_adden - 1] = C.eif_no
+    outs, stats = [], []
+    for timed in (False, True):
+        eng = _engine(state, 1, tensor="full", kappa_perp=1e-2)
+        eng.set_timing(timed)
+        eng.load(st)
+        eng.step()
+        eng.step()
+        outs.append(eng.save())
+        stats.append(eng.stats())
+        eng.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert stats[0]["energy"] == stats[1]["energy"] and stats[0]["n_pairs"] == stats[1]["n_pairs"]
+    assert stats[0]["ms_total"] == 0.0 and stats[0]["ms_k_neigh"] == 0.0
+    assert stats[1]["ms_total"] > 0.0 and stats[1]["ms_k_neigh"] > 0.0 and stats[1]["ms_k_loop2"] > 0.0
+
+
 def test_device_store_path(gpu):
     import torch
     state, st = P.hydro2d(32)
