@@ -32,7 +32,7 @@ __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
 __global__ void k_prep(Fields, const int64_t *, int, int64_t, Box, Grid, int, double4 *, double *, float4 *,
                        uint8_t *, double4 *, double *, uint3 *, FixFrame, float2 *, double *, long long *,
-                       long long *, int, int);
+                       long long *, int, long long *, int, int);
 template <int DIM, bool EXT>
 __global__ void k_neigh(NeighArgs, const double4 *__restrict__, const float4 *__restrict__, uint32_t *__restrict__,
                         int *__restrict__);
@@ -254,6 +254,7 @@ struct sph_ctx {
   // parts of d_stripes and h_rec from kStepLL on
   long long *d_sweep = nullptr, *h_sweep = nullptr;
   int sweep_hint = 4;             // sweeps the next step launches before it synchronises
+  int sweeps_zeroed = 0;          // k_prep cleared the regions of sweeps 1..this for the next batch
   sph_stats stats{};
   bool have_step = false;
   // the resident store holds the derivatives of the last successful iterate
@@ -565,6 +566,11 @@ static hipError_t copy_record(sph_ctx *c, int last_sweep) {
   const size_t ll = kStepLL + (last_sweep > 0 ? (size_t)(last_sweep + 1) * kSweepRegion : 0);
   return hipMemcpyAsync(c->h_rec, c->d_stripes, sizeof(long long) * ll, hipMemcpyDeviceToHost, c->stream);
 }
+
+// The Newton sweeps cap, and the last sweep of a step's first batch (the
+// sweeps launched before its first synchronisation).
+static int sweep_cap(const sph_ctx *c) { return c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100; }
+static int first_batch_last(const sph_ctx *c) { return std::min(sweep_cap(c), std::max(1, c->sweep_hint)); }
 constexpr int kWhichInt = 6;        // d_int slot: parity of the last sweep that ran
 constexpr int kExtTotalInt = 16;    // h_int slot: records of the extended grid
 constexpr int kRedoNeigh = 1000; // internal: the lists overflowed, search again (not an SPH_ERR code)
@@ -775,10 +781,15 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   c->ff = ff;
   // the start-state copy only for c1simple (a deferred search is
   // c1advanced's, whose first sweep reads the fields)
+  // with the counter regions of the first batch of Newton sweeps when they
+  // follow this search (c1advanced)
+  const int zlast = defer ? first_batch_last(c) : 0;
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, defer ? nullptr : c->st[0], c->tr[0], c->pfix, ff, c->s1[0],
                      c->eps,
-                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL, pos_kept ? 0 : 1);
+                     c->flags_pending ? c->d_stripes + kFlagSlots : nullptr, c->d_stripes, kZeroLL,
+                     c->d_sweep + kSweepRegion, zlast * kSweepRegion, pos_kept ? 0 : 1);
+  c->sweeps_zeroed = zlast;
   c->prep_version = c->pos_version;
   c->prep_cur = c->cur;
   c->prep_n = n;
@@ -964,7 +975,7 @@ static int run_loop1_simple(sph_ctx *c, const Phys &P) {
 // no host round trip: each sweep runs only if the one before left a particle
 // active (while maxval(eps) > tol, circuit1.f90:114) and no error stopped the
 // search or loop 1 (k_loop1 decides on the device); their counters go to
-// the sweep regions, zeroed here first.
+// the sweep regions, zeroed here first (by k_prep for a step's first batch).
 static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
   const int n = (int)c->n;
   Fields F = c->fields();
@@ -977,8 +988,10 @@ static void launch_sweeps(sph_ctx *c, const Phys &P, int first, int last) {
   a.which = c->d_int + kWhichInt;
   const int grad = (P.diff || P.fld) ? (P.ddw == 404 ? 2 : 1) : 0;
   const int nreg = (last - first + 1) * kSweepRegion;
-  hipLaunchKernelGGL(k_zero_ll, dim3((nreg + 255) / 256), dim3(256), 0, c->stream,
-                     c->d_sweep + (size_t)first * kSweepRegion, nreg);
+  if (!(first == 1 && last <= c->sweeps_zeroed))
+    hipLaunchKernelGGL(k_zero_ll, dim3((nreg + 255) / 256), dim3(256), 0, c->stream,
+                       c->d_sweep + (size_t)first * kSweepRegion, nreg);
+  c->sweeps_zeroed = 0;
   tolerate(mark(c, 9));
   for (int it = first; it <= last; it++) {
     a.cur = c->st[(it - 1) & 1];
@@ -1178,7 +1191,7 @@ static int enqueue_tail(sph_ctx *c, const Phys &P) {
 // step searches again with room and starts over; a batch that ends with
 // particles still active is followed by more sweeps and the tail again.
 static int run_advanced(sph_ctx *c, const Phys &P) {
-  const int maxsw = c->opt.max_sweeps > 0 ? c->opt.max_sweeps : 100;
+  const int maxsw = sweep_cap(c);
   int first = 1, sweeps = 0, bits = 0;
   long long pairs = 0;
   float ms1 = 0.f;

@@ -326,14 +326,18 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
 // particle 0's (loop 2 skips terms that are exactly zero otherwise; neither
 // changes before loop 2).
 // zero[0, nzero): the step's counters, cleared by the first block before the
-// kernels that count into them run (they all follow k_prep on its stream).
+// kernels that count into them run (they all follow k_prep on its stream);
+// zero2[0, nzero2): the counter regions of the step's first Newton sweeps,
+// cleared by the whole grid.
 __global__ void k_prep(Fields F, const int64_t *orig, int n, int64_t n_real, Box b, Grid g, int dim, double4 *pm,
                        double *h0, float4 *pf, uint8_t *real, double4 *st, double *tr, uint3 *pfix, FixFrame ff,
-                       float2 *s1, double *eps, long long *flags, long long *zero, int nzero, int pos) {
+                       float2 *s1, double *eps, long long *flags, long long *zero, int nzero, long long *zero2,
+                       int nzero2, int pos) {
   __shared__ long long lds[4];
   int s = gtid();
   if (blockIdx.x == 0)
     for (int k = threadIdx.x; k < nzero; k += blockDim.x) zero[k] = 0;
+  for (int k = s; k < nzero2; k += gridDim.x * blockDim.x) zero2[k] = 0;
   int fl = 0;
   if (s < n) {
     if (flags) {  // null: the caller still holds the flags of unchanged v and B
