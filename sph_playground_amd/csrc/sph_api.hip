@@ -261,8 +261,8 @@ struct sph_ctx {
   // of this state (nothing loaded since): a leapfrog pass then starts at its
   // predictor, as every pass after main.f90:219's initial iterate does
   bool derivs_ready = false;
-  hipEvent_t ev[11]{};            // 8: the short waits of read_ints; 9, 10: a batch of Newton sweeps
-  bool timing = false;            // record the phase and kernel events (all but 8) for the stats' times
+  hipEvent_t ev[11]{};            // the stats' phase and kernel times (8 unused; 9, 10: a batch of sweeps)
+  bool timing = false;            // record them (sph_set_timing)
   // differentiable diffusion (sph_diff_*)
   bool diff_ready = false;
   Phys diff_phys{};
@@ -471,19 +471,20 @@ static int map_err(int bits) {
   return SPH_OK;
 }
 
-// The short waits inside a step poll their event: the runtime's blocking
-// wait adds its wake-up to every host round trip of the step.
-static hipError_t wait_event(hipEvent_t e) {
+// The short waits inside a step poll the stream until its work is done:
+// the runtime's blocking wait adds its wake-up to every host round trip of
+// the step, and an event recorded to wait on adds the processing of its own
+// marker (a few microseconds after the last copy).
+static hipError_t wait_stream(sph_ctx *c) {
   hipError_t r;
-  while ((r = hipEventQuery(e)) == hipErrorNotReady) {
+  while ((r = hipStreamQuery(c->stream)) == hipErrorNotReady) {
   }
   return r;
 }
 
 static int read_ints(sph_ctx *c, int first, int count) {
   HIPCHK(hipMemcpyAsync(c->h_int + first, c->d_int + first, sizeof(int) * count, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipEventRecord(c->ev[8], c->stream));
-  HIPCHK(wait_event(c->ev[8]));
+  HIPCHK(wait_stream(c));
   return SPH_OK;
 }
 
@@ -837,8 +838,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       tb = c->ext_tmp_bytes;
       HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->ext_tmp, tb, c->ext_count, c->ecs, E + 1, c->stream));
       HIPCHK(hipMemcpyAsync(c->h_int + kExtTotalInt, c->ecs + E, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipEventRecord(c->ev[8], c->stream));
-      HIPCHK(wait_event(c->ev[8]));
+      HIPCHK(wait_stream(c));
       int64_t total = c->h_int[kExtTotalInt];
       // 8 records of padding: the search reads whole batches past a row's
       // end.  The search addresses records with 32-bit byte offsets, so a
@@ -910,8 +910,7 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
       return SPH_OK;
     }
     HIPCHK(copy_record(c, 0));
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
+    HIPCHK(wait_stream(c));
     const long long *ll = c->h_rec;
     const int fin = finish_neighbours(c, ll, copied_err(ll, 0, kErrSearch));
     if (fin != kRedoNeigh) return fin;
@@ -963,8 +962,7 @@ static int run_loop1_simple(sph_ctx *c, const Phys &P) {
   c->final_st = c->st[0];
   c->stats.sweeps = 0;
   HIPCHK(hipMemcpyAsync(c->h_ll, c->d_stripes + kErrLL, sizeof(long long) * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipEventRecord(c->ev[8], c->stream));
-  HIPCHK(wait_event(c->ev[8]));
+  HIPCHK(wait_stream(c));
   RCHK(take_neighbours(c));
   c->stats.ms_k_loop1 = elapsed(c, c->ev[5], c->ev[6]);
   c->stats.loop1_pairs = c->stats.n_pairs;
@@ -1205,8 +1203,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
     c->stats.loop1_pairs = 0;
     const int rc = enqueue_tail(c, P);
     HIPCHK(copy_record(c, 0));
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
+    HIPCHK(wait_stream(c));
     const int rc2 = finish_loop2(c);
     return rc != SPH_OK ? rc : rc2;
   }
@@ -1215,8 +1212,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
     launch_sweeps(c, P, first, last);
     if (c->neigh_pending && c->flags_pending) {
       HIPCHK(copy_record(c, 0));
-      HIPCHK(hipEventRecord(c->ev[8], c->stream));
-      HIPCHK(wait_event(c->ev[8]));
+      HIPCHK(wait_stream(c));
       c->neigh_pending = false;
       const int fin = take_neighbours_now(c);
       if (fin == kRedoNeigh) {
@@ -1230,8 +1226,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
     // the search's read-back if still pending, error words, the tail's record
     // and the sweep regions: one copy
     HIPCHK(copy_record(c, last));
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
+    HIPCHK(wait_stream(c));
     ms1 += elapsed(c, c->ev[9], c->ev[10]);
     if (rc != SPH_OK) return rc;
     if (c->neigh_pending) {
@@ -1297,8 +1292,7 @@ static int run_step(sph_ctx *c, const double *state) {
     RCHK(r1);
     rc = enqueue_tail(c, P);
     HIPCHK(copy_record(c, 0));
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(wait_event(c->ev[8]));
+    HIPCHK(wait_stream(c));
     const int rc2 = finish_loop2(c);
     if (rc == SPH_OK) rc = rc2;
   }
