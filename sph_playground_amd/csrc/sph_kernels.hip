@@ -1081,13 +1081,18 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint
   const double4 *fin = (sel.which && *sel.which) ? sel.fin1 : sel.fin0;
   int s = gtid();
   int fl = 0;
+  // h and den as written here (no reload of the fields just stored)
+  double h = 0.0;
   if (s < n) {
     bool r = real[s];
+    double den, om;
     if (r && advanced) {
-      double4 v = fin[s];
+      const double4 v = fin[s];
       F.f[F_H][s] = v.x; F.f[F_DEN][s] = v.y; F.f[F_OM][s] = v.z;
+      h = v.x; den = v.y; om = v.z;
+    } else {
+      h = F.f[F_H][s]; den = F.f[F_DEN][s]; om = F.f[F_OM][s];
     }
-    double den = F.f[F_DEN][s];
     if (r && do_eos) {
       if (iso) {
         F.f[F_P][s] = den * c1 * c1;
@@ -1105,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint
       if (vx != 0.0 || vy != 0.0 || vz != 0.0) fl |= 1;
       if (!(bx == F.f[F_BX][0] && by == F.f[F_BY][0] && bz == F.f[F_BZ][0])) fl |= 2;
     }
-    if (q0) q0[s] = V4{(T)(1.0 / F.f[F_H][s]), (T)den, (T)F.f[F_OM][s], (T)F.f[F_T][s]};
+    if (q0) q0[s] = V4{(T)(1.0 / h), (T)den, (T)om, (T)F.f[F_T][s]};
     if (q1) q1[s] = V4{(T)vx, (T)vy, (T)vz, (T)F.f[F_P][s]};
     if (q2) q2[s] = V4{(T)F.f[F_C][s], (T)F.f[F_U][s], (T)F.f[F_KAPPA][s], (T)0};
     if (q3) q3[s] = V4{(T)F.f[F_DTDX][s], (T)F.f[F_DTDY][s], (T)F.f[F_DTDZ][s], (T)0};
@@ -1125,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_post1(Fields F, PostSel sel, const uint
   // the next step's grid when nothing else changes h or the positions
   if (hmax_red) {
     long long hm = -0x7FFFFFFFFFFFFFFFLL - 1;
-    if (s < n && real[s]) hm = d2ord(F.f[F_H][s]);
+    if (s < n && real[s]) hm = d2ord(h);
     const long long t = block_reduce<R_MAX>(hm, lds);
     if (threadIdx.x == 0) atomicMax(&hmax_red[stripe()], t);
   }
