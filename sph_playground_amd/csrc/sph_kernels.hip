@@ -1538,7 +1538,15 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)Fi[F_DEN][s], oma = (PT)Fi[F_OM][s], ta = (PT)Fi[F_T][s];
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);
-    const Sym3<PT> kt = tensor_of_b<PT>(P, (PT)Fi[F_BX][s], (PT)Fi[F_BY][s], (PT)Fi[F_BZ][s]);
+    // one B for every particle (every component equal to particle 0's):
+    // particle 0's, through scalar loads
+    double bxa, bya, bza;
+    if (a.b_uniform) {
+      bxa = Fi[F_BX][0]; bya = Fi[F_BY][0]; bza = Fi[F_BZ][0];
+    } else {
+      bxa = Fi[F_BX][s]; bya = Fi[F_BY][s]; bza = Fi[F_BZ][s];
+    }
+    const Sym3<PT> kt = tensor_of_b<PT>(P, (PT)bxa, (PT)bya, (PT)bza);
     const PT tra = ta * rhoa;
     // (dim + 2) k with the off-diagonal terms doubled, and the trace
     constexpr PT D2 = PT(DIM + 2);
@@ -1628,9 +1636,17 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     a.out.f[F_DDT][s] = (double)ddt;
     a.out.f[F_DBX][s] = 0.0; a.out.f[F_DBY][s] = 0.0; a.out.f[F_DBZ][s] = 0.0;
     a.out.f[F_DU][s] = 0.0;
-    // FP64 energy sum (circuit2.f90:490-492) with zero dv, du, dB, drho
-    double vx = Fi[F_VX][s], vy = Fi[F_VY][s], vz = Fi[F_VZ][s];
-    double bx = Fi[F_BX][s], by = Fi[F_BY][s], bz = Fi[F_BZ][s], rh = Fi[F_DEN][s];
+    // FP64 energy sum (circuit2.f90:490-492) with zero dv, du, dB, drho.
+    // Every velocity exactly 0 (a signed zero among them changes no term
+    // after the + 0 below) or one B for every particle: no per-particle loads
+    double vx = 0.0, vy = 0.0, vz = 0.0, bx, by, bz;
+    if (a.v_nonzero) { vx = Fi[F_VX][s]; vy = Fi[F_VY][s]; vz = Fi[F_VZ][s]; }
+    if (a.b_uniform) {
+      bx = Fi[F_BX][0]; by = Fi[F_BY][0]; bz = Fi[F_BZ][0];
+    } else {
+      bx = Fi[F_BX][s]; by = Fi[F_BY][s]; bz = Fi[F_BZ][s];
+    }
+    const double rh = Fi[F_DEN][s];
     const double z0 = 0.0;
     e_part = pa.w * (vx * z0 + vy * z0 + vz * z0 + z0 + (bx * z0 + by * z0 + bz * z0) / rh -
                      0.5 * (bx * bx + by * by + bz * bz) / rh / rh * z0 / Fi[F_OM][s]);

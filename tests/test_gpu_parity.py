@@ -268,6 +268,26 @@ def test_kept_cell_keys_vs_oracle(gpu):
     assert max(errs.values()) <= FP64_TOL, errs
 
 
+@pytest.mark.parametrize("case", ["uniform_b_zero_v", "varying_b_moving"])
+def test_fab_loop_per_particle_inputs_vs_oracle(gpu, case):
+    # the fab loop reads B once when every particle carries the same B and
+    # skips the velocities when all are zero; with isotropic conduction and
+    # c1simple it also runs on particles whose B varies and that move: both
+    # against the oracle (fields and the FP64 energy sum)
+    if case == "uniform_b_zero_v":
+        state, st = P.diffusion3d(12, ddw="fab", b=(1.0, 0.3, 0.2), h_scale=1.03)
+        okw = dict(tensor_mode=1, kappa_perp=1e-2)
+    else:
+        state, st = P.diffusion3d(12, ddw="fab", disotropic="yes", adden="no", h_scale=1.03)
+        rng = np.random.default_rng(7)
+        st[:, C.es_bx - 1:C.es_bz] = rng.normal(size=(st.shape[0], 3))
+        st[:, C.es_vx - 1:C.es_vx + 2] = 0.1 * rng.normal(size=(st.shape[0], 3))
+        okw = {}
+    eng, ref, got = _oracle_compare(state, st, kernel=1, **okw)
+    assert eng.stats()["energy"] == pytest.approx(ref["energy"], rel=1e-12, abs=1e-300)
+    eng.close()
+
+
 @pytest.mark.parametrize("adden", [True, False], ids=["c1advanced", "c1simple"])
 def test_timing_events_change_nothing(gpu, adden):
     # sph_set_timing only adds event records around the kernels: the stores
