@@ -970,7 +970,8 @@ __global__ __launch_bounds__(256) void k_loop1(Loop1Args<double4> a) {
     double hn = ha - fh / dfdh;
     if (hn <= 0.) atomicOr(a.err, EB_NEG_H);
     if (qbad) atomicOr(a.err, EB_Q);
-    double ep = fabs(hn - ha) / a.h0[s];
+    // h at entry: the first sweep starts from it
+    double ep = fabs(hn - ha) / (first ? ha : a.h0[s]);
     a.eps[s] = ep;
     still = ep > a.tol;
     a.next[s] = make_double4(hn, den, osm, ta);
@@ -1534,8 +1535,8 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     const Phys &P = a.p;
     const double4 pa = a.pm[s];
     const double *const *Fi = a.out.f;
-    const double ha_d = Fi[F_H][s];
-    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)Fi[F_DEN][s], oma = (PT)Fi[F_OM][s], ta = (PT)Fi[F_T][s];
+    const double ha_d = Fi[F_H][s], rho_d = Fi[F_DEN][s], om_d = Fi[F_OM][s];
+    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)rho_d, oma = (PT)om_d, ta = (PT)Fi[F_T][s];
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);
     // one B for every particle (every component equal to particle 0's):
@@ -1646,10 +1647,10 @@ __global__ __launch_bounds__(256) void k_loop2_fab(Loop2Args<V4> a) {
     } else {
       bx = Fi[F_BX][s]; by = Fi[F_BY][s]; bz = Fi[F_BZ][s];
     }
-    const double rh = Fi[F_DEN][s];
+    const double rh = rho_d;
     const double z0 = 0.0;
     e_part = pa.w * (vx * z0 + vy * z0 + vz * z0 + z0 + (bx * z0 + by * z0 + bz * z0) / rh -
-                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * z0 / Fi[F_OM][s]);
+                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * z0 / om_d);
   }
   __shared__ double red[4];
   double w = wave_sum(e_part);
