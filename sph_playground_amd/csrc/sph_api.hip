@@ -781,9 +781,9 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
                         std::memcmp(&ff, &c->ff, sizeof(ff)) == 0;
   c->ff = ff;
   // the start-state copy only for c1simple (a deferred search is
-  // c1advanced's, whose first sweep reads the fields)
-  // with the counter regions of the first batch of Newton sweeps when they
-  // follow this search (c1advanced)
+  // c1advanced's, whose first sweep reads the fields); the counters cleared
+  // with the regions of the first batch of Newton sweeps when they follow
+  // this search (c1advanced)
   const int zlast = defer ? first_batch_last(c) : 0;
   hipLaunchKernelGGL(k_prep, dim3(G), dim3(kBlock), 0, c->stream, F, c->orig[c->cur], n, c->n_real, c->box, g, dim,
                      c->pm, c->h0, c->pf, c->real, defer ? nullptr : c->st[0], c->tr[0], c->pfix, ff, c->s1[0],
