@@ -45,7 +45,9 @@ class Engine:
         opt = Options(_PREC[precision], _TENSOR[tensor], float(kappa_perp), int(max_sweeps),
                       float(newton_tol), int(neigh_cap), int(device), int(cell_div))
         h = ct.c_void_p()
-        check(L.sph_ctx_create(_dptr(self._state), kid, ct.byref(opt), ct.byref(h)), "sph_ctx_create")
+        # the state's pointer, made once (the per-step calls pass it again)
+        self._state_p = _dptr(self._state)
+        check(L.sph_ctx_create(self._state_p, kid, ct.byref(opt), ct.byref(h)), "sph_ctx_create")
         self._h = h
         self.kernel = kid
         self.precision = precision
@@ -93,19 +95,19 @@ class Engine:
         check(lib().sph_save(self._h, ct.c_void_p(ptr), ld, 1), "sph_save")
 
     def step(self, state=None):
-        s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        sp = self._state_p if state is None else _dptr(np.ascontiguousarray(np.asarray(state, dtype=np.float64)))
         self._geometry_gen += 1
-        check(lib().sph_step(self._h, _dptr(s), None), "sph_step")
+        check(lib().sph_step(self._h, sp, None), "sph_step")
 
     def leapfrog(self, dtcap, state=None):
         """One pass of the leapfrog loop (main.f90:241-362) on the resident
         state; returns dt.  The initial iterate (main.f90:219) runs first when
         the state was loaded since the last step, so N calls after ``load``
         are the reference's run of N steps."""
-        s = self._state if state is None else np.ascontiguousarray(np.asarray(state, dtype=np.float64))
+        sp = self._state_p if state is None else _dptr(np.ascontiguousarray(np.asarray(state, dtype=np.float64)))
         dt = ct.c_double(0.0)
         self._geometry_gen += 1
-        check(lib().sph_leapfrog(self._h, _dptr(s), float(dtcap), ct.byref(dt)), "sph_leapfrog")
+        check(lib().sph_leapfrog(self._h, sp, float(dtcap), ct.byref(dt)), "sph_leapfrog")
         return dt.value
 
     PRERUN_MESSAGES = {1: "density <= 0", 2: "csound <= 0", 4: "smoothing length <= 0", 8: "pressure <= 0",
