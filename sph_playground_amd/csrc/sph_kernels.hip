@@ -234,9 +234,18 @@ __global__ void k_reorder(Fields src, Fields dst, const uint32_t *perm, const in
   int s = gtid();
   if (s >= n) return;
   uint32_t p = perm[s];
-#pragma unroll 4
-  for (int f = 0; f < F_TOTAL; f++)
-    if (f < nf) dst.f[f][s] = src.f[f][p];
+  // 16 columns' loads issued before their stores (a store may alias a later
+  // column's load for the compiler, which would otherwise wait on each load)
+  constexpr int kGroup = 16;
+#pragma unroll
+  for (int f0 = 0; f0 < F_TOTAL; f0 += kGroup) {
+    double v[kGroup];
+#pragma unroll
+    for (int k = 0; k < kGroup; k++) v[k] = f0 + k < nf ? src.f[f0 + k][p] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kGroup; k++)
+      if (f0 + k < nf) dst.f[f0 + k][s] = v[k];
+  }
   if (orig_dst) orig_dst[s] = orig_src[p];
 }
 
