@@ -1226,11 +1226,15 @@ void k_loop2(Loop2Args<V4> a) {
     // i-side values straight from the FP64 fields (coalesced, read once)
     const double *const *Fi = a.out.f;
     const double ha_d = Fi[F_H][s];
-    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)Fi[F_DEN][s], oma = (PT)Fi[F_OM][s], pa_ = (PT)Fi[F_P][s];
-    const PT vax = (PT)Fi[F_VX][s], vay = (PT)Fi[F_VY][s], vaz = (PT)Fi[F_VZ][s], ca = (PT)Fi[F_C][s];
+    // den, om, v and B in FP64 as read, for the energy term after the loop
+    const double rho_d = Fi[F_DEN][s], om_d = Fi[F_OM][s];
+    const double vx_d = Fi[F_VX][s], vy_d = Fi[F_VY][s], vz_d = Fi[F_VZ][s];
+    const double bx_d = Fi[F_BX][s], by_d = Fi[F_BY][s], bz_d = Fi[F_BZ][s];
+    const PT inv_ha = (PT)(1.0 / ha_d), rhoa = (PT)rho_d, oma = (PT)om_d, pa_ = (PT)Fi[F_P][s];
+    const PT vax = (PT)vx_d, vay = (PT)vy_d, vaz = (PT)vz_d, ca = (PT)Fi[F_C][s];
     const PT ua = (PT)Fi[F_U][s], ta = (PT)Fi[F_T][s], kappaa = (PT)Fi[F_KAPPA][s];
     const PT gax = (PT)Fi[F_DTDX][s], gay = (PT)Fi[F_DTDY][s], gaz = (PT)Fi[F_DTDZ][s];
-    const PT bax = (PT)Fi[F_BX][s], bay = (PT)Fi[F_BY][s], baz = (PT)Fi[F_BZ][s];
+    const PT bax = (PT)bx_d, bay = (PT)by_d, baz = (PT)bz_d;
     // i-side constants
     const PT wcv = (PT)P.wcv;
     const PT ihd1 = ipow<DIM + 1>(inv_ha);  // 1/h_a^(dim+1)
@@ -1506,11 +1510,10 @@ void k_loop2(Loop2Args<V4> a) {
     a.out.f[F_DBX][s] = (double)dbx; a.out.f[F_DBY][s] = (double)dby; a.out.f[F_DBZ][s] = (double)dbz;
     a.out.f[F_DU][s] = (double)du;
     // FP64 energy sum (circuit2.f90:490-492)
-    double vx = Fi[F_VX][s], vy = Fi[F_VY][s], vz = Fi[F_VZ][s];
-    double bx = Fi[F_BX][s], by = Fi[F_BY][s], bz = Fi[F_BZ][s], rh = Fi[F_DEN][s];
+    const double vx = vx_d, vy = vy_d, vz = vz_d, bx = bx_d, by = by_d, bz = bz_d, rh = rho_d;
     e_part = pa.w * (vx * (double)dvx + vy * (double)dvy + vz * (double)dvz + (double)du +
                      (bx * (double)dbx + by * (double)dby + bz * (double)dbz) / rh -
-                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * (double)drho / Fi[F_OM][s]);
+                     0.5 * (bx * bx + by * by + bz * bz) / rh / rh * (double)drho / om_d);
   }
   __shared__ double red[4];
   double w = wave_sum(e_part);
