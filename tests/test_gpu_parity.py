@@ -310,6 +310,11 @@ def test_timing_events_change_nothing(gpu, adden):
     assert stats[0]["energy"] == stats[1]["energy"] and stats[0]["n_pairs"] == stats[1]["n_pairs"]
     assert stats[0]["ms_total"] == 0.0 and stats[0]["ms_k_neigh"] == 0.0
     assert stats[1]["ms_total"] > 0.0 and stats[1]["ms_k_neigh"] > 0.0 and stats[1]["ms_k_loop2"] > 0.0
+    # only 0 and 1 are accepted
+    from sph_playground_amd import _lib
+    eng = _engine(state, 1)
+    assert _lib.lib().sph_set_timing(eng._h, 2) == 7
+    eng.close()
 
 
 def test_device_store_path(gpu):
