@@ -292,7 +292,6 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
                                          int pos) {
   double h = F.f[F_H][s];
   h0[s] = h;
-  bool r = orig[s] < n_real;
   // the start state for c1simple (c1advanced's first sweep reads the fields)
   if (st) st[s] = make_double4(h, F.f[F_DEN][s], F.f[F_OM][s], F.f[F_T][s]);
   (void)eps;
@@ -300,6 +299,7 @@ __device__ __forceinline__ void prep_one(Fields F, const int64_t *orig, int s, i
   if (s1) s1[s] = make_float2((float)F.f[F_M][s], (float)(F.f[F_T][s] * F.f[F_DEN][s]));
   else tr[s] = F.f[F_T][s] * F.f[F_DEN][s];
   if (!pos) return;
+  const bool r = orig[s] < n_real;
   double x = F.f[F_RX][s], y = F.f[F_RY][s], z = F.f[F_RZ][s];
   pm[s] = make_double4(x, y, z, F.f[F_M][s]);
   real[s] = r ? 1 : 0;
