@@ -1181,10 +1181,26 @@ static int enqueue_tail(sph_ctx *c, const Phys &P) {
   return rc;
 }
 
+// Whether loop 2's packs or terms depend on k_prep's v / B flags (the
+// "every velocity 0" and "one B for every particle" bits): the diffusion-only
+// loops skip v and B by them; the general loop reads v for every pair unless
+// no term needs it always (hydro, MHD, artificial conductivity), and B by
+// them only for a diffusion tensor from B (k_loop2).  Otherwise a step need
+// not wait for the flags before its tail: they come back at its end.
+static bool loop2_reads_flags(const Phys &P) {
+  const bool generic = !(P.diff && !P.hydro && !P.artts && !P.magneto && !P.fld &&
+                         (P.ddw == 400 || P.ddw == 401 || P.ddw == 402));
+  if (!generic) return true;
+  const bool v_always = P.hydro || P.magneto || P.artts;
+  const bool tensor_b = P.diff && !P.fld && P.diso == 0;
+  return !v_always || tensor_b;
+}
+
 // c1advanced steps: the search (its read-back left pending), a batch of
 // sweeps, the tail, then one synchronisation.  Only when k_prep's v / B
 // flags must come back before the tail picks loop 2's packs (after a load or
-// a leapfrog change of v or B) does the step wait after the sweeps.  Lists
+// a leapfrog change of v or B, for a loop 2 that reads them) does the step
+// wait after the sweeps.  Lists
 // that overflowed leave loop 1 and k_post1 idle (device-side checks), so the
 // step searches again with room and starts over; a batch that ends with
 // particles still active is followed by more sweeps and the tail again.
@@ -1210,7 +1226,7 @@ static int run_advanced(sph_ctx *c, const Phys &P) {
   for (;;) {
     const int last = std::min(maxsw, first + std::max(1, c->sweep_hint) - 1);
     launch_sweeps(c, P, first, last);
-    if (c->neigh_pending && c->flags_pending) {
+    if (c->neigh_pending && c->flags_pending && loop2_reads_flags(P)) {
       HIPCHK(copy_record(c, 0));
       HIPCHK(wait_stream(c));
       c->neigh_pending = false;
