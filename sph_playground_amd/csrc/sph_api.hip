@@ -26,7 +26,7 @@ using namespace sph;
 namespace sph {
 // kernels (sph_kernels.hip)
 __global__ void k_wrap_reduce(Fields, const int64_t *, int, int64_t, Box, int, long long *, int *);
-__global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *);
+__global__ void k_keys(Fields, int, Grid, uint32_t *, uint32_t *, int *);
 __global__ void k_check_sorted(const uint32_t *, int, int *);
 __global__ void k_reorder(Fields, Fields, const uint32_t *, const int64_t *, int64_t *, int, int);
 __global__ void k_cell_start(const uint32_t *, int, uint32_t, int *);
@@ -727,8 +727,8 @@ static int build_neighbours(sph_ctx *c, bool defer = false) {
   const bool keys_kept = c->keys_version == c->pos_version && c->keys_n == n && c->keys_nkeys == nkeys &&
                          same_grid(g, c->keys_grid);
   if (!keys_kept) {
-    hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals);
-    HIPCHK(hipMemsetAsync(c->d_int + 2, 0, sizeof(int), c->stream));
+    // k_keys clears the order check's flag (no fill of its own)
+    hipLaunchKernelGGL(k_keys, dim3(G), dim3(kBlock), 0, c->stream, F, n, g, c->keys, c->vals, c->d_int + 2);
     hipLaunchKernelGGL(k_check_sorted, dim3(std::min(G, 256u)), dim3(kBlock), 0, c->stream, c->keys, n,
                        c->d_int + 2);
     RCHK(read_ints(c, 2, 1));

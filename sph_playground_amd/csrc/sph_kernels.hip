@@ -209,8 +209,11 @@ __global__ __launch_bounds__(256) void k_wrap_reduce(Fields F, const int64_t *or
   }
 }
 
-__global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals) {
+// sorted_flag: the order check's flag, cleared here for k_check_sorted
+// (which follows on the stream)
+__global__ void k_keys(Fields F, int n, Grid g, uint32_t *keys, uint32_t *vals, int *sorted_flag) {
   int s = gtid();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sorted_flag = 0;
   if (s >= n) return;
   int c[3];
   cell_of(g, F.f[F_RX][s], F.f[F_RY][s], F.f[F_RZ][s], c);
